@@ -1,0 +1,120 @@
+/*
+ * oracle.h -- CPU restatement of TetWild's mesh-optimisation inner loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path in
+ * tetwild_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links or calls it.
+ *
+ * Arithmetic contract: plain IEEE binary64, built with -ffp-contract=off, no
+ * fast-math.  The only fused multiply-adds are explicit fma() calls inside the
+ * error-free product of the exact predicates, where fusion is the point.
+ *
+ * Pinning status per function (what independent evidence fixes each one):
+ *   orc_amips_*           golden vectors produced by the reference's own
+ *                         comformalAMIPS{Energy,Jacobian,Hessian}_new compiled
+ *                         from src/tetwild/LocalOperations.cpp:28-291
+ *                         (tests/golden/amips_golden.npz), plus 50-digit
+ *                         mpmath closed form and invariants.
+ *   orc_orient3d/collinear  exact; pinned against fractions.Fraction.
+ *   orc_point_tri_d2      Eberly's region method; pinned against an exact
+ *                         rational brute force on small inputs.  Bit-level
+ *                         agreement with geogram's point_triangle_squared_distance
+ *                         (not vendored in the reference): parity unpinned.
+ *   orc_sample_triangle   invariants (vertices present, samples in the
+ *                         triangle, spacing, count formula).  Bit-level
+ *                         agreement with geogram vec3 arithmetic: parity unpinned.
+ *   orc_envelope_*        brute force over all facets (exact-rational decisions
+ *                         away from the eps^2 rounding band).
+ *   orc_newton_*          invariants (monotone energy, no flips, step bounds,
+ *                         is_moved semantics, X = X0 - a H^-1 J).  Bit-level
+ *                         agreement with Eigen's ColPivHouseholderQR:
+ *                         parity unpinned.
+ */
+#ifndef TWG_ORACLE_H
+#define TWG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_ENERGY 1e50 /* State::MAX_ENERGY, src/tetwild/State.h:29 */
+
+/* ---- (a) AMIPS, src/tetwild/LocalOperations.cpp:28-291 ---------------- */
+/* T = 12 doubles v0xyz v1xyz v2xyz v3xyz, as the reference's const double* T */
+double orc_amips_energy(const double* T);
+void orc_amips_jacobian(const double* T, double* g3);
+void orc_amips_hessian(const double* T, double* h9);
+/* E, grad (3) and Hessian (9) in one pass; same bits as the three above */
+double orc_amips_ejh(const double* T, double* g3, double* h9);
+double orc_rcbrt(double x);
+
+/* batch helpers over SoA vertex arrays and int32 tets (n x 4) */
+void orc_amips_batch(const int32_t* tets, int64_t n, const double* x, const double* y,
+                     const double* z, int mode, double* E, double* G, double* H);
+
+/* ---- exact predicates ------------------------------------------------- */
+/* sign of det[q-p; r-p; s-p]: +1 POSITIVE, 0, -1 (CGAL::orientation) */
+int orc_orient3d(const double* p, const double* q, const double* r, const double* s);
+/* 1 if p, q, r are exactly collinear (CGAL::collinear / Triangle_3::is_degenerate) */
+int orc_collinear3(const double* p, const double* q, const double* r);
+
+/* calTetQualities, src/tetwild/LocalOperations.cpp:695-773, 862-884 */
+void orc_tet_qualities(const int32_t* tets, int64_t n, const double* x, const double* y,
+                       const double* z, double* slim_energy);
+/* isTetFlip per tet, src/tetwild/LocalOperations.cpp:535-553 */
+void orc_tet_flip(const int32_t* tets, int64_t n, const double* x, const double* y,
+                  const double* z, uint8_t* flipped);
+
+/* ---- (b) envelope ------------------------------------------------------ */
+/* sampleTriangle, src/tetwild/Common.cpp:143-228.  Returns sample count; writes
+ * at most cap samples (xyz triples) into ps.  A negative return never happens. */
+int64_t orc_sample_triangle(const double* v0, const double* v1, const double* v2,
+                            double d, double* ps, int64_t cap);
+/* squared distance point -> triangle (Eberly regions); closest point into cp */
+double orc_point_tri_d2(const double* p, const double* a, const double* b, const double* c,
+                        double* cp);
+
+/* Surface for envelope queries: V (nv x 3), F (nf x 3).  The oracle's decision
+ * for one point is: out <=> min over ALL facets f of d2(p, f) > eps2. */
+typedef struct orc_surface orc_surface;
+orc_surface* orc_surface_create(const double* V, int64_t nv, const int32_t* F, int64_t nf);
+void orc_surface_destroy(orc_surface* s);
+/* 1 if the point is out of the envelope (isPointOutEnvelop semantics) */
+int orc_point_out(const orc_surface* s, const double* p, double eps2);
+/* isFaceOutEnvelop_sampling semantics; returns 1 (out) / 0 (in).  n_samples
+ * receives the sample count (0 for a degenerate triangle). */
+int orc_face_out(const orc_surface* s, const double* tri9, double eps2, double d,
+                 int64_t* n_samples);
+void orc_face_out_batch(const orc_surface* s, const double* tri9, int64_t n, double eps2,
+                        double d, uint8_t* out, int64_t* total_samples);
+void orc_point_out_batch(const orc_surface* s, const double* p3, int64_t n, double eps2,
+                         uint8_t* out);
+/* nearest_facet: closest point on the surface, squared distance and facet id */
+int64_t orc_nearest(const orc_surface* s, const double* p, double* cp, double* d2);
+/* brute-force versions (O(nf) per point) used to pin the tree traversal */
+int orc_point_out_brute(const orc_surface* s, const double* p, double eps2);
+double orc_min_d2_brute(const orc_surface* s, const double* p);
+
+/* ---- (c) smoothing, src/tetwild/VertexSmoother.cpp:114-702 ------------- */
+/* One frozen one-ring problem.  ring_tets: m tets (int32 x4, indices into
+ * x/y/z), v_id the smoothed vertex.  Writes the new position (p_out) and returns
+ * is_moved (NewtonsMethod's return value). */
+int orc_newtons_method(const int32_t* ring_tets, int m, int32_t v_id, const double* x,
+                       const double* y, const double* z, double* p_out);
+/* NewtonsUpdate: E, J (3), H (9 row-major), X0.  Returns 0 when the reference
+ * returns false. */
+int orc_newtons_update(const int32_t* ring_tets, int m, int32_t v_id, const double* x,
+                       const double* y, const double* z, double* E, double* J, double* H,
+                       double* X0);
+/* getNewEnergy over the ring in the given order */
+double orc_new_energy(const int32_t* ring_tets, int m, const double* x, const double* y,
+                      const double* z);
+/* 3x3 solve by Householder QR with column pivoting: returns 0 if rank < 3 */
+int orc_solve3_qrcp(const double* A, const double* b, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

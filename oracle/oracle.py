@@ -1,0 +1,271 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and the reference AMIPS
+build (_ref/libref_amips.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_int = ctypes.c_int
+P = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        sig = {
+            "orc_amips_energy": (c_dbl, [P]),
+            "orc_amips_jacobian": (None, [P, P]),
+            "orc_amips_hessian": (None, [P, P]),
+            "orc_amips_ejh": (c_dbl, [P, P, P]),
+            "orc_rcbrt": (c_dbl, [c_dbl]),
+            "orc_amips_batch": (None, [P, c_i64, P, P, P, c_int, P, P, P]),
+            "orc_orient3d": (c_int, [P, P, P, P]),
+            "orc_collinear3": (c_int, [P, P, P]),
+            "orc_tet_qualities": (None, [P, c_i64, P, P, P, P]),
+            "orc_tet_flip": (None, [P, c_i64, P, P, P, P]),
+            "orc_sample_triangle": (c_i64, [P, P, P, c_dbl, P, c_i64]),
+            "orc_point_tri_d2": (c_dbl, [P, P, P, P, P]),
+            "orc_surface_create": (P, [P, c_i64, P, c_i64]),
+            "orc_surface_destroy": (None, [P]),
+            "orc_point_out": (c_int, [P, P, c_dbl]),
+            "orc_face_out": (c_int, [P, P, c_dbl, c_dbl, P]),
+            "orc_face_out_batch": (None, [P, P, c_i64, c_dbl, c_dbl, P, P]),
+            "orc_point_out_batch": (None, [P, P, c_i64, c_dbl, P]),
+            "orc_nearest": (c_i64, [P, P, P, P]),
+            "orc_point_out_brute": (c_int, [P, P, c_dbl]),
+            "orc_min_d2_brute": (c_dbl, [P, P]),
+            "orc_newtons_method": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P]),
+            "orc_newtons_update": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P, P, P, P]),
+            "orc_new_energy": (c_dbl, [P, c_int, P, P, P]),
+            "orc_solve3_qrcp": (c_int, [P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def ref():
+    """The reference's own AMIPS functions, compiled by ref_build.sh.  Returns
+    None when the library is absent (it is built only where the reference
+    sources exist)."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libref_amips.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        L.ref_amips_energy.restype = c_dbl
+        L.ref_amips_energy.argtypes = [P]
+        L.ref_amips_jacobian.argtypes = [P, P]
+        L.ref_amips_hessian.argtypes = [P, P]
+        L.ref_amips_batch_ejh.argtypes = [P, ctypes.c_long, P, P, P]
+        L.ref_amips_batch_e.argtypes = [P, ctypes.c_long, P]
+        _REF = L
+    return _REF
+
+
+def _p(a):
+    return a.ctypes.data_as(P)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- AMIPS ----
+def amips_batch(tets, x, y, z, mode=2):
+    tets = np.ascontiguousarray(tets, dtype=np.int32)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    n = tets.shape[0]
+    E = np.empty(n)
+    G = np.empty((3, n))
+    H = np.empty((9, n))
+    lib().orc_amips_batch(_p(tets), n, _p(x), _p(y), _p(z), mode, _p(E), _p(G), _p(H))
+    return E, G, H
+
+
+def amips_ejh_T(T):
+    """T: (n, 12) gathered tets -> E (n), G (n,3), H (n,9)."""
+    T = _f64(T).reshape(-1, 12)
+    n = T.shape[0]
+    E = np.empty(n)
+    G = np.empty((n, 3))
+    H = np.empty((n, 9))
+    L = lib()
+    for i in range(n):
+        E[i] = L.orc_amips_ejh(_p(T[i]), _p(G[i]), _p(H[i]))
+    return E, G, H
+
+
+def ref_ejh_T(T):
+    R = ref()
+    if R is None:
+        return None
+    T = _f64(T).reshape(-1, 12)
+    n = T.shape[0]
+    E = np.empty(n)
+    G = np.empty((n, 3))
+    H = np.empty((n, 9))
+    R.ref_amips_batch_ejh(_p(T), n, _p(E), _p(G), _p(H))
+    return E, G, H
+
+
+def rcbrt(x):
+    return lib().orc_rcbrt(float(x))
+
+
+def tet_qualities(tets, x, y, z):
+    tets = np.ascontiguousarray(tets, dtype=np.int32)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    out = np.empty(tets.shape[0])
+    lib().orc_tet_qualities(_p(tets), tets.shape[0], _p(x), _p(y), _p(z), _p(out))
+    return out
+
+
+def tet_flip(tets, x, y, z):
+    tets = np.ascontiguousarray(tets, dtype=np.int32)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    out = np.empty(tets.shape[0], dtype=np.uint8)
+    lib().orc_tet_flip(_p(tets), tets.shape[0], _p(x), _p(y), _p(z), _p(out))
+    return out
+
+
+def orient3d(p, q, r, s):
+    a = [_f64(v) for v in (p, q, r, s)]
+    return lib().orc_orient3d(*[_p(v) for v in a])
+
+
+def collinear3(p, q, r):
+    a = [_f64(v) for v in (p, q, r)]
+    return lib().orc_collinear3(*[_p(v) for v in a])
+
+
+# ------------------------------------------------------------- envelope ----
+def sample_triangle(v0, v1, v2, d):
+    a = [_f64(v) for v in (v0, v1, v2)]
+    L = lib()
+    n = L.orc_sample_triangle(_p(a[0]), _p(a[1]), _p(a[2]), float(d), None, 0)
+    ps = np.empty((max(n, 1), 3))
+    L.orc_sample_triangle(_p(a[0]), _p(a[1]), _p(a[2]), float(d), _p(ps), n)
+    return ps[:n]
+
+
+def point_tri_d2(p, a, b, c):
+    v = [_f64(t) for t in (p, a, b, c)]
+    cp = np.empty(3)
+    d2 = lib().orc_point_tri_d2(_p(v[0]), _p(v[1]), _p(v[2]), _p(v[3]), _p(cp))
+    return d2, cp
+
+
+class Surface:
+    """Envelope surface with the oracle's implicit AABB tree over the facets
+    in the order given."""
+
+    def __init__(self, V, F):
+        self.V = _f64(V).reshape(-1, 3)
+        self.F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        self.h = lib().orc_surface_create(_p(self.V), self.V.shape[0], _p(self.F),
+                                          self.F.shape[0])
+        if not self.h:
+            raise ValueError("empty surface")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_surface_destroy(self.h)
+            self.h = None
+
+    def face_out(self, tris, eps2, d):
+        tris = _f64(tris).reshape(-1, 9)
+        out = np.empty(tris.shape[0], dtype=np.uint8)
+        tot = c_i64(0)
+        lib().orc_face_out_batch(self.h, _p(tris), tris.shape[0], float(eps2), float(d),
+                                 _p(out), ctypes.byref(tot))
+        return out, tot.value
+
+    def point_out(self, pts, eps2):
+        pts = _f64(pts).reshape(-1, 3)
+        out = np.empty(pts.shape[0], dtype=np.uint8)
+        lib().orc_point_out_batch(self.h, _p(pts), pts.shape[0], float(eps2), _p(out))
+        return out
+
+    def point_out_brute(self, pts, eps2):
+        pts = _f64(pts).reshape(-1, 3)
+        L = lib()
+        return np.array([L.orc_point_out_brute(self.h, _p(np.ascontiguousarray(p)), float(eps2))
+                         for p in pts], dtype=np.uint8)
+
+    def min_d2_brute(self, pts):
+        pts = _f64(pts).reshape(-1, 3)
+        L = lib()
+        return np.array([L.orc_min_d2_brute(self.h, _p(np.ascontiguousarray(p))) for p in pts])
+
+    def nearest(self, pts):
+        pts = _f64(pts).reshape(-1, 3)
+        n = pts.shape[0]
+        cp = np.empty((n, 3))
+        d2 = np.empty(n)
+        fid = np.empty(n, dtype=np.int64)
+        L = lib()
+        for i in range(n):
+            dd = c_dbl(0.0)
+            fid[i] = L.orc_nearest(self.h, _p(np.ascontiguousarray(pts[i])), _p(cp[i]),
+                                   ctypes.byref(dd))
+            d2[i] = dd.value
+        return cp, d2, fid
+
+
+# ------------------------------------------------------------ smoothing ----
+def newtons_method(ring_tets, v_id, x, y, z):
+    rt = np.ascontiguousarray(ring_tets, dtype=np.int32).reshape(-1, 4)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    p = np.empty(3)
+    moved = lib().orc_newtons_method(_p(rt), rt.shape[0], int(v_id), _p(x), _p(y), _p(z), _p(p))
+    return bool(moved), p
+
+
+def newtons_update(ring_tets, v_id, x, y, z):
+    rt = np.ascontiguousarray(ring_tets, dtype=np.int32).reshape(-1, 4)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    E = c_dbl(0.0)
+    J = np.empty(3)
+    H = np.empty(9)
+    X0 = np.empty(3)
+    ok = lib().orc_newtons_update(_p(rt), rt.shape[0], int(v_id), _p(x), _p(y), _p(z),
+                                  ctypes.byref(E), _p(J), _p(H), _p(X0))
+    return bool(ok), E.value, J, H.reshape(3, 3), X0
+
+
+def new_energy(ring_tets, x, y, z):
+    rt = np.ascontiguousarray(ring_tets, dtype=np.int32).reshape(-1, 4)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    return lib().orc_new_energy(_p(rt), rt.shape[0], _p(x), _p(y), _p(z))
+
+
+def solve3(A, b):
+    A = _f64(A).reshape(9)
+    b = _f64(b).reshape(3)
+    x = np.empty(3)
+    full = lib().orc_solve3_qrcp(_p(A), _p(b), _p(x))
+    return bool(full), x

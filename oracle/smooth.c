@@ -1,0 +1,241 @@
+/*
+ * smooth.c -- oracle: the VertexSmoother Newton step on one frozen one-ring.
+ * TEST INFRASTRUCTURE ONLY.
+ *
+ * NewtonsMethod  src/tetwild/VertexSmoother.cpp:465-542
+ * getNewEnergy   src/tetwild/VertexSmoother.cpp:544-625 (scalar branch)
+ * NewtonsUpdate  src/tetwild/VertexSmoother.cpp:627-702 (scalar branch)
+ * isFlip         src/tetwild/LocalOperations.cpp:535-573 (rounded vertices)
+ *
+ * The linear solve H X = H X0 - a J is a Householder QR with column pivoting
+ * (Businger-Golub, as in Golub & Van Loan 5.4.1) with the rank rule of Eigen's
+ * ColPivHouseholderQR (pivot squared norm below (max column norm * 2^-52)^2 /
+ * rows * (rows - k) ends the nonzero pivots; the remaining unknowns are 0).
+ * Eigen is not vendored in the reference, so ulp-level agreement with it is
+ * unpinned.  Sums over the ring run in ring order, as in the reference.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+#define MAX_STEP 15
+#define MAX_IT 20
+
+static inline void coords(const double* x, const double* y, const double* z, int32_t v, double* p)
+{
+    p[0] = x[v];
+    p[1] = y[v];
+    p[2] = z[v];
+}
+
+/* position override for the smoothed vertex */
+typedef struct {
+    const double *x, *y, *z;
+    int32_t v;
+    double p[3];
+} view;
+
+static inline void vget(const view* w, int32_t v, double* p)
+{
+    if (v == w->v) { p[0] = w->p[0]; p[1] = w->p[1]; p[2] = w->p[2]; }
+    else coords(w->x, w->y, w->z, v, p);
+}
+
+static int ring_flip(const int32_t* rt, int m, const view* w)
+{
+    for (int i = 0; i < m; ++i) {
+        double T[12];
+        for (int j = 0; j < 4; ++j) vget(w, rt[4 * i + j], T + 3 * j);
+        if (orc_orient3d(T, T + 3, T + 6, T + 9) != 1) return 1;
+    }
+    return 0;
+}
+
+static double ring_energy(const int32_t* rt, int m, const view* w)
+{
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double T[12];
+        for (int j = 0; j < 4; ++j) vget(w, rt[4 * i + j], T + 3 * j);
+        s += orc_amips_energy(T);
+    }
+    if (isinf(s) || isnan(s) || s <= 0 || s > ORC_MAX_ENERGY) s = ORC_MAX_ENERGY;
+    return s;
+}
+
+static int ring_update(const int32_t* rt, int m, const view* w, double* E, double* J, double* H,
+                       double* X0)
+{
+    double e = 0.0;
+    for (int k = 0; k < 3; ++k) { J[k] = 0.0; X0[k] = w->p[k]; }
+    for (int k = 0; k < 9; ++k) H[k] = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const int32_t* t = rt + 4 * i;
+        int start = 0;
+        for (int j = 0; j < 4; ++j)
+            if (t[j] == w->v) { start = j; break; }
+        double T[12], g[3], h[9];
+        for (int j = 0; j < 4; ++j) vget(w, t[(start + j) % 4], T + 3 * j);
+        e += orc_amips_ejh(T, g, h);
+        for (int k = 0; k < 3; ++k) J[k] += g[k];
+        for (int k = 0; k < 9; ++k) H[k] += h[k];
+    }
+    if (isinf(e)) e = ORC_MAX_ENERGY;
+    *E = e;
+    if (isnan(e) || e <= 0) return 0;
+    for (int k = 0; k < 3; ++k)
+        if (!isfinite(J[k])) return 0;
+    for (int k = 0; k < 9; ++k)
+        if (!isfinite(H[k])) return 0;
+    return 1;
+}
+
+int orc_solve3_qrcp(const double* Ain, const double* b, double* x)
+{
+    double A[3][3];
+    int perm[3] = {0, 1, 2};
+    double tau[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = Ain[3 * i + j];
+
+    double norm[3];
+    double maxn = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        norm[j] = sqrt(A[0][j] * A[0][j] + A[1][j] * A[1][j] + A[2][j] * A[2][j]);
+        if (norm[j] > maxn) maxn = norm[j];
+    }
+    double thr = (maxn * 0x1p-52) * (maxn * 0x1p-52) / 3.0;
+    int rank = 3;
+    for (int k = 0; k < 3; ++k) {
+        int big = k;
+        for (int j = k + 1; j < 3; ++j)
+            if (norm[j] > norm[big]) big = j;
+        if (rank == 3 && norm[big] * norm[big] < thr * (double)(3 - k)) rank = k;
+        if (big != k) {
+            for (int i = 0; i < 3; ++i) { double t = A[i][k]; A[i][k] = A[i][big]; A[i][big] = t; }
+            double t = norm[k]; norm[k] = norm[big]; norm[big] = t;
+            int p = perm[k]; perm[k] = perm[big]; perm[big] = p;
+        }
+        /* Householder vector for A[k..2][k]: v = (1, ess), H = I - tau v v^T */
+        double tail = 0.0;
+        for (int i = k + 1; i < 3; ++i) tail += A[i][k] * A[i][k];
+        double c0 = A[k][k];
+        double beta;
+        if (tail <= 0x1p-1022) {
+            tau[k] = 0.0;
+            beta = c0;
+            for (int i = k + 1; i < 3; ++i) A[i][k] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+            for (int i = k + 1; i < 3; ++i) A[i][k] = A[i][k] / (c0 - beta);
+            tau[k] = (beta - c0) / beta;
+        }
+        A[k][k] = beta;
+        /* apply to the remaining columns */
+        for (int j = k + 1; j < 3; ++j) {
+            if (tau[k] == 0.0) break;
+            double w = A[k][j];
+            for (int i = k + 1; i < 3; ++i) w += A[i][k] * A[i][j];
+            A[k][j] -= tau[k] * w;
+            for (int i = k + 1; i < 3; ++i) A[i][j] -= tau[k] * A[i][k] * w;
+        }
+        /* column norms of the trailing block, recomputed directly */
+        for (int j = k + 1; j < 3; ++j) {
+            double s = 0.0;
+            for (int i = k + 1; i < 3; ++i) s += A[i][j] * A[i][j];
+            norm[j] = sqrt(s);
+        }
+    }
+    if (rank == 0) {
+        x[0] = x[1] = x[2] = 0.0;
+        return 0;
+    }
+    double c[3] = {b[0], b[1], b[2]};
+    for (int k = 0; k < rank; ++k) {
+        if (tau[k] == 0.0) continue;
+        double w = c[k];
+        for (int i = k + 1; i < 3; ++i) w += A[i][k] * c[i];
+        c[k] -= tau[k] * w;
+        for (int i = k + 1; i < 3; ++i) c[i] -= tau[k] * A[i][k] * w;
+    }
+    for (int i = rank - 1; i >= 0; --i) {
+        double s = c[i];
+        for (int j = i + 1; j < rank; ++j) s -= A[i][j] * c[j];
+        c[i] = s / A[i][i];
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < rank; ++i) out[perm[i]] = c[i];
+    x[0] = out[0];
+    x[1] = out[1];
+    x[2] = out[2];
+    return rank == 3;
+}
+
+int orc_newtons_update(const int32_t* ring_tets, int m, int32_t v_id, const double* x,
+                       const double* y, const double* z, double* E, double* J, double* H,
+                       double* X0)
+{
+    view w = {x, y, z, v_id, {x[v_id], y[v_id], z[v_id]}};
+    return ring_update(ring_tets, m, &w, E, J, H, X0);
+}
+
+double orc_new_energy(const int32_t* ring_tets, int m, const double* x, const double* y,
+                      const double* z)
+{
+    view w = {x, y, z, -1, {0, 0, 0}};
+    return ring_energy(ring_tets, m, &w);
+}
+
+int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, const double* y,
+                       const double* z, double* p_out)
+{
+    view w = {x, y, z, v_id, {x[v_id], y[v_id], z[v_id]}};
+    int is_moved = 0;
+    double old_energy = 0.0;
+    double J[3], H[9], X0[3];
+    for (int step = 0; step < MAX_STEP; ++step) {
+        if (!ring_update(rt, m, &w, &old_energy, J, H, X0)) break;
+        double old_p[3] = {w.p[0], w.p[1], w.p[2]};
+        double a = 1.0;
+        int step_taken = 0;
+        double new_energy = 0.0;
+        for (int it = 0; it < MAX_IT; ++it) {
+            double rhs[3], X[3];
+            for (int i = 0; i < 3; ++i) {
+                double hx = H[3 * i + 0] * X0[0] + H[3 * i + 1] * X0[1] + H[3 * i + 2] * X0[2];
+                rhs[i] = hx - a * J[i];
+            }
+            orc_solve3_qrcp(H, rhs, X);
+            if (!(isfinite(X[0]) && isfinite(X[1]) && isfinite(X[2]))) {
+                a /= 2.0;
+                continue;
+            }
+            w.p[0] = X[0]; w.p[1] = X[1]; w.p[2] = X[2];
+            if (ring_flip(rt, m, &w)) {
+                memcpy(w.p, old_p, sizeof old_p);
+                a /= 2.0;
+                continue;
+            }
+            new_energy = ring_energy(rt, m, &w);
+            if (new_energy >= old_energy || isinf(new_energy) || isnan(new_energy)) {
+                memcpy(w.p, old_p, sizeof old_p);
+                a /= 2.0;
+                continue;
+            }
+            step_taken = 1;
+            break;
+        }
+        if (step_taken && fabs(new_energy - old_energy) < 1e-5) step_taken = 0;
+        if (!step_taken) {
+            is_moved = step != 0;
+            break;
+        }
+        is_moved = 1;
+    }
+    p_out[0] = w.p[0];
+    p_out[1] = w.p[1];
+    p_out[2] = w.p[2];
+    return is_moved;
+}

@@ -1,0 +1,175 @@
+"""GPU parity of (a): AMIPS E/G/H, calTetQualities, isTetFlip and the mixed
+FP32/FP64 path, through the C-ABI, against the CPU oracle.
+
+Bar: FP64 outputs bit-exact with the oracle (same operation order, no FMA);
+orientation / flip / MAX_ENERGY decisions identical; mixed energies within
+their certified bound, fallback energies bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tetwild_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLD = "tests/golden/amips_golden.npz"
+
+
+def to_dev(dev, *arrs):
+    return [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in arrs]
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+def golden_mesh():
+    g = np.load(GOLD)
+    T = g["T"].reshape(-1, 4, 3)
+    n = T.shape[0]
+    x = np.ascontiguousarray(T[:, :, 0].reshape(-1))
+    y = np.ascontiguousarray(T[:, :, 1].reshape(-1))
+    z = np.ascontiguousarray(T[:, :, 2].reshape(-1))
+    tets = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
+    return x, y, z, tets, g
+
+
+@pytest.mark.parametrize("case", ["lattice", "aspect", "golden", "gauss"])
+def test_amips_fp64_bitexact(ctx, dev, case):
+    print(f"case={case}", flush=True)
+    if case == "lattice":
+        x, y, z, tets, _ = W.kuhn_lattice(24, seed=1)
+    elif case == "aspect":
+        x, y, z, tets = W.aspect_tets(40000, seed=2)
+    elif case == "golden":
+        x, y, z, tets, _ = golden_mesh()
+    else:
+        st = W.Stream(3, 0)
+        x, y, z = (st.normal(40000) for _ in range(3))
+        tets = st.integers(4 * 10000, 0, 39999).reshape(-1, 4).astype(np.int32)
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    for mode in (0, 1, 2):
+        E, G, H = ctx.amips_eval(tt, tx, ty, tz, mode=mode)
+        torch.cuda.synchronize()
+        Eo, Go, Ho = O.amips_batch(tets, x, y, z, mode)
+        assert np.array_equal(bits(E.cpu().numpy()), bits(Eo)), f"E mode {mode}"
+        if mode >= 1:
+            assert np.array_equal(bits(G.cpu().numpy()), bits(Go)), f"G mode {mode}"
+        if mode >= 2:
+            assert np.array_equal(bits(H.cpu().numpy()), bits(Ho)), "H"
+
+
+def test_amips_matches_reference_golden(ctx, dev):
+    """GPU energies vs the reference's own functions (fixture), to the
+    conditioning-scaled tolerance the oracle meets."""
+    x, y, z, tets, g = golden_mesh()
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    E, G, H = ctx.amips_eval(tt, tx, ty, tz)
+    E = E.cpu().numpy()
+    k = g["kind"]
+    fin = k != 4
+    P = g["T"].reshape(-1, 4, 3)
+    emin = np.min([np.linalg.norm(P[:, i] - P[:, j], axis=1)
+                   for i in range(4) for j in range(i + 1, 4)], axis=0)
+    cond = np.maximum(1.0, (np.abs(P).max(axis=(1, 2)) / emin) ** 2)
+    r = np.abs(E[fin] - g["E"][fin]) / g["E"][fin]
+    assert np.all(r <= 1e-13 * cond[fin])
+    assert np.all(np.isinf(E[~fin]))
+    print(f"max rel err vs reference: {r.max():.3e} (median {np.median(r):.3e})")
+
+
+def test_qualities_and_flip(ctx, dev):
+    x, y, z, tets, _ = W.kuhn_lattice(20, seed=5)
+    st = W.Stream(4, 0)
+    # mix in inverted and exactly flat tets
+    inv = tets[:5000][:, [0, 1, 3, 2]]
+    xf = np.concatenate([x, [0.0, 1.0, 0.0, 1.0]])
+    yf = np.concatenate([y, [0.0, 0.0, 1.0, 1.0]])
+    zf = np.concatenate([z, [0.0, 0.0, 0.0, 0.0]])
+    nv = len(x)
+    flat = np.array([[nv, nv + 1, nv + 2, nv + 3]] * 7, dtype=np.int32)
+    allt = np.ascontiguousarray(np.concatenate([tets, inv, flat]).astype(np.int32))
+    perm = np.argsort(st.uniform(len(allt)))
+    allt = np.ascontiguousarray(allt[perm])
+    tx, ty, tz, tt = to_dev(dev, xf, yf, zf, allt)
+    q, fb = ctx.tet_qualities(tt, tx, ty, tz)
+    fl = ctx.tet_flip(tt, tx, ty, tz)
+    qo = O.tet_qualities(allt, xf, yf, zf)
+    flo = O.tet_flip(allt, xf, yf, zf)
+    assert np.array_equal(bits(q.cpu().numpy()), bits(qo))
+    assert np.array_equal(fl.cpu().numpy(), flo)
+    assert (qo == 1e50).sum() == 5007
+
+
+def test_orientation_near_degenerate(ctx, dev):
+    """exact predicate on the GPU: coplanar integer points and 1-ulp nudges"""
+    st = W.Stream(13, 0)
+    pts, tets = [], []
+    for i in range(4000):
+        a = st.integers(9, -1000, 1000).reshape(3, 3).astype(float)
+        u, v = st.integers(2, -3, 3)
+        d = a[0] + u * (a[1] - a[0]) + v * (a[2] - a[0])
+        P = np.vstack([a, d])
+        k = int(st.integers(1, 0, 3)[0]); c = int(st.integers(1, 0, 2)[0])
+        m = int(st.integers(1, 0, 2)[0])
+        if m == 1:
+            P[k, c] = np.nextafter(P[k, c], np.inf)
+        elif m == 2:
+            P[k, c] = np.nextafter(P[k, c], -np.inf)
+        pts.append(P)
+        tets.append([4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3])
+    P = np.concatenate(pts)
+    x, y, z = (np.ascontiguousarray(P[:, j]) for j in range(3))
+    tets = np.array(tets, dtype=np.int32)
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    fl = ctx.tet_flip(tt, tx, ty, tz).cpu().numpy()
+    assert np.array_equal(fl, O.tet_flip(tets, x, y, z))
+    assert 0 < fl.sum() < len(fl)
+
+
+def test_mixed_certified(ctx, dev):
+    x, y, z, tets = W.aspect_tets(60000, seed=8)
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    tol = 2.0**-16
+    E64, G64, H64 = ctx.amips_eval(tt, tx, ty, tz)
+    E, G, H, fb = ctx.amips_eval_mixed(tt, tx, ty, tz, tol=tol)
+    E64 = E64.cpu().numpy(); E = E.cpu().numpy(); fb = fb.cpu().numpy().astype(bool)
+    assert np.array_equal(bits(E[fb]), bits(E64[fb]))
+    rel = np.abs(E[~fb] - E64[~fb]) / E64[~fb]
+    assert np.all(rel <= tol)
+    assert 0.05 < fb.mean() < 0.95
+    print(f"fallback rate {fb.mean():.3f}, max rel err (fp32 part) {rel.max():.3e}")
+    # qualities: same decisions, energies within tol
+    q64, _ = ctx.tet_qualities(tt, tx, ty, tz)
+    qm, qfb = ctx.tet_qualities(tt, tx, ty, tz, precision=1, tol=tol)
+    q64 = q64.cpu().numpy(); qm = qm.cpu().numpy()
+    assert np.array_equal(q64 == 1e50, qm == 1e50)
+    ok = q64 < 1e50
+    assert np.all(np.abs(qm[ok] - q64[ok]) <= tol * q64[ok])
+
+
+def test_invalid_ids_and_empty(ctx, dev):
+    x, y, z, tets, _ = W.kuhn_lattice(2, seed=1)
+    bad = tets.copy()
+    bad[3, 2] = len(x)
+    bad[5, 0] = -1
+    tx, ty, tz, tt = to_dev(dev, x, y, z, bad)
+    E, G, H = ctx.amips_eval(tt, tx, ty, tz)
+    E = E.cpu().numpy()
+    assert np.isnan(E[3]) and np.isnan(E[5]) and np.isfinite(np.delete(E, [3, 5])).all()
+    fl = ctx.tet_flip(tt, tx, ty, tz).cpu().numpy()
+    assert fl[3] == 1 and fl[5] == 1
+    empty = torch.empty((0, 4), dtype=torch.int32, device=dev)
+    E, G, H = ctx.amips_eval(empty, tx, ty, tz)
+    assert E.numel() == 0
+
+
+def test_large_lattice_checksum(ctx, dev):
+    """2M tets: whole-array bit equality with the oracle"""
+    x, y, z, tets, _ = W.kuhn_lattice(70, seed=0)
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    E, G, H = ctx.amips_eval(tt, tx, ty, tz)
+    Eo, Go, Ho = O.amips_batch(tets, x, y, z, 2)
+    assert np.array_equal(bits(E.cpu().numpy()), bits(Eo))
+    assert np.array_equal(bits(H.cpu().numpy()), bits(Ho))

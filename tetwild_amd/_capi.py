@@ -1,0 +1,72 @@
+"""ctypes binding of libtetwild_amd.so (the C-ABI declared in
+include/tetwild_gpu.h).  Loading fails loudly when the library has not been
+built: there is no host fallback for any entry point."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtetwild_amd.so")
+
+TWG_OK = 0
+TWG_EINVAL = 1
+TWG_EDEVICE = 2
+TWG_ENOMEM = 3
+TWG_FP64 = 0
+TWG_MIXED = 1
+TWG_AMIPS_E = 0
+TWG_AMIPS_EG = 1
+TWG_AMIPS_EGH = 2
+MAX_ENERGY = 1e50
+
+P = ctypes.c_void_p
+i32 = ctypes.c_int
+i64 = ctypes.c_int64
+f64 = ctypes.c_double
+sz = ctypes.c_size_t
+
+# name -> argument types (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "twg_ctx_create": [i32, ctypes.POINTER(P)],
+    "twg_ctx_destroy": [P],
+    "twg_ctx_set_stream": [P, P],
+    "twg_ctx_stream": [P],
+    "twg_ctx_sync": [P],
+    "twg_last_error": [],
+    "twg_malloc": [P, sz, ctypes.POINTER(P)],
+    "twg_free": [P, P],
+    "twg_memcpy_h2d": [P, P, P, sz],
+    "twg_memcpy_d2h": [P, P, P, sz],
+    "twg_amips_eval": [P, P, i64, P, P, P, i64, i32, P, P, P],
+    "twg_amips_eval_mixed": [P, P, i64, P, P, P, i64, i32, f64, P, P, P, P],
+    "twg_tet_qualities": [P, P, i64, P, P, P, i64, i32, f64, P, P],
+    "twg_tet_flip": [P, P, i64, P, P, P, i64, P],
+}
+_RESTYPES = {"twg_last_error": ctypes.c_char_p, "twg_ctx_stream": P}
+
+_LIB = None
+
+
+class TWGError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                "g.build()'` (or make -C tetwild_amd/csrc); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, i32)
+        _LIB = L
+    return _LIB
+
+
+def check(status, what=""):
+    if status != TWG_OK:
+        msg = lib().twg_last_error()
+        raise TWGError(f"{what}: status {status}: {msg.decode() if msg else ''}")

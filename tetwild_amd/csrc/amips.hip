@@ -1,0 +1,317 @@
+// amips.hip -- (a) per-tet AMIPS energy / gradient / Hessian, calTetQualities
+// and the flip test, one thread per tet.
+//
+// Data path: the tet's four vertex ids come in as one 16-byte load; the twelve
+// coordinates are gathered from the SoA x/y/z arrays (each vertex is shared by
+// ~24 tets of a lattice, so the gathers mostly hit L2); results leave as
+// planar SoA stores, 64 lanes x 8 B = 512 contiguous bytes per store
+// instruction.  Vertex ids outside [0, nv) make the tet's outputs NaN (flip
+// test: flipped) instead of reading out of bounds.
+#include "twg_internal.hpp"
+#include "twg_math.hpp"
+
+namespace twg {
+namespace {
+
+__device__ __forceinline__ bool load_tet(const int32_t* __restrict__ tets, int64_t i,
+                                         const double* __restrict__ x,
+                                         const double* __restrict__ y,
+                                         const double* __restrict__ z, int64_t nv, Tet& T)
+{
+    int4 t = reinterpret_cast<const int4*>(tets)[i];
+    int id[4] = {t.x, t.y, t.z, t.w};
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ok = ok && id[j] >= 0 && (int64_t)id[j] < nv;
+    if (!ok) return false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        T.p[j][0] = x[id[j]];
+        T.p[j][1] = y[id[j]];
+        T.p[j][2] = z[id[j]];
+    }
+    return true;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void amips_eval_kernel(
+    const int32_t* __restrict__ tets, int64_t n, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int64_t nv, double* __restrict__ E,
+    double* __restrict__ G, double* __restrict__ H)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Tet T;
+    if (!load_tet(tets, i, x, y, z, nv, T)) {
+        double q = __builtin_nan("");
+        E[i] = q;
+        if (MODE >= 1)
+            for (int k = 0; k < 3; ++k) G[k * n + i] = q;
+        if (MODE >= 2)
+            for (int k = 0; k < 9; ++k) H[k * n + i] = q;
+        return;
+    }
+    AmipsCore c = amips_core(T);
+    E[i] = c.E;
+    if (MODE >= 1) {
+        double g[3], u[3], al[3], be[3];
+        amips_grad(T, c, g, u, al, be);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[k * n + i] = g[k];
+        if (MODE >= 2) {
+            double h[9];
+            amips_hess(c, u, al, be, h);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) H[k * n + i] = h[k];
+        }
+    }
+}
+
+// FP32 gradient / Hessian from FP32 edge vectors (mixed path)
+__device__ __forceinline__ void grad_hess32(const AmipsCore32& c, const float e21[3],
+                                            const float e31[3], float* g, float* h, bool want_h)
+{
+    float nx = e21[1] * e31[2] - e21[2] * e31[1];
+    float ny = e21[2] * e31[0] - e21[0] * e31[2];
+    float nz = e21[0] * e31[1] - e21[1] * e31[0];
+    float rQ = 1.0f / c.Q;
+    float rV = 1.0f / c.V6;
+    float al[3] = {c.sx * rQ, c.sy * rQ, c.sz * rQ};
+    float be[3] = {nx * rV, ny * rV, nz * rV};
+    float u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        u[k] = (2.0f / 3.0f) * be[k] - al[k];
+        g[k] = c.E * u[k];
+    }
+    if (!want_h) return;
+    float D = 3.0f * c.w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            float t = u[i] * u[j] - al[i] * al[j] + (2.0f / 3.0f) * (be[i] * be[j]);
+            float v = c.E * t;
+            if (i == j) v += D;
+            h[i * 3 + j] = v;
+            h[j * 3 + i] = v;
+        }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void amips_mixed_kernel(
+    const int32_t* __restrict__ tets, int64_t n, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int64_t nv, float tol,
+    double* __restrict__ E, float* __restrict__ G, float* __restrict__ H,
+    uint8_t* __restrict__ fallback)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Tet T;
+    if (!load_tet(tets, i, x, y, z, nv, T)) {
+        E[i] = __builtin_nan("");
+        if (MODE >= 1)
+            for (int k = 0; k < 3; ++k) G[k * n + i] = __builtin_nanf("");
+        if (MODE >= 2)
+            for (int k = 0; k < 9; ++k) H[k * n + i] = __builtin_nanf("");
+        if (fallback) fallback[i] = 1;
+        return;
+    }
+    float d[3][3], e21[3], e31[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d[0][k] = (float)(T.p[1][k] - T.p[0][k]);
+        d[1][k] = (float)(T.p[2][k] - T.p[0][k]);
+        d[2][k] = (float)(T.p[3][k] - T.p[0][k]);
+        e21[k] = (float)(T.p[2][k] - T.p[1][k]);
+        e31[k] = (float)(T.p[3][k] - T.p[1][k]);
+    }
+    AmipsCore32 c = amips_core32(d);
+    float g[3], h[9];
+    bool fb = !(c.rel <= tol) || !(fabsf_(c.E) < __builtin_inff());
+    if (!fb) {
+        E[i] = (double)c.E;
+        if (MODE >= 1) grad_hess32(c, e21, e31, g, h, MODE >= 2);
+    } else {
+        AmipsCore c64 = amips_core(T);
+        E[i] = c64.E;
+        if (MODE >= 1) {
+            double gd[3], u[3], al[3], be[3], hd[9];
+            amips_grad(T, c64, gd, u, al, be);
+            for (int k = 0; k < 3; ++k) g[k] = (float)gd[k];
+            if (MODE >= 2) {
+                amips_hess(c64, u, al, be, hd);
+                for (int k = 0; k < 9; ++k) h[k] = (float)hd[k];
+            }
+        }
+    }
+    if (MODE >= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[k * n + i] = g[k];
+    if (MODE >= 2)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H[k * n + i] = h[k];
+    if (fallback) fallback[i] = fb ? 1 : 0;
+}
+
+// calTetQuality_AMIPS semantics (src/tetwild/LocalOperations.cpp:862-884)
+__device__ __forceinline__ double quality_from(double e)
+{
+    if (is_inf(e) || is_nan(e) || e <= 0) return kMaxEnergy;
+    return e;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void tet_quality_kernel(
+    const int32_t* __restrict__ tets, int64_t n, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int64_t nv, float tol,
+    double* __restrict__ out, uint8_t* __restrict__ fallback)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Tet T;
+    uint8_t fb = 0;
+    double e;
+    if (!load_tet(tets, i, x, y, z, nv, T)) {
+        e = __builtin_nan("");
+        fb = 1;
+    } else if (orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1) {
+        e = kMaxEnergy;
+    } else if (!MIXED) {
+        e = quality_from(amips_energy(T));
+    } else {
+        float d[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            d[0][k] = (float)(T.p[1][k] - T.p[0][k]);
+            d[1][k] = (float)(T.p[2][k] - T.p[0][k]);
+            d[2][k] = (float)(T.p[3][k] - T.p[0][k]);
+        }
+        AmipsCore32 c = amips_core32(d);
+        if (c.rel <= tol && fabsf_(c.E) < __builtin_inff() && c.E > 0.0f) {
+            e = (double)c.E;
+        } else {
+            e = quality_from(amips_energy(T));
+            fb = 1;
+        }
+    }
+    out[i] = e;
+    if (fallback) fallback[i] = fb;
+}
+
+__global__ __launch_bounds__(kBlock) void tet_flip_kernel(const int32_t* __restrict__ tets,
+                                                          int64_t n, const double* __restrict__ x,
+                                                          const double* __restrict__ y,
+                                                          const double* __restrict__ z, int64_t nv,
+                                                          uint8_t* __restrict__ flipped)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Tet T;
+    if (!load_tet(tets, i, x, y, z, nv, T)) {
+        flipped[i] = 1;
+        return;
+    }
+    flipped[i] = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
+}
+
+int check_common(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x, const double* y,
+                 const double* z, int64_t nv, const char* fn)
+{
+    TWG_REQUIRE(ctx != nullptr, "%s: ctx is NULL", fn);
+    TWG_REQUIRE(n >= 0 && nv >= 0, "%s: negative size", fn);
+    TWG_REQUIRE(n == 0 || (tets && x && y && z), "%s: NULL array", fn);
+    TWG_REQUIRE(((uintptr_t)tets & 15) == 0, "%s: tets must be 16-byte aligned", fn);
+    return TWG_OK;
+}
+
+}  // namespace
+}  // namespace twg
+
+using namespace twg;
+
+extern "C" {
+
+int twg_amips_eval(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x,
+                   const double* y, const double* z, int64_t nv, int mode, double* E, double* G,
+                   double* H)
+{
+    if (int r = check_common(ctx, tets, n, x, y, z, nv, "twg_amips_eval")) return r;
+    TWG_REQUIRE(mode >= TWG_AMIPS_E && mode <= TWG_AMIPS_EGH, "twg_amips_eval: bad mode %d", mode);
+    TWG_REQUIRE(n == 0 || (E && (mode < 1 || G) && (mode < 2 || H)),
+                "twg_amips_eval: NULL output");
+    if (n == 0) return TWG_OK;
+    if (int r = bind(ctx)) return r;
+    dim3 g(grid_for(n)), b(kBlock);
+    if (mode == 0)
+        hipLaunchKernelGGL(amips_eval_kernel<0>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
+    else if (mode == 1)
+        hipLaunchKernelGGL(amips_eval_kernel<1>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
+    else
+        hipLaunchKernelGGL(amips_eval_kernel<2>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
+    TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+int twg_amips_eval_mixed(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x,
+                         const double* y, const double* z, int64_t nv, int mode, double tol,
+                         double* E, float* G, float* H, uint8_t* fallback)
+{
+    if (int r = check_common(ctx, tets, n, x, y, z, nv, "twg_amips_eval_mixed")) return r;
+    TWG_REQUIRE(mode >= TWG_AMIPS_E && mode <= TWG_AMIPS_EGH, "twg_amips_eval_mixed: bad mode %d",
+                mode);
+    TWG_REQUIRE(n == 0 || (E && (mode < 1 || G) && (mode < 2 || H)),
+                "twg_amips_eval_mixed: NULL output");
+    TWG_REQUIRE(tol >= 0.0, "twg_amips_eval_mixed: negative tol");
+    if (n == 0) return TWG_OK;
+    if (int r = bind(ctx)) return r;
+    dim3 g(grid_for(n)), b(kBlock);
+    float t = (float)tol;
+    if (mode == 0)
+        hipLaunchKernelGGL(amips_mixed_kernel<0>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, t, E, G,
+                           H, fallback);
+    else if (mode == 1)
+        hipLaunchKernelGGL(amips_mixed_kernel<1>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, t, E, G,
+                           H, fallback);
+    else
+        hipLaunchKernelGGL(amips_mixed_kernel<2>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, t, E, G,
+                           H, fallback);
+    TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+int twg_tet_qualities(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x,
+                      const double* y, const double* z, int64_t nv, int precision, double tol,
+                      double* slim_energy, uint8_t* fallback)
+{
+    if (int r = check_common(ctx, tets, n, x, y, z, nv, "twg_tet_qualities")) return r;
+    TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED,
+                "twg_tet_qualities: bad precision %d", precision);
+    TWG_REQUIRE(n == 0 || slim_energy, "twg_tet_qualities: NULL output");
+    if (n == 0) return TWG_OK;
+    if (int r = bind(ctx)) return r;
+    dim3 g(grid_for(n)), b(kBlock);
+    if (precision == TWG_FP64)
+        hipLaunchKernelGGL(tet_quality_kernel<false>, g, b, 0, ctx->cur, tets, n, x, y, z, nv,
+                           (float)tol, slim_energy, fallback);
+    else
+        hipLaunchKernelGGL(tet_quality_kernel<true>, g, b, 0, ctx->cur, tets, n, x, y, z, nv,
+                           (float)tol, slim_energy, fallback);
+    TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+int twg_tet_flip(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x, const double* y,
+                 const double* z, int64_t nv, uint8_t* flipped)
+{
+    if (int r = check_common(ctx, tets, n, x, y, z, nv, "twg_tet_flip")) return r;
+    TWG_REQUIRE(n == 0 || flipped, "twg_tet_flip: NULL output");
+    if (n == 0) return TWG_OK;
+    if (int r = bind(ctx)) return r;
+    hipLaunchKernelGGL(tet_flip_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->cur, tets, n, x,
+                       y, z, nv, flipped);
+    TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// twg_internal.hpp -- context, error reporting and launch helpers shared by
+// the translation units of libtetwild_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/tetwild_gpu.h"
+
+struct twg_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t cur = nullptr;
+    int num_cus = 256;
+    // growable device scratch owned by the context
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+namespace twg {
+
+void set_error(const char* fmt, ...);
+
+// returns nonzero (TWG_EDEVICE) and records the message on failure
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+int ensure_scratch(twg_ctx* ctx, size_t bytes);
+
+// Activates ctx->device for the calling thread.
+int bind(twg_ctx* ctx);
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock)
+{
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace twg
+
+#define TWG_HIP(expr)                                                    \
+    do {                                                                 \
+        hipError_t _e = (expr);                                          \
+        if (_e != hipSuccess) return twg::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define TWG_REQUIRE(cond, ...)                       \
+    do {                                             \
+        if (!(cond)) {                               \
+            twg::set_error(__VA_ARGS__);             \
+            return TWG_EINVAL;                       \
+        }                                            \
+    } while (0)
+
+// after a kernel launch
+#define TWG_LAUNCHED() TWG_HIP(hipGetLastError())

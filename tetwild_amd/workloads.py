@@ -1,0 +1,341 @@
+"""Seeded synthetic workloads for the mesh-optimisation inner loop.
+
+All random streams come from SplitMix64 counters (so they do not depend on
+numpy's generator versions) with Box-Muller normals.
+
+  kuhn_lattice(n, jitter)   jittered Kuhn lattice (6 tets per cube) in the unit
+                            cube: n cubes per axis -> (n+1)^3 vertices, 6 n^3 tets
+  aspect_tets(n)            separate tets: a regular tet stretched along one
+                            (pancake) or two (needle) axes by 10^U[0,4], then a
+                            random rotation and translation
+  bumpy_sphere(freq)        geodesic icosphere of the given frequency displaced
+                            radially by 0.05 * (sum of 8 random real spherical
+                            harmonics)
+  query_triangles(...)      near-equilateral query triangles offset from a
+                            surface along the facet normal
+  icosphere_jittered(...)   subdivided icosphere with radial Gaussian jitter
+"""
+import math
+
+import numpy as np
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """Counter-based SplitMix64 stream: draw k values at a time."""
+
+    def __init__(self, seed, tag=0):
+        self.base = np.uint64((seed * 0x100000001B3 + tag * 0x51ED27) & 0xFFFFFFFFFFFFFFFF)
+        self.pos = 0
+
+    def u64(self, k):
+        with np.errstate(over="ignore"):
+            idx = np.arange(self.pos + 1, self.pos + 1 + k, dtype=np.uint64)
+            z = self.base + idx * _G
+            self.pos += k
+            return _mix(z)
+
+    def uniform(self, k, lo=0.0, hi=1.0):
+        """U(lo, hi), never exactly lo."""
+        u = ((self.u64(k) >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0**-53
+        return lo + (hi - lo) * u
+
+    def normal(self, k):
+        m = (k + 1) // 2
+        u1 = self.uniform(m)
+        u2 = self.uniform(m)
+        r = np.sqrt(-2.0 * np.log(u1))
+        z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return z[:k]
+
+    def integers(self, k, lo, hi):
+        """uniform integers in [lo, hi]"""
+        span = np.uint64(hi - lo + 1)
+        return (self.u64(k) % span).astype(np.int64) + lo
+
+
+# ------------------------------------------------------------------- tets --
+_KUHN = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def _orient_filter(T):
+    """sign of det[q-p; r-p; s-p] for gathered (n,4,3) tets when certain, 0 when
+    the floating-point filter cannot decide."""
+    a = T[:, 1] - T[:, 0]
+    b = T[:, 2] - T[:, 0]
+    c = T[:, 3] - T[:, 0]
+    m1 = b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1]
+    m2 = b[:, 2] * c[:, 0] - b[:, 0] * c[:, 2]
+    m3 = b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0]
+    det = a[:, 0] * m1 + a[:, 1] * m2 + a[:, 2] * m3
+    perm = (np.abs(a[:, 0]) * (np.abs(b[:, 1] * c[:, 2]) + np.abs(b[:, 2] * c[:, 1])) +
+            np.abs(a[:, 1]) * (np.abs(b[:, 2] * c[:, 0]) + np.abs(b[:, 0] * c[:, 2])) +
+            np.abs(a[:, 2]) * (np.abs(b[:, 0] * c[:, 1]) + np.abs(b[:, 1] * c[:, 0])))
+    bound = 2.0**-50 * perm
+    s = np.zeros(len(det), dtype=np.int8)
+    s[det > bound] = 1
+    s[-det > bound] = -1
+    return s
+
+
+def _orient_exact(t):
+    from fractions import Fraction as F
+    p = [[F(float(v)) for v in row] for row in t]
+    a = [p[1][i] - p[0][i] for i in range(3)]
+    b = [p[2][i] - p[0][i] for i in range(3)]
+    c = [p[3][i] - p[0][i] for i in range(3)]
+    det = (a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) +
+           a[2] * (b[0] * c[1] - b[1] * c[0]))
+    return (det > 0) - (det < 0)
+
+
+def orientation(tets, x, y, z, chunk=1 << 22):
+    out = np.empty(len(tets), dtype=np.int8)
+    for s in range(0, len(tets), chunk):
+        t = tets[s:s + chunk]
+        T = np.stack([np.stack([x[t[:, j]], y[t[:, j]], z[t[:, j]]], axis=1) for j in range(4)],
+                     axis=1)
+        o = _orient_filter(T)
+        for i in np.nonzero(o == 0)[0]:
+            o[i] = _orient_exact(T[i])
+        out[s:s + chunk] = o
+    return out
+
+
+def kuhn_lattice(n, jitter=0.15, seed=0):
+    """Returns x, y, z (float64 SoA), tets (int32, (6 n^3, 4)), n_swapped."""
+    n1 = n + 1
+    h = 1.0 / n
+    g = np.arange(n1, dtype=np.float64) * h
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    # vertex id = i + n1*(j + n1*k) -> use Fortran-like order of (i, j, k)
+    x = np.ascontiguousarray(X.transpose(2, 1, 0).reshape(-1))
+    y = np.ascontiguousarray(Y.transpose(2, 1, 0).reshape(-1))
+    z = np.ascontiguousarray(Z.transpose(2, 1, 0).reshape(-1))
+    nv = n1 ** 3
+    st = Stream(seed, 2)
+    x += st.normal(nv) * (jitter * h)
+    y += st.normal(nv) * (jitter * h)
+    z += st.normal(nv) * (jitter * h)
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ii = ii.transpose(2, 1, 0).reshape(-1)
+    jj = jj.transpose(2, 1, 0).reshape(-1)
+    kk = kk.transpose(2, 1, 0).reshape(-1)
+    base = (ii + n1 * (jj + n1 * kk)).astype(np.int64)
+    step = np.array([1, n1, n1 * n1], dtype=np.int64)
+    tets = np.empty((len(base), 6, 4), dtype=np.int64)
+    for t, perm in enumerate(_KUHN):
+        v = base.copy()
+        tets[:, t, 0] = v
+        v = v + step[perm[0]]
+        tets[:, t, 1] = v
+        v = v + step[perm[1]]
+        tets[:, t, 2] = v
+        tets[:, t, 3] = base + step.sum()
+    tets = tets.reshape(-1, 4).astype(np.int32)
+    o = orientation(tets, x, y, z)
+    bad = o <= 0
+    tmp = tets[bad, 2].copy()
+    tets[bad, 2] = tets[bad, 3]
+    tets[bad, 3] = tmp
+    return x, y, z, np.ascontiguousarray(tets), int(bad.sum())
+
+
+def _rotations(st, k):
+    q = st.normal(4 * k).reshape(k, 4)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, a, b, c = q.T
+    R = np.empty((k, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (b * b + c * c)
+    R[:, 0, 1] = 2 * (a * b - c * w)
+    R[:, 0, 2] = 2 * (a * c + b * w)
+    R[:, 1, 0] = 2 * (a * b + c * w)
+    R[:, 1, 1] = 1 - 2 * (a * a + c * c)
+    R[:, 1, 2] = 2 * (b * c - a * w)
+    R[:, 2, 0] = 2 * (a * c - b * w)
+    R[:, 2, 1] = 2 * (b * c + a * w)
+    R[:, 2, 2] = 1 - 2 * (a * a + b * b)
+    return R
+
+
+REGULAR_TET = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.5, math.sqrt(3) / 2, 0.0],
+                        [0.5, math.sqrt(3) / 6, math.sqrt(2.0 / 3.0)]])
+
+
+def aspect_tets(n, seed=0, max_log10=4.0, size=0.01):
+    """Separate tets with aspect ratio 10^U[0, max_log10]: returns x, y, z of
+    4n vertices and tets (n, 4) = [4i, 4i+1, 4i+2, 4i+3], positively oriented."""
+    st = Stream(seed, 4)
+    a = 10.0 ** st.uniform(n, 0.0, max_log10)
+    needle = st.u64(n) & np.uint64(1)
+    S = np.ones((n, 3))
+    S[:, 2] = np.where(needle == 0, 1.0 / a, a)       # pancake: squash z
+    S[:, 1] = np.where(needle == 0, 1.0, a)           # needle: stretch y and z
+    P = (REGULAR_TET - REGULAR_TET.mean(0))[None, :, :] * S[:, None, :] * size
+    R = _rotations(st, n)
+    P = np.einsum("nij,nvj->nvi", R, P)
+    P += st.uniform(3 * n, -1.0, 1.0).reshape(n, 1, 3)
+    x = np.ascontiguousarray(P[:, :, 0].reshape(-1))
+    y = np.ascontiguousarray(P[:, :, 1].reshape(-1))
+    z = np.ascontiguousarray(P[:, :, 2].reshape(-1))
+    tets = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
+    o = orientation(tets, x, y, z)
+    bad = o < 0
+    tmp = tets[bad, 2].copy()
+    tets[bad, 2] = tets[bad, 3]
+    tets[bad, 3] = tmp
+    return x, y, z, np.ascontiguousarray(tets)
+
+
+# --------------------------------------------------------------- surfaces --
+def _icosahedron():
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+                 dtype=np.float64)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9],
+                  [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2],
+                  [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+                  [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    return V / np.linalg.norm(V, axis=1, keepdims=True), F
+
+
+def geodesic_sphere(freq):
+    """Unit geodesic sphere: each icosahedron face split into freq^2 triangles;
+    10 freq^2 + 2 vertices and 20 freq^2 facets, outward oriented."""
+    V0, F0 = _icosahedron()
+    ij = [(i, j) for i in range(freq + 1) for j in range(freq + 1 - i)]
+    idx = {p: k for k, p in enumerate(ij)}
+    I = np.array([p[0] for p in ij], dtype=np.float64)
+    J = np.array([p[1] for p in ij], dtype=np.float64)
+    tris = []
+    for i in range(freq):
+        for j in range(freq - i):
+            tris.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
+            if j < freq - i - 1:
+                tris.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
+    tris = np.array(tris, dtype=np.int64)
+    allV, allF = [], []
+    npf = len(ij)
+    for f, (a, b, c) in enumerate(F0):
+        A, B, C = V0[a], V0[b], V0[c]
+        P = (A[None, :] * (freq - I - J)[:, None] + B[None, :] * I[:, None] +
+             C[None, :] * J[:, None]) / freq
+        allV.append(P)
+        allF.append(tris + f * npf)
+    V = np.concatenate(allV)
+    F = np.concatenate(allF)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    key = np.round(V * 1e9).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    order = np.argsort(first)
+    remap = np.empty_like(order)
+    remap[order] = np.arange(len(order))
+    V = V[first[order]]
+    F = remap[inv[F]]
+    # outward orientation
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[F].mean(1)) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return np.ascontiguousarray(V), np.ascontiguousarray(F.astype(np.int32))
+
+
+def _real_sph_harm(l, m, theta, phi):
+    """orthonormal real spherical harmonic Y_lm (theta polar, phi azimuth)"""
+    from scipy.special import lpmv
+    am = abs(m)
+    norm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - am) / math.factorial(l + am))
+    P = lpmv(am, l, np.cos(theta)) * ((-1) ** am)  # remove Condon-Shortley phase
+    if m == 0:
+        return norm * P
+    if m > 0:
+        return math.sqrt(2) * norm * P * np.cos(am * phi)
+    return math.sqrt(2) * norm * P * np.sin(am * phi)
+
+
+def bumpy_sphere(freq=224, n_harm=8, amp=0.05, seed=0):
+    V, F = geodesic_sphere(freq)
+    st = Stream(seed, 3)
+    ls = st.integers(n_harm, 1, 8)
+    ms = np.array([int(st.integers(1, -l, l)[0]) for l in ls])
+    theta = np.arccos(np.clip(V[:, 2], -1, 1))
+    phi = np.arctan2(V[:, 1], V[:, 0])
+    r = np.ones(len(V))
+    for l, m in zip(ls, ms):
+        r += amp * _real_sph_harm(int(l), int(m), theta, phi)
+    return np.ascontiguousarray(V * r[:, None]), F
+
+
+def icosphere_jittered(levels=4, sigma=0.002, seed=0):
+    V, F = geodesic_sphere(2 ** levels)
+    st = Stream(seed, 5)
+    r = 1.0 + sigma * st.normal(len(V))
+    return np.ascontiguousarray(V * r[:, None]), F
+
+
+def facet_normals(V, F):
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    area2 = np.linalg.norm(n, axis=1)
+    return n / area2[:, None], 0.5 * area2
+
+
+def surface_points(V, F, n, seed=0, tag=6):
+    """n points uniform by area on the surface: (points, facet ids, normals)"""
+    st = Stream(seed, tag)
+    nrm, area = facet_normals(V, F)
+    cdf = np.cumsum(area)
+    cdf /= cdf[-1]
+    f = np.searchsorted(cdf, st.uniform(n), side="right")
+    f = np.minimum(f, len(F) - 1)
+    u = st.uniform(n)
+    v = st.uniform(n)
+    s = np.sqrt(u)
+    b0, b1, b2 = 1 - s, s * (1 - v), s * v
+    P = (V[F[f, 0]] * b0[:, None] + V[F[f, 1]] * b1[:, None] + V[F[f, 2]] * b2[:, None])
+    return P, f, nrm[f]
+
+
+def query_triangles(V, F, n, eps=1e-3, edge=0.002, seed=0, chunk=1 << 21):
+    """(n, 9) near-equilateral triangles: centre uniform by area on the
+    surface, in the facet plane, edge * (1 + U[-0.1, 0.1]) long, offset along
+    the facet normal by +-U[0, 2 eps]."""
+    out = np.empty((n, 9))
+    for s in range(0, n, chunk):
+        k = min(chunk, n - s)
+        C, f, N = surface_points(V, F, k, seed, tag=7 + s // chunk)
+        st = Stream(seed, 1000 + s // chunk)
+        # tangent frame
+        a = np.where(np.abs(N[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
+        t1 = np.cross(N, a)
+        t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+        t2 = np.cross(N, t1)
+        ang = st.uniform(k, 0, 2 * np.pi)
+        e = edge * (1 + st.uniform(k, -0.1, 0.1))
+        rad = e / math.sqrt(3)
+        off = st.uniform(k, -2 * eps, 2 * eps)
+        for j in range(3):
+            th = ang + j * 2 * np.pi / 3
+            P = C + rad[:, None] * (np.cos(th)[:, None] * t1 + np.sin(th)[:, None] * t2) + \
+                off[:, None] * N
+            out[s:s + k, 3 * j:3 * j + 3] = P
+    return out
+
+
+def envelope_samples(V, F, n, eps=1e-3, band=0.01, seed=0):
+    """(n, 3) points at distance ~ eps * (1 + U[-band, band]) from the surface
+    along the facet normal, either side."""
+    P, f, N = surface_points(V, F, n, seed, tag=11)
+    st = Stream(seed, 12)
+    d = eps * (1 + st.uniform(n, -band, band))
+    sgn = np.where((st.u64(n) & np.uint64(1)) == 0, 1.0, -1.0)
+    return np.ascontiguousarray(P + (sgn * d)[:, None] * N)
