@@ -82,7 +82,7 @@ def cpu_baseline_amips(x, y, z, tets, E_gpu, H_gpu, budget_s=12.0):
         if kind == "reference":
             O.ref().ref_amips_batch_ejh(O._p(T), n, O._p(E), O._p(G), O._p(H))
         else:
-            Es = np.empty(n); Gs = np.empty((3, n)); Hs = np.empty((9, n))
+            Es = np.empty(n); Gs = np.empty((n, 3)); Hs = np.empty((n, 9))
             O.lib().orc_amips_batch(O._p(np.ascontiguousarray(t)), n, O._p(x), O._p(y), O._p(z),
                                     2, O._p(Es), O._p(Gs), O._p(Hs))
         reps += 1
@@ -93,7 +93,7 @@ def cpu_baseline_amips(x, y, z, tets, E_gpu, H_gpu, budget_s=12.0):
     # parity on the sample: GPU vs restatement, bit for bit
     Eo, Go, Ho = O.amips_batch(t, x, y, z, 2)
     mism_e = int(np.count_nonzero(E_gpu[:n].view(np.int64) != Eo.view(np.int64)))
-    mism_h = int(np.count_nonzero(H_gpu[:, :n].view(np.int64) != Ho.view(np.int64)))
+    mism_h = int(np.count_nonzero(H_gpu[:n].view(np.int64) != Ho.view(np.int64)))
     ref_rel = None
     if kind == "reference":
         ok = np.isfinite(E)
@@ -131,8 +131,8 @@ def main():
     tx, ty, tz = (torch.from_numpy(a).to(dev) for a in (x, y, z))
     tt = torch.from_numpy(tets).to(dev)
     E = torch.empty(n, dtype=torch.float64, device=dev)
-    G = torch.empty((3, n), dtype=torch.float64, device=dev)
-    H = torch.empty((9, n), dtype=torch.float64, device=dev)
+    G = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    H = torch.empty((n, 9), dtype=torch.float64, device=dev)
     out = (E, G, H)
 
     def step():

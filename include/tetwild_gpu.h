@@ -15,8 +15,8 @@
  *             a tet whose vertex ids fall outside [0, nv) gets NaN outputs
  *             (flip test: flipped) instead of an out-of-bounds read
  *   tets      int32 [n][4] vertex ids, 16-byte aligned          (AoS, 16 B)
- *   gradients float64 planes G[k*n + i], k = 0..2               (SoA)
- *   Hessians  float64 planes H[k*n + i], k = 0..8 row-major     (SoA)
+ *   gradients float64 [n][3]                                    (per-tet rows)
+ *   Hessians  float64 [n][9] row-major 3x3                      (per-tet rows)
  *   triangles float64 [n][9] = v0xyz v1xyz v2xyz                (AoS)
  *   points    float64 [n][3]                                    (AoS)
  *
@@ -106,6 +106,54 @@ int twg_tet_qualities(twg_ctx* ctx, const int32_t* tets, int64_t n, const double
  * flipped[i] = 1 unless the exact orientation of tet i is POSITIVE. */
 int twg_tet_flip(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x,
                  const double* y, const double* z, int64_t nv, uint8_t* flipped);
+
+/* ---- (b) envelope ------------------------------------------------------ */
+/* Builds the device AABB tree of a triangle surface.  V (nv x 3 float64) and
+ * F (nf x 3 int32) are HOST arrays; facets are Morton-ordered and the tree
+ * uses MeshFacetsAABBWithEps's implicit layout (root 1, children 2n / 2n+1,
+ * split at b + (e-b)/2; src/tetwild/geogram/mesh_AABB.cpp:356-379, the
+ * constructor of GEO::MeshFacetsAABBWithEps, which MeshRefinement::refine
+ * builds at src/tetwild/MeshRefinement.cpp:209-213).  Synchronous.  Facet ids
+ * out of [0, nv) are rejected. */
+int twg_bvh_build(twg_ctx* ctx, const double* V, int64_t nv, const int32_t* F, int64_t nf,
+                  twg_bvh** out);
+int twg_bvh_destroy(twg_bvh* bvh);
+int twg_bvh_info(const twg_bvh* bvh, int64_t* nf, int64_t* nnode, int32_t* depth, double* rmax);
+/* order[b] = caller's facet id at tree position b (HOST array of nf) */
+int twg_bvh_facet_order(const twg_bvh* bvh, int32_t* order);
+
+/* isPointOutEnvelop / isPointOutBoundaryEnvelop
+ * (src/tetwild/LocalOperations.cpp:1034-1044, 1111-1121): out[i] = 1 iff the
+ * squared distance from point i to the surface exceeds eps2.  precision
+ * TWG_MIXED traverses FP32 boxes (rounded outward) with conservative pruning;
+ * decisions are identical to TWG_FP64.  stats (device, 4 x uint64, may be NULL)
+ * accumulates samples, full traversals, boxes tested and facets tested. */
+int twg_envelope_points(twg_ctx* ctx, const twg_bvh* bvh, const double* p3, int64_t n,
+                        double eps2, int precision, uint8_t* out, unsigned long long* stats);
+
+/* isFaceOutEnvelop / isFaceOutEnvelop_sampling
+ * (src/tetwild/LocalOperations.cpp:967-975, 1046-1109): out[i] = 0 for a
+ * degenerate (exactly collinear) triangle, else 1 iff some sampleTriangle
+ * sample (spacing sampling_dist) lies farther than sqrt(eps2) from the
+ * surface. */
+int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* bvh, const double* tri9, int64_t n,
+                       double eps2, double sampling_dist, int precision, uint8_t* out,
+                       unsigned long long* stats);
+
+/* MeshFacetsAABBWithEps::nearest_facet (src/tetwild/geogram/mesh_AABB.h:130-141,
+ * mesh_AABB.cpp:381-480) with the reference's traversal order: closest point
+ * cp3[i], squared distance d2[i] and facet id (caller's numbering after
+ * twg_bvh_facet_order: facet[i] is a TREE POSITION). */
+int twg_nearest_points(twg_ctx* ctx, const twg_bvh* bvh, const double* p3, int64_t n,
+                       double* cp3, double* d2, int32_t* facet);
+
+/* sampleTriangle (src/tetwild/Common.cpp:143-228), batched in two passes:
+ * counts[i] = number of samples of triangle i; then, with offsets[i] the
+ * exclusive prefix sum of counts and total their sum, pts[offsets[i] + k]
+ * (float64 x 3) = k-th sample of triangle i in the reference's order. */
+int twg_sample_count(twg_ctx* ctx, const double* tri9, int64_t n, double d, int64_t* counts);
+int twg_sample_write(twg_ctx* ctx, const double* tri9, int64_t n, double d,
+                     const int64_t* offsets, int64_t total, double* pts);
 
 #ifdef __cplusplus
 }

@@ -166,8 +166,8 @@ static inline void gather(const int32_t* t, const double* x, const double* y, co
     }
 }
 
-/* mode 0: E; 1: E+grad; 2: E+grad+Hessian.  G and H are planar:
- * G[k*n + i], H[k*n + i] with k the row-major component index. */
+/* mode 0: E; 1: E+grad; 2: E+grad+Hessian.  G and H are per-tet rows:
+ * G[3*i + k], H[9*i + k] with k the row-major component index. */
 void orc_amips_batch(const int32_t* tets, int64_t n, const double* x, const double* y,
                      const double* z, int mode, double* E, double* G, double* H)
 {
@@ -182,11 +182,11 @@ void orc_amips_batch(const int32_t* tets, int64_t n, const double* x, const doub
             core(T, &c);
             grad(T, &c, g, u, al, be);
             E[i] = c.E;
-            for (int k = 0; k < 3; ++k) G[k * n + i] = g[k];
+            for (int k = 0; k < 3; ++k) G[3 * i + k] = g[k];
         } else {
             E[i] = orc_amips_ejh(T, g, h);
-            for (int k = 0; k < 3; ++k) G[k * n + i] = g[k];
-            for (int k = 0; k < 9; ++k) H[k * n + i] = h[k];
+            for (int k = 0; k < 3; ++k) G[3 * i + k] = g[k];
+            for (int k = 0; k < 9; ++k) H[9 * i + k] = h[k];
         }
     }
 }

@@ -100,8 +100,8 @@ def amips_batch(tets, x, y, z, mode=2):
     x, y, z = _f64(x), _f64(y), _f64(z)
     n = tets.shape[0]
     E = np.empty(n)
-    G = np.empty((3, n))
-    H = np.empty((9, n))
+    G = np.empty((n, 3))
+    H = np.empty((n, 9))
     lib().orc_amips_batch(_p(tets), n, _p(x), _p(y), _p(z), mode, _p(E), _p(G), _p(H))
     return E, G, H
 

@@ -83,7 +83,7 @@ def main():
     t0 = time.perf_counter()
     O.ref().ref_amips_batch_ejh(O._p(TT), n, O._p(E2), O._p(G2), O._p(H2))
     t_ref = time.perf_counter() - t0
-    Es = np.empty(n); Gs = np.empty((3, n)); Hs = np.empty((9, n))
+    Es = np.empty(n); Gs = np.empty((n, 3)); Hs = np.empty((n, 9))
     t0 = time.perf_counter()
     O.lib().orc_amips_batch(O._p(np.ascontiguousarray(t)), n, O._p(x), O._p(y), O._p(z), 2,
                             O._p(Es), O._p(Gs), O._p(Hs))

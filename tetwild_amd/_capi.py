@@ -40,6 +40,15 @@ SIGNATURES = {
     "twg_amips_eval_mixed": [P, P, i64, P, P, P, i64, i32, f64, P, P, P, P],
     "twg_tet_qualities": [P, P, i64, P, P, P, i64, i32, f64, P, P],
     "twg_tet_flip": [P, P, i64, P, P, P, i64, P],
+    "twg_bvh_build": [P, P, i64, P, i64, ctypes.POINTER(P)],
+    "twg_bvh_destroy": [P],
+    "twg_bvh_info": [P, P, P, P, P],
+    "twg_bvh_facet_order": [P, P],
+    "twg_envelope_points": [P, P, P, i64, f64, i32, P, P],
+    "twg_envelope_faces": [P, P, P, i64, f64, f64, i32, P, P],
+    "twg_nearest_points": [P, P, P, i64, P, P, P],
+    "twg_sample_count": [P, P, i64, f64, P],
+    "twg_sample_write": [P, P, i64, f64, P, i64, P],
 }
 _RESTYPES = {"twg_last_error": ctypes.c_char_p, "twg_ctx_stream": P}
 

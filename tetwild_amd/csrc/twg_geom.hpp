@@ -1,0 +1,315 @@
+// twg_geom.hpp -- geometry of the envelope test: vec3 helpers with geogram's
+// evaluation order, Eberly's point-triangle distance, triangle sampling and
+// point-box distances.  FP64 expressions follow the same operation order as
+// the host restatement, so their results are bit-identical.
+#pragma once
+
+#include "twg_math.hpp"
+
+namespace twg {
+
+template <typename R>
+struct V3 {
+    R x, y, z;
+};
+
+template <typename R>
+TWG_HD V3<R> mk3(R x, R y, R z)
+{
+    V3<R> r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+template <typename R> TWG_HD V3<R> operator-(V3<R> a, V3<R> b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename R> TWG_HD V3<R> operator+(V3<R> a, V3<R> b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename R> TWG_HD V3<R> mulv(V3<R> a, R s) { return mk3(a.x * s, a.y * s, a.z * s); }
+template <typename R> TWG_HD V3<R> divv(V3<R> a, R s) { return mk3(a.x / s, a.y / s, a.z / s); }
+template <typename R> TWG_HD R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R> TWG_HD R len2(V3<R> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+TWG_HD double len(V3<double> a) { return __builtin_sqrt(len2(a)); }
+TWG_HD double dist(V3<double> a, V3<double> b) { return len(a - b); }
+template <typename R>
+TWG_HD V3<R> cross(V3<R> a, V3<R> b)
+{
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// geogram normalize: v * (1 / |v|) unless |v| <= 1e-30
+TWG_HD V3<double> normalize(V3<double> v)
+{
+    double s = len(v);
+    if (s > 1e-30) s = 1.0 / s;
+    return mulv(v, s);
+}
+
+template <typename R>
+TWG_HD R absr(R v) { return v < R(0) ? -v : v; }
+
+// ---- point - segment / point - triangle squared distance ---------------
+template <typename R>
+TWG_HD R point_seg_d2(V3<R> p, V3<R> a, V3<R> b, V3<R>& cp)
+{
+    R l2 = len2(a - b);
+    R t = dot(p - a, b - a);
+    if (t <= R(0) || l2 == R(0)) {
+        cp = a;
+        return len2(p - a);
+    }
+    if (t > l2) {
+        cp = b;
+        return len2(p - b);
+    }
+    R l1 = t / l2;
+    R l0 = R(1) - l1;
+    cp = mulv(a, l0) + mulv(b, l1);
+    return len2(p - cp);
+}
+
+// Eberly, "Distance Between Point and Triangle in 3D" (region method); the
+// method behind geogram's point_triangle_squared_distance.
+template <typename R, bool WANT_CP>
+TWG_HD R point_tri_d2(V3<R> p, V3<R> V0, V3<R> V1, V3<R> V2, V3<R>& cp)
+{
+    V3<R> diff = V0 - p;
+    V3<R> e0 = V1 - V0;
+    V3<R> e1 = V2 - V0;
+    R a00 = len2(e0);
+    R a01 = dot(e0, e1);
+    R a11 = len2(e1);
+    R b0 = dot(diff, e0);
+    R b1 = dot(diff, e1);
+    R c = len2(diff);
+    R det = absr(a00 * a11 - a01 * a01);
+    R s = a01 * b1 - a11 * b0;
+    R t = a01 * b0 - a00 * b1;
+    R d2;
+    const R one = R(1), zero = R(0), two = R(2);
+
+    if (det < R(1e-30)) {
+        V3<R> q;
+        R best = point_seg_d2(p, V0, V1, cp);
+        R cur = point_seg_d2(p, V0, V2, q);
+        if (cur < best) { best = cur; cp = q; }
+        cur = point_seg_d2(p, V1, V2, q);
+        if (cur < best) { best = cur; cp = q; }
+        return best;
+    }
+    if (s + t <= det) {
+        if (s < zero) {
+            if (t < zero) {
+                if (b0 < zero) {
+                    t = zero;
+                    if (-b0 >= a00) { s = one; d2 = a00 + two * b0 + c; }
+                    else { s = -b0 / a00; d2 = b0 * s + c; }
+                } else {
+                    s = zero;
+                    if (b1 >= zero) { t = zero; d2 = c; }
+                    else if (-b1 >= a11) { t = one; d2 = a11 + two * b1 + c; }
+                    else { t = -b1 / a11; d2 = b1 * t + c; }
+                }
+            } else {
+                s = zero;
+                if (b1 >= zero) { t = zero; d2 = c; }
+                else if (-b1 >= a11) { t = one; d2 = a11 + two * b1 + c; }
+                else { t = -b1 / a11; d2 = b1 * t + c; }
+            }
+        } else if (t < zero) {
+            t = zero;
+            if (b0 >= zero) { s = zero; d2 = c; }
+            else if (-b0 >= a00) { s = one; d2 = a00 + two * b0 + c; }
+            else { s = -b0 / a00; d2 = b0 * s + c; }
+        } else {
+            R inv = one / det;
+            s *= inv;
+            t *= inv;
+            d2 = s * (a00 * s + a01 * t + two * b0) + t * (a01 * s + a11 * t + two * b1) + c;
+        }
+    } else {
+        if (s < zero) {
+            R tmp0 = a01 + b0;
+            R tmp1 = a11 + b1;
+            if (tmp1 > tmp0) {
+                R numer = tmp1 - tmp0;
+                R denom = a00 - two * a01 + a11;
+                if (numer >= denom) { s = one; t = zero; d2 = a00 + two * b0 + c; }
+                else {
+                    s = numer / denom;
+                    t = one - s;
+                    d2 = s * (a00 * s + a01 * t + two * b0) + t * (a01 * s + a11 * t + two * b1) + c;
+                }
+            } else {
+                s = zero;
+                if (tmp1 <= zero) { t = one; d2 = a11 + two * b1 + c; }
+                else if (b1 >= zero) { t = zero; d2 = c; }
+                else { t = -b1 / a11; d2 = b1 * t + c; }
+            }
+        } else if (t < zero) {
+            R tmp0 = a01 + b1;
+            R tmp1 = a00 + b0;
+            if (tmp1 > tmp0) {
+                R numer = tmp1 - tmp0;
+                R denom = a00 - two * a01 + a11;
+                if (numer >= denom) { t = one; s = zero; d2 = a11 + two * b1 + c; }
+                else {
+                    t = numer / denom;
+                    s = one - t;
+                    d2 = s * (a00 * s + a01 * t + two * b0) + t * (a01 * s + a11 * t + two * b1) + c;
+                }
+            } else {
+                t = zero;
+                if (tmp1 <= zero) { s = one; d2 = a00 + two * b0 + c; }
+                else if (b0 >= zero) { s = zero; d2 = c; }
+                else { s = -b0 / a00; d2 = b0 * s + c; }
+            }
+        } else {
+            R numer = a11 + b1 - a01 - b0;
+            if (numer <= zero) { s = zero; t = one; d2 = a11 + two * b1 + c; }
+            else {
+                R denom = a00 - two * a01 + a11;
+                if (numer >= denom) { s = one; t = zero; d2 = a00 + two * b0 + c; }
+                else {
+                    s = numer / denom;
+                    t = one - s;
+                    d2 = s * (a00 * s + a01 * t + two * b0) + t * (a01 * s + a11 * t + two * b1) + c;
+                }
+            }
+        }
+    }
+    if (d2 < zero) d2 = zero;
+    if (WANT_CP) cp = (V0 + mulv(e0, s)) + mulv(e1, t);
+    return d2;
+}
+
+// ---- sampleTriangle (src/tetwild/Common.cpp:143-228) -------------------
+// Calls emit(p) for every sample in the reference's generation order; emit
+// returns true to stop early.  Returns true iff stopped early.
+template <class Emit>
+__device__ __forceinline__ bool for_each_sample(const V3<double> vs[3], double d, Emit&& emit)
+{
+    const double sqrt3_2 = __builtin_sqrt(3.0) / 2;
+    double ls[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ls[i] = len2(vs[i] - vs[(i + 1) % 3]);
+    int max_i = 0;
+    for (int i = 1; i < 3; ++i)
+        if (!(ls[i] < ls[max_i])) max_i = i;  // last largest, as std::minmax_element
+    double N = __builtin_sqrt(ls[max_i]) / d;
+    if (N <= 1) {
+        for (int i = 0; i < 3; ++i)
+            if (emit(vs[i])) return true;
+        return false;
+    }
+    if (N == (int)N) N -= 1;
+    V3<double> v0 = vs[max_i], v1 = vs[(max_i + 1) % 3], v2 = vs[(max_i + 2) % 3];
+    V3<double> n_v0v1 = normalize(v1 - v0);
+    for (int n = 0; n <= N; n++)
+        if (emit(v0 + mulv(mulv(n_v0v1, d), (double)n))) return true;
+    if (emit(v1)) return true;
+
+    V3<double> e01 = v1 - v0;
+    double h = dist(divv(mulv(e01, dot(v2 - v0, e01)), ls[max_i]) + v0, v2);
+    int M = (int)(h / (sqrt3_2 * d));
+    if (M < 1) return emit(v2);
+
+    V3<double> n_v0v2 = normalize(v2 - v0);
+    V3<double> n_v1v2 = normalize(v2 - v1);
+    double sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
+    double tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
+    double sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
+    for (int m = 1; m <= M; m++) {
+        int n = (int)(sqrt3_2 / tan_v0 * m + 0.5);
+        int n1 = (int)(sqrt3_2 / tan_v0 * m);
+        if (m % 2 == 0 && n == n1) n += 1;
+        V3<double> v0_m = v0 + mulv(n_v0v2, m * sqrt3_2 * d / sin_v0);
+        V3<double> v1_m = v1 + mulv(n_v1v2, m * sqrt3_2 * d / sin_v1);
+        if (dist(v0_m, v1_m) <= d) break;
+        double delta_d = ((n + (m % 2) / 2.0) - m * sqrt3_2 / tan_v0) * d;
+        V3<double> v = v0_m + mulv(n_v0v1, delta_d);
+        int N1 = (int)(dist(v, v1_m) / d);
+        for (int i = 0; i <= N1; i++)
+            if (emit(v + mulv(mulv(n_v0v1, (double)i), d))) return true;
+    }
+    if (emit(v2)) return true;
+
+    N = __builtin_sqrt(ls[(max_i + 1) % 3]) / d;
+    if (N > 1) {
+        if (N == (int)N) N -= 1;
+        V3<double> nn = normalize(v2 - v1);
+        for (int n = 1; n <= N; n++)
+            if (emit(v1 + mulv(mulv(nn, d), (double)n))) return true;
+    }
+    N = __builtin_sqrt(ls[(max_i + 2) % 3]) / d;
+    if (N > 1) {
+        if (N == (int)N) N -= 1;
+        V3<double> nn = normalize(v0 - v2);
+        for (int n = 1; n <= N; n++)
+            if (emit(v2 + mulv(mulv(nn, d), (double)n))) return true;
+    }
+    return false;
+}
+
+// ---- boxes ----------------------------------------------------------------
+// Unsigned squared distance from p to an axis-aligned box (0 inside).
+template <typename R>
+TWG_HD R box_d2(V3<R> p, const R* lo, const R* hi)
+{
+    R r = R(0), d;
+    d = p.x < lo[0] ? p.x - lo[0] : (p.x > hi[0] ? p.x - hi[0] : R(0));
+    r += d * d;
+    d = p.y < lo[1] ? p.y - lo[1] : (p.y > hi[1] ? p.y - hi[1] : R(0));
+    r += d * d;
+    d = p.z < lo[2] ? p.z - lo[2] : (p.z > hi[2] ? p.z - hi[2] : R(0));
+    r += d * d;
+    return r;
+}
+
+// point_box_signed_squared_distance, src/tetwild/geogram/mesh_AABB.cpp:201-220
+TWG_HD double box_signed_d2(V3<double> p, const double* lo, const double* hi)
+{
+    double pc[3] = {p.x, p.y, p.z};
+    bool inside = true;
+    double r = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (pc[c] < lo[c]) {
+            inside = false;
+            double d = pc[c] - lo[c];
+            r += d * d;
+        } else if (pc[c] > hi[c]) {
+            inside = false;
+            double d = pc[c] - hi[c];
+            r += d * d;
+        }
+    }
+    if (inside) {
+        double d = pc[0] - lo[0];
+        double in = d * d;
+        d = pc[0] - hi[0];
+        if (d * d < in) in = d * d;
+#pragma unroll
+        for (int c = 1; c < 3; ++c) {
+            d = pc[c] - lo[c];
+            if (d * d < in) in = d * d;
+            d = pc[c] - hi[c];
+            if (d * d < in) in = d * d;
+        }
+        r = -in;
+    }
+    return r;
+}
+
+// point_box_center_squared_distance, mesh_AABB.cpp:222-238
+TWG_HD double box_center_d2(V3<double> p, const double* lo, const double* hi)
+{
+    double pc[3] = {p.x, p.y, p.z};
+    double r = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double d = pc[c] - 0.5 * (lo[c] + hi[c]);
+        r += d * d;
+    }
+    return r;
+}
+
+}  // namespace twg

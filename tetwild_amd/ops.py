@@ -76,14 +76,14 @@ class Context:
 
     # -- (a) -----------------------------------------------------------
     def amips_eval(self, tets, x, y, z, mode=C.TWG_AMIPS_EGH, out=None):
-        """E (n), G (3, n), H (9, n) float64, gradient/Hessian w.r.t. vertex 0."""
+        """E (n), G (n, 3), H (n, 9) float64, gradient/Hessian w.r.t. vertex 0."""
         n = self._tets(tets)
         nv = self._verts(x, y, z)
         dev = self.device
         if out is None:
             E = torch.empty(n, dtype=torch.float64, device=dev)
-            G = torch.empty((3, n), dtype=torch.float64, device=dev) if mode >= 1 else None
-            H = torch.empty((9, n), dtype=torch.float64, device=dev) if mode >= 2 else None
+            G = torch.empty((n, 3), dtype=torch.float64, device=dev) if mode >= 1 else None
+            H = torch.empty((n, 9), dtype=torch.float64, device=dev) if mode >= 2 else None
         else:
             E, G, H = out
         self._bind()
@@ -92,14 +92,14 @@ class Context:
         return E, G, H
 
     def amips_eval_mixed(self, tets, x, y, z, mode=C.TWG_AMIPS_EGH, tol=2.0**-16, out=None):
-        """E (n) float64, G (3, n) / H (9, n) float32, fallback (n) uint8."""
+        """E (n) float64, G (n, 3) / H (n, 9) float32, fallback (n) uint8."""
         n = self._tets(tets)
         nv = self._verts(x, y, z)
         dev = self.device
         if out is None:
             E = torch.empty(n, dtype=torch.float64, device=dev)
-            G = torch.empty((3, n), dtype=torch.float32, device=dev) if mode >= 1 else None
-            H = torch.empty((9, n), dtype=torch.float32, device=dev) if mode >= 2 else None
+            G = torch.empty((n, 3), dtype=torch.float32, device=dev) if mode >= 1 else None
+            H = torch.empty((n, 9), dtype=torch.float32, device=dev) if mode >= 2 else None
             fb = torch.empty(n, dtype=torch.uint8, device=dev)
         else:
             E, G, H, fb = out
@@ -128,3 +128,98 @@ class Context:
         C.check(self.L.twg_tet_flip(self.h, _ptr(tets), n, _ptr(x), _ptr(y), _ptr(z), nv,
                                     _ptr(out)), "twg_tet_flip")
         return out
+
+    # -- (b) -----------------------------------------------------------
+    def surface(self, V, F):
+        """Device AABB tree over a triangle surface (numpy V (nv,3) float64,
+        F (nf,3) int32 on the host)."""
+        return Surface(self, V, F)
+
+    def envelope_points(self, surf, p3, eps2, precision=C.TWG_FP64, stats=False):
+        self._dev(p3, torch.float64, "p3", (None, 3))
+        n = p3.shape[0]
+        out = torch.empty(n, dtype=torch.uint8, device=self.device)
+        st = torch.zeros(4, dtype=torch.int64, device=self.device) if stats else None
+        self._bind()
+        C.check(self.L.twg_envelope_points(self.h, surf.h, _ptr(p3), n, float(eps2), precision,
+                                           _ptr(out), _ptr(st)), "twg_envelope_points")
+        return (out, st) if stats else out
+
+    def envelope_faces(self, surf, tri9, eps2, sampling_dist, precision=C.TWG_FP64, stats=False,
+                       out=None):
+        self._dev(tri9, torch.float64, "tri9", (None, 9))
+        n = tri9.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.uint8, device=self.device)
+        st = torch.zeros(4, dtype=torch.int64, device=self.device) if stats else None
+        self._bind()
+        C.check(self.L.twg_envelope_faces(self.h, surf.h, _ptr(tri9), n, float(eps2),
+                                          float(sampling_dist), precision, _ptr(out), _ptr(st)),
+                "twg_envelope_faces")
+        return (out, st) if stats else out
+
+    def nearest(self, surf, p3):
+        """closest point (n,3), squared distance (n), facet tree position (n)"""
+        self._dev(p3, torch.float64, "p3", (None, 3))
+        n = p3.shape[0]
+        cp = torch.empty((n, 3), dtype=torch.float64, device=self.device)
+        d2 = torch.empty(n, dtype=torch.float64, device=self.device)
+        fid = torch.empty(n, dtype=torch.int32, device=self.device)
+        self._bind()
+        C.check(self.L.twg_nearest_points(self.h, surf.h, _ptr(p3), n, _ptr(cp), _ptr(d2),
+                                          _ptr(fid)), "twg_nearest_points")
+        return cp, d2, fid
+
+    def sample_triangles(self, tri9, d):
+        """sampleTriangle for every row: (counts (n), offsets (n), pts (total, 3))"""
+        self._dev(tri9, torch.float64, "tri9", (None, 9))
+        n = tri9.shape[0]
+        counts = torch.empty(n, dtype=torch.int64, device=self.device)
+        self._bind()
+        C.check(self.L.twg_sample_count(self.h, _ptr(tri9), n, float(d), _ptr(counts)),
+                "twg_sample_count")
+        offsets = torch.cumsum(counts, 0) - counts
+        total = int(counts.sum().item()) if n else 0
+        pts = torch.empty((max(total, 1), 3), dtype=torch.float64, device=self.device)
+        self._bind()
+        C.check(self.L.twg_sample_write(self.h, _ptr(tri9), n, float(d), _ptr(offsets), total,
+                                        _ptr(pts)), "twg_sample_write")
+        return counts, offsets, pts[:total]
+
+
+class Surface:
+    def __init__(self, ctx, V, F):
+        import numpy as np
+        self.ctx = ctx
+        self.V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        self.F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        C.check(ctx.L.twg_bvh_build(ctx.h, self.V.ctypes.data_as(ctypes.c_void_p), self.V.shape[0],
+                                    self.F.ctypes.data_as(ctypes.c_void_p), self.F.shape[0],
+                                    ctypes.byref(h)), "twg_bvh_build")
+        self.h = h
+        nf = ctypes.c_int64()
+        nn = ctypes.c_int64()
+        dep = ctypes.c_int32()
+        rmax = ctypes.c_double()
+        C.check(ctx.L.twg_bvh_info(h, ctypes.byref(nf), ctypes.byref(nn), ctypes.byref(dep),
+                                   ctypes.byref(rmax)), "twg_bvh_info")
+        self.nf, self.nnode, self.depth, self.rmax = nf.value, nn.value, dep.value, rmax.value
+
+    def order(self):
+        import numpy as np
+        o = np.empty(self.nf, dtype=np.int32)
+        C.check(self.ctx.L.twg_bvh_facet_order(self.h, o.ctypes.data_as(ctypes.c_void_p)),
+                "twg_bvh_facet_order")
+        return o
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.ctx.L.twg_bvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
