@@ -155,6 +155,79 @@ int twg_sample_count(twg_ctx* ctx, const double* tri9, int64_t n, double d, int6
 int twg_sample_write(twg_ctx* ctx, const double* tri9, int64_t n, double d,
                      const int64_t* offsets, int64_t total, double* pts);
 
+/* ---- (c) vertex smoothing ----------------------------------------------- */
+/* Outcome of one one-ring problem (the `continue` points of the reference's
+ * per-vertex loop bodies). */
+#define TWG_SMOOTH_MOVED 0           /* moved to p_out */
+#define TWG_SMOOTH_INVALID 1         /* a ring tet is not positively oriented */
+#define TWG_SMOOTH_NOT_MOVED 2       /* NewtonsMethod returned false */
+#define TWG_SMOOTH_FLIP_END 3        /* smoothSingle: flip at the final position */
+#define TWG_SMOOTH_REJECT_FLIP 4     /* smoothSurface: flip after projection */
+#define TWG_SMOOTH_REJECT_QUALITY 5  /* smoothSurface: max energy did not improve */
+#define TWG_SMOOTH_REJECT_ENVELOPE 6 /* smoothSurface: a one-ring face left the envelope */
+#define TWG_SMOOTH_BAD_INPUT 7       /* an index of this problem is out of range */
+
+/* A batch of independent one-ring problems on a frozen mesh snapshot.  All
+ * arrays are device arrays.  Problem p smooths vertex center[p] over the tets
+ * ring_tets[ring_off[p] .. ring_off[p+1]) (conn_tets order, as the reference
+ * gathers them, src/tetwild/VertexSmoother.cpp:145-150).  kind[p]: 0 = interior
+ * vertex (smoothSingle body, :145-212), 1 = surface vertex (smoothSurface body,
+ * :282-455, projection on sf_tree), 2 = surface vertex on the boundary
+ * (projection on b_tree).  For kinds 1 and 2: old_q holds the stored
+ * slim_energy of each ring tet, tri_ab[tri_off[p] .. tri_off[p+1]) the other
+ * two vertex ids of each one-ring surface face in the reference's order
+ * (:395-402), and new_q receives the ring's new qualities when the vertex
+ * moves.  Vertices must be rounded (posf == pos); the caller keeps rings with
+ * unrounded vertices on its exact-arithmetic path. */
+typedef struct twg_smooth_desc {
+    const double* x;
+    const double* y;
+    const double* z;
+    int64_t nv;
+    int64_t n;
+    const int32_t* center;    /* [n] */
+    const int32_t* ring_off;  /* [n + 1] */
+    const int32_t* ring_tets; /* [n_ring][4], 16-byte aligned */
+    int64_t n_ring;
+    const uint8_t* kind;      /* [n] or NULL (all interior) */
+    const double* old_q;      /* [n_ring] */
+    const int32_t* tri_off;   /* [n + 1] */
+    const int32_t* tri_ab;    /* [n_tri][2] */
+    int64_t n_tri;
+    const twg_bvh* sf_tree;
+    const twg_bvh* b_tree;
+    double eps2;
+    double sampling_dist;
+    double* p_out;            /* [n][3] final position (unchanged unless MOVED) */
+    uint8_t* status;          /* [n] TWG_SMOOTH_* */
+    double* new_q;            /* [n_ring] */
+} twg_smooth_desc;
+
+/* Runs every problem of the batch in one persistent launch: the per-vertex
+ * bodies of VertexSmoother::smoothSingle / smoothSurface with NewtonsMethod,
+ * NewtonsUpdate and getNewEnergy (src/tetwild/VertexSmoother.h:23-30,
+ * src/tetwild/VertexSmoother.cpp:114-702).  Results are bit-identical to the
+ * CPU restatement for any tile (16, 32, 64 lanes per problem; 0 = tuned or
+ * default) and chunk (problems per queue pop; 0 = tuned or default).
+ * precision TWG_MIXED screens line-search energies in FP32; results are
+ * identical to TWG_FP64. */
+int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* desc, int precision, int tile,
+                     int chunk);
+/* Times every (tile, chunk) pair on this batch and remembers the fastest for
+ * batches of a similar size on this device.  The outputs are written (by the
+ * timing runs) exactly as twg_smooth_batch would write them. */
+int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* desc, int precision, int* tile,
+                        int* chunk, float* ms);
+/* The same launch captured once into a hipGraph (counter reset + persistent
+ * kernel) and replayed by twg_smooth_plan_run on the context's stream; the
+ * descriptor's arrays must stay valid while the plan lives. */
+typedef struct twg_smooth_plan twg_smooth_plan;
+int twg_smooth_plan_create(twg_ctx* ctx, const twg_smooth_desc* desc, int precision, int tile,
+                           int chunk, twg_smooth_plan** out);
+int twg_smooth_plan_run(twg_smooth_plan* plan);
+int twg_smooth_plan_info(const twg_smooth_plan* plan, int* tile, int* chunk);
+int twg_smooth_plan_destroy(twg_smooth_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

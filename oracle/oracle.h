@@ -114,6 +114,25 @@ double orc_new_energy(const int32_t* ring_tets, int m, const double* x, const do
 /* 3x3 solve by Householder QR with column pivoting: returns 0 if rank < 3 */
 int orc_solve3_qrcp(const double* A, const double* b, double* x);
 
+/* One vertex of smoothSingle (kind 0) or smoothSurface (kind 1: surface tree
+ * projection, kind 2: boundary tree projection) on a frozen mesh.  ring_tets:
+ * the m tets around v in conn_tets order; old_q: their stored slim energies;
+ * tri_ab: the other two vertices of each one-ring surface face, in the
+ * reference's rotation order.  Writes the final position (the old one unless
+ * moved) and, for a moved surface vertex, the ring's new qualities. */
+#define ORC_SMOOTH_MOVED 0
+#define ORC_SMOOTH_INVALID 1
+#define ORC_SMOOTH_NOT_MOVED 2
+#define ORC_SMOOTH_FLIP_END 3
+#define ORC_SMOOTH_REJECT_FLIP 4
+#define ORC_SMOOTH_REJECT_QUALITY 5
+#define ORC_SMOOTH_REJECT_ENVELOPE 6
+int orc_smooth_problem(int kind, int32_t v, const int32_t* ring_tets, int m, const double* x,
+                       const double* y, const double* z, const double* old_q,
+                       const int32_t* tri_ab, int ntri, const orc_surface* sf,
+                       const orc_surface* bt, double eps2, double d, double* p_out,
+                       double* new_q);
+
 #ifdef __cplusplus
 }
 #endif

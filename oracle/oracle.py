@@ -57,6 +57,8 @@ def lib():
             "orc_newtons_update": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P, P, P, P]),
             "orc_new_energy": (c_dbl, [P, c_int, P, P, P]),
             "orc_solve3_qrcp": (c_int, [P, P, P]),
+            "orc_smooth_problem": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P, c_int,
+                                           P, P, c_dbl, c_dbl, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -269,3 +271,36 @@ def solve3(A, b):
     x = np.empty(3)
     full = lib().orc_solve3_qrcp(_p(A), _p(b), _p(x))
     return bool(full), x
+
+
+def smooth_batch(B, old_q=None, sf=None, bt=None, eps2=0.0, d=1.0):
+    """Per-problem oracle over a batch dict (workloads.smoothing_batch layout):
+    returns p_out (n,3), status (n), new_q (n_ring; 0 where not written)."""
+    L = lib()
+    x, y, z = _f64(B["x"]), _f64(B["y"]), _f64(B["z"])
+    ro, to = B["ring_off"], B["tri_off"]
+    rt_all = np.ascontiguousarray(B["ring_tets"], dtype=np.int32)
+    ab_all = np.ascontiguousarray(B["tri_ab"], dtype=np.int32).reshape(-1, 2)
+    n = len(B["center"])
+    oq = np.zeros(len(rt_all)) if old_q is None else _f64(old_q)
+    p_out = np.empty((n, 3))
+    status = np.empty(n, dtype=np.uint8)
+    new_q = np.zeros(max(len(rt_all), 1))
+    for p in range(n):
+        r0, r1 = int(ro[p]), int(ro[p + 1])
+        t0, t1 = int(to[p]), int(to[p + 1])
+        rt = np.ascontiguousarray(rt_all[r0:r1])
+        ab = np.ascontiguousarray(ab_all[t0:t1]) if t1 > t0 else np.zeros((1, 2), np.int32)
+        q = np.ascontiguousarray(oq[r0:r1]) if r1 > r0 else np.zeros(1)
+        po = np.empty(3)
+        nq = np.empty(max(r1 - r0, 1))
+        s = L.orc_smooth_problem(int(B["kind"][p]), int(B["center"][p]), _p(rt), r1 - r0,
+                                 _p(x), _p(y), _p(z), _p(q), _p(ab), t1 - t0,
+                                 sf.h if sf is not None else None,
+                                 bt.h if bt is not None else None, float(eps2), float(d),
+                                 _p(po), _p(nq))
+        status[p] = s
+        p_out[p] = po
+        if s == 0 and int(B["kind"][p]) != 0:
+            new_q[r0:r1] = nq[:r1 - r0]
+    return p_out, status, new_q

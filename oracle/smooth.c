@@ -239,3 +239,65 @@ int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, 
     p_out[2] = w.p[2];
     return is_moved;
 }
+
+/* ---- one vertex of smoothSingle / smoothSurface on a frozen mesh -------- */
+/* smoothSingle body  src/tetwild/VertexSmoother.cpp:145-212 (rounded vertices)
+ * smoothSurface body src/tetwild/VertexSmoother.cpp:282-455
+ * The caller has applied the reference's vertex filters (removed, bbox,
+ * surface, locked, isolated) and supplies the boundary flag through kind. */
+static double tet_quality_at(const view* w, const int32_t* t)
+{
+    double T[12];
+    for (int j = 0; j < 4; ++j) vget(w, t[j], T + 3 * j);
+    double e;
+    if (orc_orient3d(T, T + 3, T + 6, T + 9) != 1) {
+        e = ORC_MAX_ENERGY;
+    } else {
+        e = orc_amips_energy(T);
+        if (isinf(e) || isnan(e)) e = ORC_MAX_ENERGY;
+    }
+    if (isinf(e) || isnan(e) || e <= 0) e = ORC_MAX_ENERGY;
+    return e;
+}
+
+int orc_smooth_problem(int kind, int32_t v, const int32_t* rt, int m, const double* x,
+                       const double* y, const double* z, const double* old_q,
+                       const int32_t* tri_ab, int ntri, const orc_surface* sf,
+                       const orc_surface* bt, double eps2, double d, double* p_out,
+                       double* new_q)
+{
+    view w = {x, y, z, v, {x[v], y[v], z[v]}};
+    double p0[3] = {x[v], y[v], z[v]};
+    p_out[0] = p0[0]; p_out[1] = p0[1]; p_out[2] = p0[2];
+    if (ring_flip(rt, m, &w)) return ORC_SMOOTH_INVALID;
+    double p[3];
+    if (!orc_newtons_method(rt, m, v, x, y, z, p)) return ORC_SMOOTH_NOT_MOVED;
+    if (kind == 0) {
+        memcpy(w.p, p, sizeof p);
+        if (ring_flip(rt, m, &w)) return ORC_SMOOTH_FLIP_END;
+        memcpy(p_out, p, sizeof p);
+        return ORC_SMOOTH_MOVED;
+    }
+    double pf[3], d2;
+    orc_nearest(kind == 2 ? bt : sf, p, pf, &d2);
+    memcpy(w.p, pf, sizeof pf);
+    if (ring_flip(rt, m, &w)) return ORC_SMOOTH_REJECT_FLIP;
+    double old_max = 0.0, new_max = 0.0;
+    for (int i = 0; i < m; ++i)
+        if (old_q[i] > old_max) old_max = old_q[i];
+    for (int i = 0; i < m; ++i) {
+        new_q[i] = tet_quality_at(&w, rt + 4 * i);
+        if (new_q[i] > new_max) new_max = new_q[i];
+    }
+    if (!(new_max < old_max)) return ORC_SMOOTH_REJECT_QUALITY;
+    for (int i = 0; i < ntri; ++i) {
+        double tri[9];
+        memcpy(tri, pf, sizeof pf);
+        coords(x, y, z, tri_ab[2 * i], tri + 3);
+        coords(x, y, z, tri_ab[2 * i + 1], tri + 6);
+        if (orc_collinear3(tri, tri + 3, tri + 6)) continue;
+        if (orc_face_out(sf, tri, eps2, d, NULL)) return ORC_SMOOTH_REJECT_ENVELOPE;
+    }
+    memcpy(p_out, pf, sizeof pf);
+    return ORC_SMOOTH_MOVED;
+}

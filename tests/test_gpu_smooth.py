@@ -1,0 +1,144 @@
+"""GPU parity of (c): batched one-ring smoothing through the C-ABI against the
+CPU oracle, per problem.
+
+Bar: status, final position and (moved surface vertices) the ring's new
+qualities bit-exact, for every tile width, queue chunk, FP64 and mixed, eager
+launch and hipGraph replay."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tetwild_amd import workloads as W
+from tetwild_amd.ops import SmoothBatch
+
+pytestmark = pytest.mark.gpu
+
+EPS2, D = 1e-6, 1e-3
+
+
+@pytest.fixture(scope="module")
+def surf_data():
+    V, F = W.bumpy_sphere(freq=40, seed=0)
+    Vb, Fb = W.bumpy_sphere(freq=16, seed=0)  # a second ("boundary") tree
+    return V, F, Vb, Fb
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+def check(batch, ref, B):
+    p, s, q = (t.cpu().numpy() for t in (batch.p_out, batch.status, batch.new_q))
+    rp, rs, rq = ref
+    assert np.array_equal(s, rs), f"status mismatch at {np.nonzero(s != rs)[0][:10]}"
+    assert np.array_equal(bits(p), bits(rp))
+    moved_surf = (rs == 0) & (B["kind"] > 0)
+    for i in np.nonzero(moved_surf)[0]:
+        r0, r1 = B["ring_off"][i], B["ring_off"][i + 1]
+        assert np.array_equal(bits(q[r0:r1]), bits(rq[r0:r1]))
+
+
+def test_interior_batch_all_tiles(ctx):
+    B = W.smoothing_batch(600, seed=11)
+    ref = O.smooth_batch(B)
+    batch = SmoothBatch(ctx, B)
+    for tile in (16, 32, 64):
+        for chunk in (1, 4):
+            for prec in (0, 1):
+                print(f"tile={tile} chunk={chunk} precision={prec}", flush=True)
+                batch.p_out.zero_()
+                batch.status.fill_(255)
+                batch.run(precision=prec, tile=tile, chunk=chunk)
+                torch.cuda.synchronize()
+                check(batch, ref, B)
+    assert (ref[1] == 0).mean() > 0.8
+
+
+def test_surface_batch(ctx, surf_data):
+    V, F, Vb, Fb = surf_data
+    S = O.Surface(V, F)
+    sf = ctx.surface(V, F)
+    bt = ctx.surface(Vb, Fb)
+    Sb = O.Surface(Vb, Fb[bt.order()])
+    S_sf_order = O.Surface(V, F[sf.order()])
+    B = W.smoothing_batch(400, seed=12, surface_frac=0.5, V=V, F=F, boundary_frac=0.3)
+    q = O.tet_qualities(B["ring_tets"], B["x"], B["y"], B["z"])
+    # the oracle projects with trees in the device's facet order (ties)
+    ref = O.smooth_batch(B, q, S_sf_order, Sb, EPS2, D)
+    batch = SmoothBatch(ctx, B, sf=sf, bt=bt, eps2=EPS2, sampling_dist=D, old_q=q)
+    for tile, prec in ((16, 0), (32, 1), (64, 0)):
+        print(f"tile={tile} precision={prec}", flush=True)
+        batch.status.fill_(255)
+        batch.new_q.zero_()
+        batch.run(precision=prec, tile=tile, chunk=2)
+        torch.cuda.synchronize()
+        check(batch, ref, B)
+    st = ref[1]
+    print("statuses", np.bincount(st, minlength=8))
+    assert ((st == 0) & (B["kind"] == 1)).any() and ((st == 0) & (B["kind"] == 2)).any()
+    del S
+
+
+def test_edge_statuses(ctx, surf_data):
+    B = W.smoothing_batch(40, seed=13)
+    # problem 0: invert one ring tet -> INVALID
+    r0 = B["ring_off"][0]
+    B["ring_tets"][r0, [1, 2]] = B["ring_tets"][r0, [2, 1]]
+    # problem 1: centre already at the Newton result -> NOT_MOVED (or a tiny move)
+    p1 = O.smooth_batch({k: v for k, v in B.items()})[0][1]
+    c1 = B["center"][1]
+    B["x"][c1], B["y"][c1], B["z"][c1] = p1
+    ref = O.smooth_batch(B)  # before the out-of-range edits below
+    # problem 2: vertex id out of range -> BAD_INPUT
+    B["ring_tets"][B["ring_off"][2] + 1, 3] = len(B["x"]) + 5
+    # problem 3: centre id out of range -> BAD_INPUT
+    B["center"][3] = -7
+    good = np.ones(40, bool)
+    good[[2, 3]] = False
+    batch = SmoothBatch(ctx, B)
+    batch.run(tile=16)
+    torch.cuda.synchronize()
+    st = batch.status.cpu().numpy()
+    assert st[0] == 1 and st[2] == 7 and st[3] == 7
+    assert st[1] in (0, 2)
+    assert np.array_equal(st[good], ref[1][good])
+    assert np.array_equal(bits(batch.p_out.cpu().numpy()[good]), bits(ref[0][good]))
+
+
+def test_graph_plan_replays(ctx, surf_data):
+    V, F, _, _ = surf_data
+    sf = ctx.surface(V, F)
+    B = W.smoothing_batch(500, seed=14, surface_frac=0.2, V=V, F=F)
+    S = O.Surface(V, F[sf.order()])
+    q = O.tet_qualities(B["ring_tets"], B["x"], B["y"], B["z"])
+    ref = O.smooth_batch(B, q, S, S, EPS2, D)
+    batch = SmoothBatch(ctx, B, sf=sf, bt=sf, eps2=EPS2, sampling_dist=D, old_q=q)
+    plan = batch.plan(precision=1, tile=32, chunk=4)
+    for rep in range(3):
+        batch.status.fill_(255)
+        batch.p_out.zero_()
+        torch.cuda.synchronize()
+        plan.run()
+        torch.cuda.synchronize()
+        check(batch, ref, B)
+        # eager work between replays
+        batch.run(precision=0, tile=16, chunk=1)
+        torch.cuda.synchronize()
+        check(batch, ref, B)
+    plan.close()
+
+
+def test_autotune_and_empty(ctx):
+    B = W.smoothing_batch(300, seed=15)
+    ref = O.smooth_batch(B)
+    batch = SmoothBatch(ctx, B)
+    tile, chunk, ms = batch.autotune()
+    assert tile in (16, 32, 64) and chunk in (1, 4, 16) and ms > 0
+    torch.cuda.synchronize()
+    check(batch, ref, B)
+    E = W.smoothing_batch(1, seed=1)
+    E0 = {k: (v[:0] if k in ("center", "kind") else v) for k, v in E.items()}
+    E0["ring_off"] = E["ring_off"][:1]
+    E0["tri_off"] = E["tri_off"][:1]
+    SmoothBatch(ctx, E0).run()

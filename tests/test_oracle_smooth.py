@@ -1,0 +1,124 @@
+"""CPU pins of the oracle's Newton smoothing step (no GPU).
+
+Eigen's ColPivHouseholderQR is not vendored in the reference, so the solve
+is pinned by mathematics (numpy's LU solve, rank-deficient cases, the
+identity X = X0 - a H^-1 J); NewtonsUpdate is pinned against the reference's
+own compiled per-tet functions summed with the reference's rotation rule;
+NewtonsMethod by its invariants (accepted positions lower the ring energy,
+never flip a tet, and the is_moved semantics)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tetwild_amd import workloads as W
+
+
+def test_solve3_qrcp():
+    st = W.Stream(40, 0)
+    for k in range(300):
+        A = st.normal(9).reshape(3, 3)
+        if k % 3 == 0:
+            A = A @ A.T + 0.1 * np.eye(3)
+        b = st.normal(3)
+        full, x = O.solve3(A, b)
+        assert full
+        ref = np.linalg.solve(A, b)
+        cond = np.linalg.cond(A)
+        assert np.max(np.abs(x - ref)) <= 1e-14 * cond * max(1.0, np.max(np.abs(ref)))
+    # exactly rank 2: the unknown of the zero pivot column is 0, A x = b on range(A)
+    A = np.array([[2.0, 0, 0], [0, 0, 0], [0, 0, 3]])
+    b = np.array([1.0, 0.0, 6.0])
+    full, x = O.solve3(A, b)
+    assert not full
+    assert np.array_equal(x, [0.5, 0.0, 2.0])
+    # all-zero H: Eigen's rank rule compares against a zero threshold, keeps
+    # three pivots and divides by zero; NewtonsMethod then halves a
+    full, x = O.solve3(np.zeros((3, 3)), np.ones(3))
+    assert not np.all(np.isfinite(x))
+
+
+def ring_problem(seed, size=24):
+    B = W.smoothing_batch(1, seed=seed)
+    return B
+
+
+def test_newtons_update_vs_reference_functions():
+    R = O.ref()
+    if R is None:
+        pytest.skip("reference AMIPS library not built here")
+    for seed in range(20):
+        B = W.smoothing_batch(1, seed=seed)
+        rt, v = B["ring_tets"], int(B["center"][0])
+        x, y, z = B["x"], B["y"], B["z"]
+        ok, E, J, H, X0 = O.newtons_update(rt, v, x, y, z)
+        assert ok
+        # reference per-tet functions, tets rotated so v comes first, summed in ring order
+        Er, Jr, Hr = 0.0, np.zeros(3), np.zeros(9)
+        for t in rt:
+            s = list(t).index(v)
+            T = np.array([[x[t[(s + j) % 4]], y[t[(s + j) % 4]], z[t[(s + j) % 4]]]
+                          for j in range(4)]).reshape(12)
+            e, g, h = O.ref_ejh_T(T)
+            Er += e[0]
+            Jr += g[0]
+            Hr += h[0]
+        scale = Er / 0.01
+        assert abs(E - Er) <= 1e-11 * Er
+        assert np.max(np.abs(J - Jr)) <= 1e-10 * scale
+        assert np.max(np.abs(H.reshape(9) - Hr)) <= 1e-9 * scale / 0.01
+        assert np.array_equal(X0, [x[v], y[v], z[v]])
+
+
+def test_newtons_method_invariants():
+    B = W.smoothing_batch(200, seed=3)
+    x, y, z = B["x"], B["y"], B["z"]
+    moved_any = 0
+    for p in range(200):
+        r0, r1 = B["ring_off"][p], B["ring_off"][p + 1]
+        rt = B["ring_tets"][r0:r1]
+        v = int(B["center"][p])
+        E0 = O.new_energy(rt, x, y, z)
+        moved, pos = O.newtons_method(rt, v, x, y, z)
+        if not moved:
+            continue
+        moved_any += 1
+        x2, y2, z2 = x.copy(), y.copy(), z.copy()
+        x2[v], y2[v], z2[v] = pos
+        assert O.new_energy(rt, x2, y2, z2) < E0
+        assert not O.tet_flip(rt, x2, y2, z2).any()
+        # smoothing again from the result changes it by at most a tiny step
+        moved2, pos2 = O.newtons_method(rt, v, x2, y2, z2)
+        assert np.linalg.norm(pos2 - pos) < 1e-2 * 0.01
+    assert moved_any > 150
+
+
+def test_newton_step_identity():
+    # the first accepted full step equals X0 - H^-1 J (a = 1) when it is accepted
+    B = W.smoothing_batch(30, seed=5)
+    x, y, z = B["x"], B["y"], B["z"]
+    for p in range(30):
+        r0, r1 = B["ring_off"][p], B["ring_off"][p + 1]
+        rt = B["ring_tets"][r0:r1]
+        v = int(B["center"][p])
+        ok, E, J, H, X0 = O.newtons_update(rt, v, x, y, z)
+        full, X = O.solve3(H.reshape(9), H @ X0 - J)
+        step = X0 - np.linalg.solve(H, J)
+        assert np.max(np.abs(X - step)) <= 1e-13 * np.linalg.cond(H) * np.max(np.abs(X0))
+
+
+def test_smooth_problem_statuses():
+    V, F = W.bumpy_sphere(freq=24, seed=0)
+    S = O.Surface(V, F)
+    B = W.smoothing_batch(120, seed=7, surface_frac=0.4, V=V, F=F, boundary_frac=0.3)
+    q = O.tet_qualities(B["ring_tets"], B["x"], B["y"], B["z"])
+    p_out, status, new_q = O.smooth_batch(B, q, S, S, 1e-6, 1e-3)
+    assert set(np.unique(status)) <= {0, 1, 2, 3, 4, 5, 6}
+    assert (status == 0).sum() > 60
+    moved_surf = (status == 0) & (B["kind"] > 0)
+    assert moved_surf.any()
+    # a moved surface vertex lies on the surface (projection) and improves max energy
+    for p in np.nonzero(moved_surf)[0][:10]:
+        cp, d2, _ = S.nearest(p_out[p][None])
+        assert d2[0] < 1e-16  # on the surface up to rounding (distance < 1e-8)
+        r0, r1 = B["ring_off"][p], B["ring_off"][p + 1]
+        assert new_q[r0:r1].max() < q[r0:r1].max()

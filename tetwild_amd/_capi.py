@@ -49,8 +49,26 @@ SIGNATURES = {
     "twg_nearest_points": [P, P, P, i64, P, P, P],
     "twg_sample_count": [P, P, i64, f64, P],
     "twg_sample_write": [P, P, i64, f64, P, i64, P],
+    "twg_smooth_batch": [P, P, i32, i32, i32],
+    "twg_smooth_autotune": [P, P, i32, P, P, P],
+    "twg_smooth_plan_create": [P, P, i32, i32, i32, ctypes.POINTER(P)],
+    "twg_smooth_plan_run": [P],
+    "twg_smooth_plan_info": [P, P, P],
+    "twg_smooth_plan_destroy": [P],
 }
 _RESTYPES = {"twg_last_error": ctypes.c_char_p, "twg_ctx_stream": P}
+
+TWG_SMOOTH_STATUS = ["MOVED", "INVALID", "NOT_MOVED", "FLIP_END", "REJECT_FLIP",
+                     "REJECT_QUALITY", "REJECT_ENVELOPE", "BAD_INPUT"]
+
+
+class SmoothDesc(ctypes.Structure):
+    """twg_smooth_desc"""
+    _fields_ = [("x", P), ("y", P), ("z", P), ("nv", i64), ("n", i64), ("center", P),
+                ("ring_off", P), ("ring_tets", P), ("n_ring", i64), ("kind", P), ("old_q", P),
+                ("tri_off", P), ("tri_ab", P), ("n_tri", i64), ("sf_tree", P), ("b_tree", P),
+                ("eps2", f64), ("sampling_dist", f64), ("p_out", P), ("status", P),
+                ("new_q", P)]
 
 _LIB = None
 

@@ -2,171 +2,17 @@
 // isPointOutEnvelop, nearest_facet and sampleTriangle, over a flattened
 // implicit AABB tree resident in HBM.
 //
-// Tree: MeshFacetsAABBWithEps's implicit layout (src/tetwild/geogram/
-// mesh_AABB.cpp:356-379): root 1, children 2n and 2n+1, facet range split at
-// b + (e-b)/2, one facet per leaf, facets in Morton order.  Node n is a 64-byte
-// record (FP64 box + leaf facet id) so that the two children of a node -- the
-// pair every traversal step reads -- are one 128-byte line; the mixed path
-// reads a 32-byte FP32 copy of the boxes (rounded outward) instead.
-//
-// Decision: a sample p is in the envelope iff some facet f has
-// d2(p, f) <= eps_2 (d2 = the FP64 Eberly distance, bit-identical to the host
-// restatement).  The traversal prunes a box only when its squared distance
-// exceeds eps_2 by a margin larger than the rounding error of any facet
-// distance inside it, so the result does not depend on traversal order, on
-// the facet-hint shortcut (src/tetwild/LocalOperations.cpp:1078-1098), or on
-// FP32 boxes.  One thread per query; the traversal is stackless (a bit per
-// tree level marks a pending far child).
+// The tree layout and the decision rule are described in twg_bvh.hpp.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
-#include "twg_geom.hpp"
+#include "twg_bvh.hpp"
 #include "twg_internal.hpp"
-
-struct BvhNode64 {
-    double lo[3];
-    double hi[3];
-    int32_t leaf;  // facet position for leaves, -1 internal
-    int32_t pad[3];
-};
-struct BvhNode32 {
-    float lo[3];
-    float hi[3];
-    int32_t leaf;
-    int32_t pad;
-};
-static_assert(sizeof(BvhNode64) == 64, "node64");
-static_assert(sizeof(BvhNode32) == 32, "node32");
-
-struct twg_bvh {
-    int device = 0;
-    int64_t nf = 0, nnode = 0;
-    int depth = 0;
-    double rmax = 0.0;
-    BvhNode64* node64 = nullptr;
-    BvhNode32* node32 = nullptr;
-    double* tri = nullptr;  // nf x 9, tree order
-    std::vector<int32_t> order;  // tree position -> caller's facet id
-};
 
 namespace twg {
 namespace {
-
-struct BvhView {
-    const BvhNode64* __restrict__ n64;
-    const BvhNode32* __restrict__ n32;
-    const double* __restrict__ tri;
-    int64_t nf;
-};
-
-__device__ __forceinline__ V3<double> ld3(const double* p) { return mk3(p[0], p[1], p[2]); }
-
-__device__ __forceinline__ double facet_d2(const BvhView& B, int f, V3<double> p)
-{
-    const double* t = B.tri + 9 * (int64_t)f;
-    V3<double> cp;
-    return point_tri_d2<double, false>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
-}
-
-struct Stats {
-    unsigned long long samples = 0, traversals = 0, boxes = 0, leaves = 0;
-};
-
-// Tf: FP32 pruning threshold, conservative for the rounding of p to FP32 and
-// of the FP32 box distance (see the header comment of this file).
-__device__ __forceinline__ float f32_threshold(V3<double> p, double sqrtT)
-{
-    double pm = fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
-    double ep = 2.0e-7 * pm;
-    double t = (sqrtT + ep) * (sqrtT + ep) * (1.0 + 0x1p-19);
-    float tf = (float)t;
-    if ((double)tf < t) tf = u2f32(f2u(tf) + 1u);  // t > 0: next float up
-    return tf;
-}
-
-template <bool F32, bool STATS>
-__device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double T, double sqrtT,
-                            int& hint, Stats& st)
-{
-    if (hint >= 0) {
-        if (STATS) st.leaves++;
-        if (facet_d2(B, hint, p) <= eps2) return true;
-    }
-    if (STATS) st.traversals++;
-    if (B.nf == 1) {
-        if (STATS) st.leaves++;
-        bool in = facet_d2(B, 0, p) <= eps2;
-        if (in) hint = 0;
-        return in;
-    }
-    V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
-    float Tf = F32 ? f32_threshold(p, sqrtT) : 0.f;
-    uint32_t trail = 0;
-    int depth = 0;
-    int64_t n = 1;
-    // the root box
-    bool ok;
-    if (F32) ok = box_d2(pf, B.n32[1].lo, B.n32[1].hi) <= Tf;
-    else ok = box_d2(p, B.n64[1].lo, B.n64[1].hi) <= T;
-    if (!ok) return false;
-    for (;;) {
-        // n is internal here: look at both children
-        int64_t c = 2 * n;
-        double dl, dr;
-        bool okl, okr;
-        int leafl, leafr;
-        if (F32) {
-            const BvhNode32 L = B.n32[c], R = B.n32[c + 1];
-            float fl = box_d2(pf, L.lo, L.hi), fr = box_d2(pf, R.lo, R.hi);
-            okl = fl <= Tf;
-            okr = fr <= Tf;
-            dl = fl;
-            dr = fr;
-            leafl = L.leaf;
-            leafr = R.leaf;
-        } else {
-            const BvhNode64 L = B.n64[c], R = B.n64[c + 1];
-            dl = box_d2(p, L.lo, L.hi);
-            dr = box_d2(p, R.lo, R.hi);
-            okl = dl <= T;
-            okr = dr <= T;
-            leafl = L.leaf;
-            leafr = R.leaf;
-        }
-        if (STATS) st.boxes += 2;
-        bool descend = okl || okr;
-        if (descend) {
-            bool left = okl && (!okr || dl <= dr);
-            if (okl && okr) trail |= 1u << (depth + 1);
-            n = left ? c : c + 1;
-            depth++;
-            int leaf = left ? leafl : leafr;
-            if (leaf < 0) continue;
-            if (STATS) st.leaves++;
-            if (facet_d2(B, leaf, p) <= eps2) {
-                hint = leaf;
-                return true;
-            }
-        }
-        // backtrack to the deepest pending far child
-        for (;;) {
-            if (trail == 0) return false;
-            int d = 31 - __clz(trail);
-            trail &= ~(1u << d);
-            n = (n >> (depth - d)) ^ 1;
-            depth = d;
-            int leaf = F32 ? B.n32[n].leaf : B.n64[n].leaf;
-            if (leaf < 0) break;
-            if (STATS) st.leaves++;
-            if (facet_d2(B, leaf, p) <= eps2) {
-                hint = leaf;
-                return true;
-            }
-        }
-    }
-}
 
 // XCD-aware block order: consecutive logical blocks (neighbouring queries)
 // land on the same XCD, so each XCD's L2 sees a compact region of the tree.
@@ -217,24 +63,11 @@ __global__ __launch_bounds__(kBlock) void faces_kernel(BvhView B, const double* 
     int64_t i = logical_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
-        const double* t = tri9 + 9 * i;
-        V3<double> vs[3] = {ld3(t), ld3(t + 3), ld3(t + 6)};
-        uint8_t res = 0;
-        if (!collinear3(t, t + 3, t + 6)) {
-            int hint = -1;
-            bool stopped = for_each_sample(vs, d, [&](V3<double> s) {
-                if (STATS) st.samples++;
-                return !in_envelope<F32, STATS>(B, s, eps2, T, sqrtT, hint, st);
-            });
-            res = stopped ? 1 : 0;
-        }
-        out[i] = res;
+        out[i] = face_out<F32, STATS>(B, tri9 + 9 * i, eps2, d, T, sqrtT, st) ? 1 : 0;
     }
     if (STATS) flush_stats(st, stats);
 }
 
-// nearest_facet with the reference's traversal order
-// (src/tetwild/geogram/mesh_AABB.h:130-141, mesh_AABB.cpp:381-480)
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
                                                          int64_t n, double* __restrict__ cp3,
                                                          double* __restrict__ d2out,
@@ -242,53 +75,9 @@ __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double
 {
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    V3<double> p = ld3(p3 + 3 * i);
-    // get_nearest_facet_hint: descend towards the nearer box centre
-    int64_t nd = 1;
-    while (B.n64[nd].leaf < 0) {
-        const BvhNode64& L = B.n64[2 * nd];
-        const BvhNode64& R = B.n64[2 * nd + 1];
-        nd = box_center_d2(p, L.lo, L.hi) < box_center_d2(p, R.lo, R.hi) ? 2 * nd : 2 * nd + 1;
-    }
-    int best = B.n64[nd].leaf;
-    V3<double> np = ld3(B.tri + 9 * (int64_t)best);
-    double sq = len2(np - p);
-    int64_t stk_n[64];
-    double stk_d[64];
-    int sp = 0;
-    stk_n[sp] = 1;
-    stk_d[sp] = -__builtin_inf();
-    sp++;
-    while (sp > 0) {
-        --sp;
-        int64_t nn = stk_n[sp];
-        double dn = stk_d[sp];
-        if (!(dn < sq)) continue;
-        int leaf = B.n64[nn].leaf;
-        if (leaf >= 0) {
-            const double* t = B.tri + 9 * (int64_t)leaf;
-            V3<double> cp;
-            double dd = point_tri_d2<double, true>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
-            if (dd < sq) {
-                best = leaf;
-                np = cp;
-                sq = dd;
-            }
-            continue;
-        }
-        const BvhNode64& L = B.n64[2 * nn];
-        const BvhNode64& R = B.n64[2 * nn + 1];
-        double dl = box_signed_d2(p, L.lo, L.hi);
-        double dr = box_signed_d2(p, R.lo, R.hi);
-        if (sp + 2 > 64) break;  // unreachable: depth <= 30 is enforced at build
-        if (dl < dr) {
-            stk_n[sp] = 2 * nn + 1; stk_d[sp] = dr; sp++;
-            stk_n[sp] = 2 * nn; stk_d[sp] = dl; sp++;
-        } else {
-            stk_n[sp] = 2 * nn; stk_d[sp] = dl; sp++;
-            stk_n[sp] = 2 * nn + 1; stk_d[sp] = dr; sp++;
-        }
-    }
+    V3<double> np;
+    double sq;
+    int best = nearest_facet(B, ld3(p3 + 3 * i), np, sq);
     cp3[3 * i + 0] = np.x;
     cp3[3 * i + 1] = np.y;
     cp3[3 * i + 2] = np.z;
@@ -393,22 +182,6 @@ float round_up(double v)
     float f = (float)v;
     if ((double)f < v) f = std::nextafter(f, INFINITY);
     return f;
-}
-
-BvhView view(const twg_bvh* b)
-{
-    BvhView v;
-    v.n64 = b->node64;
-    v.n32 = b->node32;
-    v.tri = b->tri;
-    v.nf = b->nf;
-    return v;
-}
-
-double prune_threshold(const twg_bvh* b, double eps2)
-{
-    double r = b->rmax + std::sqrt(eps2);
-    return eps2 + eps2 * 0x1p-20 + 0x1p-44 * (r * r);
 }
 
 }  // namespace
@@ -570,9 +343,9 @@ int twg_envelope_points(twg_ctx* ctx, const twg_bvh* B, const double* p3, int64_
     if (int r = check_env(ctx, B, p3, n, out, eps2, precision, "twg_envelope_points")) return r;
     if (n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
-    double T = prune_threshold(B, eps2), sT = std::sqrt(T);
+    double T = bvh_prune_threshold(B, eps2), sT = std::sqrt(T);
     dim3 g(grid_for(n)), b(kBlock);
-    BvhView v = view(B);
+    BvhView v = bvh_view(B);
     if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((points_kernel<false, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats);
         else hipLaunchKernelGGL((points_kernel<false, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats);
@@ -593,9 +366,9 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
                 "twg_envelope_faces: sampling_dist must be positive");
     if (n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
-    double T = prune_threshold(B, eps2), sT = std::sqrt(T);
+    double T = bvh_prune_threshold(B, eps2), sT = std::sqrt(T);
     dim3 g(grid_for(n)), b(kBlock);
-    BvhView v = view(B);
+    BvhView v = bvh_view(B);
     if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats);
         else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats);
@@ -614,7 +387,7 @@ int twg_nearest_points(twg_ctx* ctx, const twg_bvh* B, const double* p3, int64_t
     TWG_REQUIRE(n == 0 || (d2 && facet), "twg_nearest_points: NULL output");
     if (n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
-    hipLaunchKernelGGL(nearest_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->cur, view(B), p3,
+    hipLaunchKernelGGL(nearest_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->cur, bvh_view(B), p3,
                        n, cp3, d2, facet);
     TWG_LAUNCHED();
     return TWG_OK;

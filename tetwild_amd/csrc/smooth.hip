@@ -1,0 +1,778 @@
+// smooth.hip -- (c) the VertexSmoother step on thousands of frozen one-rings
+// in one persistent launch.
+//
+// Per problem (a vertex and the tets around it) the kernel runs the body of
+// smoothSingle (src/tetwild/VertexSmoother.cpp:145-212) or smoothSurface
+// (:282-455): the validity check, NewtonsMethod (:465-542) with
+// NewtonsUpdate (:627-702) and getNewEnergy (:544-625), and for surface
+// vertices the projection (nearest_facet), the flip and quality checks and the
+// envelope test of the one-ring surface faces.
+//
+// Mapping: a tile of TILE lanes (16, 32 or 64) owns a problem; lanes take the
+// ring's tets TILE at a time.  Sums over the ring are formed in ring order --
+// the per-tet values of a chunk go through LDS and lane q adds quantity q
+// sequentially -- so every result is independent of TILE and bit-identical
+// to the host restatement.  Tiles pull problems from an atomic queue in
+// chunks, so rings of 4..200 tets balance across the machine.  The 3x3 solve
+// runs redundantly in every lane of the tile (no broadcast needed).
+//
+// Mixed precision: the line-search energy sums are screened in FP32 with a
+// forward error bound; only sums whose accept/reject or |dE| < 1e-5 decision
+// the bound cannot settle are recomputed in FP64.  Positions, decisions and
+// statuses are identical to the FP64 path.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+#include "twg_bvh.hpp"
+#include "twg_internal.hpp"
+
+namespace twg {
+namespace {
+
+constexpr int kQ = 10;  // E, J(3), H(6 unique) per tet
+
+struct SmoothArgs {
+    const double* __restrict__ x;
+    const double* __restrict__ y;
+    const double* __restrict__ z;
+    int64_t nv;
+    const int32_t* __restrict__ center;
+    const int32_t* __restrict__ ring_off;
+    const int4* __restrict__ ring;
+    int64_t n_ring;
+    int64_t n;
+    const uint8_t* __restrict__ kind;
+    const double* __restrict__ old_q;
+    const int32_t* __restrict__ tri_off;
+    const int32_t* __restrict__ tri_ab;
+    int64_t n_tri;
+    BvhView sf, bt;
+    bool has_sf, has_bt;
+    double eps2, d, T, sqrtT;
+    double* __restrict__ p_out;
+    uint8_t* __restrict__ status;
+    double* __restrict__ new_q;
+    unsigned* counter;
+    unsigned chunk;
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int TILE>
+struct TileCtx {
+    int tl;         // lane within the tile
+    int tbase;      // first thread of the tile in the block
+    int wbase;      // first lane of the tile in the wave
+    double* red;    // block LDS, kQ doubles per thread
+};
+
+template <int TILE>
+__device__ __forceinline__ bool tile_any(const TileCtx<TILE>& c, bool pred)
+{
+    unsigned long long b = __ballot(pred);
+    unsigned long long m = TILE == 64 ? ~0ull : ((1ull << TILE) - 1ull);
+    return ((b >> c.wbase) & m) != 0ull;
+}
+
+template <int TILE>
+__device__ __forceinline__ double tile_max(double v)
+{
+    for (int off = TILE / 2; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, TILE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ V3<double> vtx(const SmoothArgs& A, int id)
+{
+    return mk3(A.x[id], A.y[id], A.z[id]);
+}
+
+// tet t with vertex v at position p; rot: v first, the rest cyclic
+__device__ __forceinline__ void load_ring_tet(const SmoothArgs& A, int4 t, int v, V3<double> p,
+                                              bool rot, Tet& T)
+{
+    int id[4] = {t.x, t.y, t.z, t.w};
+    int start = 0;
+    if (rot) {
+#pragma unroll
+        for (int j = 3; j >= 0; --j)
+            if (id[j] == v) start = j;  // first occurrence
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int k = id[(start + j) & 3];
+        V3<double> q = k == v ? p : vtx(A, k);
+        T.p[j][0] = q.x;
+        T.p[j][1] = q.y;
+        T.p[j][2] = q.z;
+    }
+}
+
+// NewtonsUpdate: sums of E, J, H over the ring (rotated tets) in ring order
+template <int TILE>
+__device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+                            int v, V3<double> pos, double& E, double J[3], double H[9])
+{
+    double acc = 0.0;
+    for (int s = 0; s < m; s += TILE) {
+        int j = s + c.tl;
+        double val[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) val[q] = 0.0;
+        if (j < m) {
+            Tet T;
+            load_ring_tet(A, rt[j], v, pos, true, T);
+            AmipsCore k = amips_core(T);
+            double g[3], u[3], al[3], be[3], h[9];
+            amips_grad(T, k, g, u, al, be);
+            amips_hess(k, u, al, be, h);
+            val[0] = k.E;
+            val[1] = g[0]; val[2] = g[1]; val[3] = g[2];
+            val[4] = h[0]; val[5] = h[1]; val[6] = h[2];
+            val[7] = h[4]; val[8] = h[5]; val[9] = h[8];
+        }
+        double* mine = c.red + (c.tbase + c.tl) * kQ;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) mine[q] = val[q];
+        wave_sync();
+        int cnt = m - s < TILE ? m - s : TILE;
+        if (c.tl < kQ)
+            for (int r = 0; r < cnt; ++r) acc += c.red[(c.tbase + r) * kQ + c.tl];
+        wave_sync();
+    }
+    E = __shfl(acc, 0, TILE);
+    J[0] = __shfl(acc, 1, TILE);
+    J[1] = __shfl(acc, 2, TILE);
+    J[2] = __shfl(acc, 3, TILE);
+    double h00 = __shfl(acc, 4, TILE), h01 = __shfl(acc, 5, TILE), h02 = __shfl(acc, 6, TILE);
+    double h11 = __shfl(acc, 7, TILE), h12 = __shfl(acc, 8, TILE), h22 = __shfl(acc, 9, TILE);
+    H[0] = h00; H[1] = h01; H[2] = h02;
+    H[3] = h01; H[4] = h11; H[5] = h12;
+    H[6] = h02; H[7] = h12; H[8] = h22;
+    if (is_inf(E)) E = kMaxEnergy;
+    if (is_nan(E) || E <= 0) return false;
+    for (int k = 0; k < 3; ++k)
+        if (!is_finite(J[k])) return false;
+    for (int k = 0; k < 9; ++k)
+        if (!is_finite(H[k])) return false;
+    return true;
+}
+
+// getNewEnergy with v at X: sum in ring order, clamped to MAX_ENERGY
+template <int TILE>
+__device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+                              int v, V3<double> X)
+{
+    double acc = 0.0;
+    for (int s = 0; s < m; s += TILE) {
+        int j = s + c.tl;
+        double e = 0.0;
+        if (j < m) {
+            Tet T;
+            load_ring_tet(A, rt[j], v, X, false, T);
+            e = amips_energy(T);
+        }
+        c.red[(c.tbase + c.tl) * kQ] = e;
+        wave_sync();
+        int cnt = m - s < TILE ? m - s : TILE;
+        if (c.tl == 0)
+            for (int r = 0; r < cnt; ++r) acc += c.red[(c.tbase + r) * kQ];
+        wave_sync();
+    }
+    double s = __shfl(acc, 0, TILE);
+    if (is_inf(s) || is_nan(s) || s <= 0 || s > kMaxEnergy) s = kMaxEnergy;
+    return s;
+}
+
+// FP32 screen of ring_energy: sum of FP32 energies and a bound on its distance
+// to the FP64 sum (per-tet bound from amips_core32 plus the summation error of
+// both sums).  ok = false when any tet's FP32 value is unusable.
+template <int TILE>
+__device__ void ring_energy32(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+                              int v, V3<double> X, double& sum, double& bound, bool& ok)
+{
+    double s = 0.0, b = 0.0;
+    bool good = true;
+    for (int base = 0; base < m; base += TILE) {
+        int j = base + c.tl;
+        if (j < m) {
+            Tet T;
+            load_ring_tet(A, rt[j], v, X, false, T);
+            float d[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                d[0][k] = (float)(T.p[1][k] - T.p[0][k]);
+                d[1][k] = (float)(T.p[2][k] - T.p[0][k]);
+                d[2][k] = (float)(T.p[3][k] - T.p[0][k]);
+            }
+            AmipsCore32 k = amips_core32(d);
+            if (!(fabsf_(k.E) < 3.0e38f) || !(k.rel < 1.0f)) good = false;
+            double e = (double)k.E;
+            s += e;
+            b += e * (double)k.rel / (1.0 - (double)k.rel);
+        }
+    }
+    // any summation order: the bound covers the difference to the sequential sum
+    for (int off = TILE / 2; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, TILE);
+        b += __shfl_xor(b, off, TILE);
+    }
+    good = !tile_any(c, !good);
+    sum = s;
+    // b bounds |E32 - E| per tet; the FP64 energies' own error is below
+    // 2^-29 of that (the 2^-20 factor covers it), and both sums carry at most
+    // m u sum|terms| of summation error (u = 2^-53; 2^-51 leaves a factor 2)
+    bound = b * (1.0 + 0x1p-20) + (double)m * 0x1p-51 * (s + b);
+    ok = good;
+}
+
+template <int TILE>
+__device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+                          int v, V3<double> X)
+{
+    bool f = false;
+    for (int s = 0; s < m; s += TILE) {
+        int j = s + c.tl;
+        if (j < m && !f) {
+            Tet T;
+            load_ring_tet(A, rt[j], v, X, false, T);
+            f = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
+        }
+    }
+    return tile_any(c, f);
+}
+
+// 3x3 Householder QR with column pivoting (see the host restatement)
+__device__ void solve3_qrcp(const double* Ain, const double* b, double* x)
+{
+    double A[3][3];
+    int perm[3] = {0, 1, 2};
+    double tau[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = Ain[3 * i + j];
+    double norm[3];
+    double maxn = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        norm[j] = __builtin_sqrt(A[0][j] * A[0][j] + A[1][j] * A[1][j] + A[2][j] * A[2][j]);
+        if (norm[j] > maxn) maxn = norm[j];
+    }
+    double thr = (maxn * 0x1p-52) * (maxn * 0x1p-52) / 3.0;
+    int rank = 3;
+    for (int k = 0; k < 3; ++k) {
+        int big = k;
+        for (int j = k + 1; j < 3; ++j)
+            if (norm[j] > norm[big]) big = j;
+        if (rank == 3 && norm[big] * norm[big] < thr * (double)(3 - k)) rank = k;
+        if (big != k) {
+            for (int i = 0; i < 3; ++i) {
+                double t = A[i][k];
+                A[i][k] = A[i][big];
+                A[i][big] = t;
+            }
+            double t = norm[k];
+            norm[k] = norm[big];
+            norm[big] = t;
+            int p = perm[k];
+            perm[k] = perm[big];
+            perm[big] = p;
+        }
+        double tail = 0.0;
+        for (int i = k + 1; i < 3; ++i) tail += A[i][k] * A[i][k];
+        double c0 = A[k][k];
+        double beta;
+        if (tail <= 0x1p-1022) {
+            tau[k] = 0.0;
+            beta = c0;
+            for (int i = k + 1; i < 3; ++i) A[i][k] = 0.0;
+        } else {
+            beta = __builtin_sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+            for (int i = k + 1; i < 3; ++i) A[i][k] = A[i][k] / (c0 - beta);
+            tau[k] = (beta - c0) / beta;
+        }
+        A[k][k] = beta;
+        for (int j = k + 1; j < 3; ++j) {
+            if (tau[k] == 0.0) break;
+            double w = A[k][j];
+            for (int i = k + 1; i < 3; ++i) w += A[i][k] * A[i][j];
+            A[k][j] -= tau[k] * w;
+            for (int i = k + 1; i < 3; ++i) A[i][j] -= tau[k] * A[i][k] * w;
+        }
+        for (int j = k + 1; j < 3; ++j) {
+            double s = 0.0;
+            for (int i = k + 1; i < 3; ++i) s += A[i][j] * A[i][j];
+            norm[j] = __builtin_sqrt(s);
+        }
+    }
+    if (rank == 0) {
+        x[0] = x[1] = x[2] = 0.0;
+        return;
+    }
+    double c[3] = {b[0], b[1], b[2]};
+    for (int k = 0; k < rank; ++k) {
+        if (tau[k] == 0.0) continue;
+        double w = c[k];
+        for (int i = k + 1; i < 3; ++i) w += A[i][k] * c[i];
+        c[k] -= tau[k] * w;
+        for (int i = k + 1; i < 3; ++i) c[i] -= tau[k] * A[i][k] * w;
+    }
+    for (int i = rank - 1; i >= 0; --i) {
+        double s = c[i];
+        for (int j = i + 1; j < rank; ++j) s -= A[i][j] * c[j];
+        c[i] = s / A[i][i];
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < rank; ++i) out[perm[i]] = c[i];
+    x[0] = out[0];
+    x[1] = out[1];
+    x[2] = out[2];
+}
+
+// NewtonsMethod; returns is_moved, pout = the final position
+template <int TILE, bool MIXED>
+__device__ bool newtons_method(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+                               int v, V3<double> p0, V3<double>& pout)
+{
+    V3<double> pos = p0;
+    bool is_moved = false;
+    for (int step = 0; step < 15; ++step) {
+        double old_energy, J[3], H[9];
+        if (!ring_update<TILE>(A, c, rt, m, v, pos, old_energy, J, H)) break;
+        V3<double> X0 = pos;
+        double a = 1.0;
+        bool taken = false;
+        bool tiny = false;  // |new - old| < 1e-5 for the accepted step
+        for (int it = 0; it < 20; ++it) {
+            double rhs[3], Xs[3];
+            rhs[0] = (H[0] * X0.x + H[1] * X0.y + H[2] * X0.z) - a * J[0];
+            rhs[1] = (H[3] * X0.x + H[4] * X0.y + H[5] * X0.z) - a * J[1];
+            rhs[2] = (H[6] * X0.x + H[7] * X0.y + H[8] * X0.z) - a * J[2];
+            solve3_qrcp(H, rhs, Xs);
+            if (!(is_finite(Xs[0]) && is_finite(Xs[1]) && is_finite(Xs[2]))) {
+                a /= 2.0;
+                continue;
+            }
+            V3<double> X = mk3(Xs[0], Xs[1], Xs[2]);
+            if (ring_flip<TILE>(A, c, rt, m, v, X)) {
+                a /= 2.0;
+                continue;
+            }
+            bool decided = false, reject = false;
+            if (MIXED) {
+                double s32, b32;
+                bool ok;
+                ring_energy32<TILE>(A, c, rt, m, v, X, s32, b32, ok);
+                // the FP64 sum lies in [s32 - b32, s32 + b32] and is finite
+                // below MAX_ENERGY when ok; decide what the interval allows
+                if (ok && s32 - b32 > 0 && s32 + b32 < kMaxEnergy) {
+                    if (s32 - b32 >= old_energy) {
+                        decided = true;
+                        reject = true;
+                    } else if (s32 + b32 < old_energy) {
+                        double lo = old_energy - (s32 + b32), hi = old_energy - (s32 - b32);
+                        if (lo >= 1e-5) { decided = true; reject = false; tiny = false; }
+                        else if (hi < 1e-5) { decided = true; reject = false; tiny = true; }
+                    }
+                }
+            }
+            if (!decided) {
+                double new_energy = ring_energy<TILE>(A, c, rt, m, v, X);
+                reject = new_energy >= old_energy || is_inf(new_energy) || is_nan(new_energy);
+                tiny = !reject && fabs_(new_energy - old_energy) < 1e-5;
+            }
+            if (reject) {
+                a /= 2.0;
+                continue;
+            }
+            taken = true;
+            pos = X;
+            break;
+        }
+        if (taken && tiny) taken = false;
+        if (!taken) {
+            is_moved = step != 0;
+            break;
+        }
+        is_moved = true;
+    }
+    pout = pos;
+    return is_moved;
+}
+
+__device__ __forceinline__ double quality_at(const Tet& T)
+{
+    double e;
+    if (orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1) {
+        e = kMaxEnergy;
+    } else {
+        e = amips_energy(T);
+        if (is_inf(e) || is_nan(e)) e = kMaxEnergy;
+    }
+    if (is_inf(e) || is_nan(e) || e <= 0) e = kMaxEnergy;
+    return e;
+}
+
+template <int TILE, bool MIXED>
+__device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64_t p)
+{
+    const int v = A.center[p];
+    const int64_t r0 = A.ring_off[p], r1 = A.ring_off[p + 1];
+    const int kind = A.kind ? A.kind[p] : 0;
+    int64_t t0 = 0, t1 = 0;
+    if (kind != 0 && A.tri_off) {
+        t0 = A.tri_off[p];
+        t1 = A.tri_off[p + 1];
+    }
+    // input validation: every index read below stays inside its array
+    bool bad = v < 0 || v >= A.nv || r0 < 0 || r1 < r0 || r1 > A.n_ring || kind > 2 ||
+               (kind != 0 && (!A.has_sf || (kind == 2 && !A.has_bt) || !A.old_q || !A.new_q ||
+                              t0 < 0 || t1 < t0 || t1 > A.n_tri));
+    const int m = bad ? 0 : (int)(r1 - r0);
+    const int4* rt = A.ring + r0;
+    if (!bad) {
+        bool bid = false;
+        for (int j = c.tl; j < m; j += TILE) {
+            int4 t = rt[j];
+            bid = bid || t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 ||
+                  t.z >= A.nv || t.w < 0 || t.w >= A.nv;
+        }
+        for (int64_t k = t0 + c.tl; k < t1; k += TILE) {
+            int a = A.tri_ab[2 * k], b = A.tri_ab[2 * k + 1];
+            bid = bid || a < 0 || a >= A.nv || b < 0 || b >= A.nv;
+        }
+        bad = tile_any(c, bid);
+    }
+    V3<double> p0 = bad ? mk3(0.0, 0.0, 0.0) : vtx(A, v);
+    uint8_t st = TWG_SMOOTH_MOVED;
+    V3<double> pfin = p0;
+    if (bad) {
+        st = TWG_SMOOTH_BAD_INPUT;
+    } else if (ring_flip<TILE>(A, c, rt, m, v, p0)) {
+        st = TWG_SMOOTH_INVALID;
+    } else {
+        V3<double> pn;
+        if (!newtons_method<TILE, MIXED>(A, c, rt, m, v, p0, pn)) {
+            st = TWG_SMOOTH_NOT_MOVED;
+        } else if (kind == 0) {
+            if (ring_flip<TILE>(A, c, rt, m, v, pn)) st = TWG_SMOOTH_FLIP_END;
+            else pfin = pn;
+        } else {
+            V3<double> pf;
+            double sq;
+            nearest_facet(kind == 2 ? A.bt : A.sf, pn, pf, sq);
+            if (ring_flip<TILE>(A, c, rt, m, v, pf)) {
+                st = TWG_SMOOTH_REJECT_FLIP;
+            } else {
+                double om = 0.0, nm = 0.0;
+                for (int j = c.tl; j < m; j += TILE) {
+                    double o = A.old_q[r0 + j];
+                    om = o > om ? o : om;
+                    Tet T;
+                    load_ring_tet(A, rt[j], v, pf, false, T);
+                    double q = quality_at(T);
+                    nm = q > nm ? q : nm;
+                }
+                om = tile_max<TILE>(om);
+                nm = tile_max<TILE>(nm);
+                if (!(nm < om)) {
+                    st = TWG_SMOOTH_REJECT_QUALITY;
+                } else {
+                    bool out = false;
+                    Stats sts;
+                    for (int64_t k = t0 + c.tl; k < t1 && !out; k += TILE) {
+                        double t9[9];
+                        t9[0] = pf.x; t9[1] = pf.y; t9[2] = pf.z;
+                        V3<double> qa = vtx(A, A.tri_ab[2 * k]), qb = vtx(A, A.tri_ab[2 * k + 1]);
+                        t9[3] = qa.x; t9[4] = qa.y; t9[5] = qa.z;
+                        t9[6] = qb.x; t9[7] = qb.y; t9[8] = qb.z;
+                        out = face_out<false, false>(A.sf, t9, A.eps2, A.d, A.T, A.sqrtT, sts);
+                    }
+                    if (tile_any(c, out)) {
+                        st = TWG_SMOOTH_REJECT_ENVELOPE;
+                    } else {
+                        pfin = pf;
+                        for (int j = c.tl; j < m; j += TILE) {
+                            Tet T;
+                            load_ring_tet(A, rt[j], v, pf, false, T);
+                            A.new_q[r0 + j] = quality_at(T);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (c.tl == 0) {
+        A.status[p] = st;
+        A.p_out[3 * p + 0] = pfin.x;
+        A.p_out[3 * p + 1] = pfin.y;
+        A.p_out[3 * p + 2] = pfin.z;
+    }
+}
+
+template <int TILE, bool MIXED>
+__global__ __launch_bounds__(kBlock) void smooth_kernel(SmoothArgs A)
+{
+    __shared__ double red[kBlock * kQ];
+    TileCtx<TILE> c;
+    c.tl = threadIdx.x % TILE;
+    c.tbase = threadIdx.x - c.tl;
+    c.wbase = (threadIdx.x % 64) - c.tl;
+    c.red = red;
+    for (;;) {
+        unsigned start = 0;
+        if (c.tl == 0) start = atomicAdd(A.counter, A.chunk);
+        start = __shfl(start, 0, TILE);
+        if ((int64_t)start >= A.n) break;
+        int64_t end = (int64_t)start + A.chunk;
+        if (end > A.n) end = A.n;
+        for (int64_t p = start; p < end; ++p) solve_problem<TILE, MIXED>(A, c, p);
+    }
+}
+
+__global__ void reset_counter(unsigned* c) { *c = 0u; }
+
+using KernelFn = void (*)(SmoothArgs);
+
+KernelFn pick(int tile, bool mixed)
+{
+    if (tile == 16) return mixed ? smooth_kernel<16, true> : smooth_kernel<16, false>;
+    if (tile == 32) return mixed ? smooth_kernel<32, true> : smooth_kernel<32, false>;
+    return mixed ? smooth_kernel<64, true> : smooth_kernel<64, false>;
+}
+
+int make_args(twg_ctx* ctx, const twg_smooth_desc* d, unsigned* counter, unsigned chunk,
+              SmoothArgs& A)
+{
+    TWG_REQUIRE(d != nullptr, "smooth: NULL descriptor");
+    TWG_REQUIRE(d->n >= 0 && d->n < (int64_t(1) << 31) - (int64_t(1) << 24),
+                "smooth: n out of range");
+    TWG_REQUIRE(d->n == 0 || (d->x && d->y && d->z && d->center && d->ring_off && d->ring_tets &&
+                              d->p_out && d->status),
+                "smooth: NULL array");
+    TWG_REQUIRE(((uintptr_t)d->ring_tets & 15) == 0, "smooth: ring_tets must be 16-byte aligned");
+    TWG_REQUIRE(d->nv >= 0 && d->n_ring >= 0 && d->n_tri >= 0, "smooth: negative size");
+    TWG_REQUIRE(!d->sf_tree || d->sf_tree->device == ctx->device, "smooth: sf_tree on another device");
+    TWG_REQUIRE(!d->b_tree || d->b_tree->device == ctx->device, "smooth: b_tree on another device");
+    TWG_REQUIRE(d->sampling_dist > 0 || !d->kind, "smooth: sampling_dist must be positive");
+    A.x = d->x;
+    A.y = d->y;
+    A.z = d->z;
+    A.nv = d->nv;
+    A.center = d->center;
+    A.ring_off = d->ring_off;
+    A.ring = reinterpret_cast<const int4*>(d->ring_tets);
+    A.n_ring = d->n_ring;
+    A.n = d->n;
+    A.kind = d->kind;
+    A.old_q = d->old_q;
+    A.tri_off = d->tri_off;
+    A.tri_ab = d->tri_ab;
+    A.n_tri = d->tri_ab ? d->n_tri : 0;
+    A.has_sf = d->sf_tree != nullptr;
+    A.has_bt = d->b_tree != nullptr;
+    BvhView none = {nullptr, nullptr, nullptr, 0};
+    A.sf = d->sf_tree ? bvh_view(d->sf_tree) : none;
+    A.bt = d->b_tree ? bvh_view(d->b_tree) : none;
+    A.eps2 = d->eps2;
+    A.d = d->sampling_dist;
+    A.T = d->sf_tree ? bvh_prune_threshold(d->sf_tree, d->eps2) : 0.0;
+    A.sqrtT = std::sqrt(A.T);
+    A.p_out = d->p_out;
+    A.status = d->status;
+    A.new_q = d->new_q;
+    A.counter = counter;
+    A.chunk = chunk;
+    return TWG_OK;
+}
+
+int grid_blocks(twg_ctx* ctx, KernelFn k)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k),
+                                                     kBlock, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return ctx->num_cus * per_cu;
+}
+
+int enqueue(twg_ctx* ctx, const SmoothArgs& A, int tile, bool mixed, hipStream_t s)
+{
+    KernelFn k = pick(tile, mixed);
+    int blocks = grid_blocks(ctx, k);
+    int64_t need = (A.n * tile + kBlock - 1) / kBlock;  // no more tiles than problems
+    if (need < blocks) blocks = (int)std::max<int64_t>(need, 1);
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, A.counter);
+    TWG_LAUNCHED();
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, s, A);
+    TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+// autotuner cache: (device, mixed, size class) -> (tile, chunk)
+std::map<std::tuple<int, bool, int>, std::pair<int, unsigned>> g_tuned;
+
+int size_class(int64_t n) { return n < 4096 ? 0 : (n < 65536 ? 1 : 2); }
+
+}  // namespace
+}  // namespace twg
+
+using namespace twg;
+
+struct twg_smooth_plan {
+    twg_ctx* ctx = nullptr;
+    SmoothArgs args;
+    unsigned* counter = nullptr;
+    int tile = 32;
+    unsigned chunk = 1;
+    bool mixed = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+};
+
+extern "C" {
+
+int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int* tile_out,
+                        int* chunk_out, float* ms_out)
+{
+    TWG_REQUIRE(ctx != nullptr, "twg_smooth_autotune: ctx is NULL");
+    TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED, "twg_smooth_autotune: bad precision");
+    if (int r = bind(ctx)) return r;
+    if (int r = ensure_scratch(ctx, 256)) return r;
+    SmoothArgs A;
+    if (int r = make_args(ctx, d, (unsigned*)ctx->scratch, 1, A)) return r;
+    const int tiles[3] = {16, 32, 64};
+    const unsigned chunks[3] = {1, 4, 16};
+    hipEvent_t e0, e1;
+    TWG_HIP(hipEventCreate(&e0));
+    TWG_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bt = 32;
+    unsigned bc = 1;
+    for (int t : tiles)
+        for (unsigned ch : chunks) {
+            A.chunk = ch;
+            if (int r = enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur)) return r;  // warm
+            TWG_HIP(hipEventRecord(e0, ctx->cur));
+            for (int rep = 0; rep < 2; ++rep)
+                if (int r = enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur)) return r;
+            TWG_HIP(hipEventRecord(e1, ctx->cur));
+            TWG_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            TWG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= 2;
+            if (ms < best) {
+                best = ms;
+                bt = t;
+                bc = ch;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_tuned[std::make_tuple(ctx->device, precision == TWG_MIXED, size_class(d->n))] = {bt, bc};
+    if (tile_out) *tile_out = bt;
+    if (chunk_out) *chunk_out = (int)bc;
+    if (ms_out) *ms_out = best;
+    return TWG_OK;
+}
+
+int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int tile, int chunk)
+{
+    TWG_REQUIRE(ctx != nullptr, "twg_smooth_batch: ctx is NULL");
+    TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED, "twg_smooth_batch: bad precision");
+    TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64, "twg_smooth_batch: tile must be 0, 16, 32 or 64");
+    TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_batch: chunk out of range");
+    if (d && d->n == 0) return TWG_OK;
+    if (int r = bind(ctx)) return r;
+    if (int r = ensure_scratch(ctx, 256)) return r;
+    auto it = g_tuned.find(std::make_tuple(ctx->device, precision == TWG_MIXED, size_class(d ? d->n : 0)));
+    int t = tile ? tile : (it != g_tuned.end() ? it->second.first : 32);
+    unsigned ch = chunk ? (unsigned)chunk : (it != g_tuned.end() ? it->second.second : 1u);
+    SmoothArgs A;
+    if (int r = make_args(ctx, d, (unsigned*)ctx->scratch, ch, A)) return r;
+    return enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur);
+}
+
+int twg_smooth_plan_create(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int tile,
+                           int chunk, twg_smooth_plan** out)
+{
+    TWG_REQUIRE(ctx && d && out, "twg_smooth_plan_create: NULL argument");
+    TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED, "twg_smooth_plan_create: bad precision");
+    TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64, "twg_smooth_plan_create: bad tile");
+    TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_plan_create: chunk out of range");
+    if (int r = bind(ctx)) return r;
+    twg_smooth_plan* P = new twg_smooth_plan();
+    P->ctx = ctx;
+    P->mixed = precision == TWG_MIXED;
+    auto it = g_tuned.find(std::make_tuple(ctx->device, P->mixed, size_class(d->n)));
+    P->tile = tile ? tile : (it != g_tuned.end() ? it->second.first : 32);
+    P->chunk = chunk ? (unsigned)chunk : (it != g_tuned.end() ? it->second.second : 1u);
+    hipError_t e = hipMalloc(&P->counter, 256);
+    if (e != hipSuccess) {
+        delete P;
+        return hip_fail(e, "twg_smooth_plan_create", __FILE__, __LINE__);
+    }
+    if (int r = make_args(ctx, d, P->counter, P->chunk, P->args)) {
+        (void)hipFree(P->counter);
+        delete P;
+        return r;
+    }
+    // capture [reset counter, persistent kernel] into a graph on a private
+    // default-priority stream
+    e = hipStreamCreateWithFlags(&P->capture_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(P->capture_stream, hipStreamCaptureModeThreadLocal);
+    int r = TWG_OK;
+    if (e == hipSuccess) r = enqueue(ctx, P->args, P->tile, P->mixed, P->capture_stream);
+    hipError_t e2 = hipStreamEndCapture(P->capture_stream, &P->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess && r == TWG_OK) e = hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess || r != TWG_OK) {
+        if (P->graph) (void)hipGraphDestroy(P->graph);
+        if (P->capture_stream) (void)hipStreamDestroy(P->capture_stream);
+        (void)hipFree(P->counter);
+        delete P;
+        return r ? r : hip_fail(e, "twg_smooth_plan_create capture", __FILE__, __LINE__);
+    }
+    *out = P;
+    return TWG_OK;
+}
+
+int twg_smooth_plan_run(twg_smooth_plan* P)
+{
+    TWG_REQUIRE(P != nullptr, "twg_smooth_plan_run: NULL plan");
+    if (int r = bind(P->ctx)) return r;
+    TWG_HIP(hipGraphLaunch(P->exec, P->ctx->cur));
+    return TWG_OK;
+}
+
+int twg_smooth_plan_info(const twg_smooth_plan* P, int* tile, int* chunk)
+{
+    TWG_REQUIRE(P != nullptr, "twg_smooth_plan_info: NULL plan");
+    if (tile) *tile = P->tile;
+    if (chunk) *chunk = (int)P->chunk;
+    return TWG_OK;
+}
+
+int twg_smooth_plan_destroy(twg_smooth_plan* P)
+{
+    if (!P) return TWG_OK;
+    (void)hipSetDevice(P->ctx->device);
+    (void)hipStreamSynchronize(P->ctx->cur);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->capture_stream) (void)hipStreamDestroy(P->capture_stream);
+    if (P->counter) (void)hipFree(P->counter);
+    delete P;
+    return TWG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// twg_bvh.hpp -- device AABB tree of the envelope surface and its traversals
+// (shared by the envelope kernels and the smoothing kernel).
+//
+// Tree: MeshFacetsAABBWithEps's implicit layout (src/tetwild/geogram/
+// mesh_AABB.cpp:356-379): root 1, children 2n and 2n+1, facet range split at
+// b + (e-b)/2, one facet per leaf, facets in Morton order.  Node n is a 64-byte
+// record (FP64 box + leaf facet id) so that the two children of a node -- the
+// pair every traversal step reads -- are one 128-byte line; the mixed path
+// reads a 32-byte FP32 copy of the boxes (rounded outward) instead.
+//
+// Decision: a sample p is in the envelope iff some facet f has
+// d2(p, f) <= eps_2 (d2 = the FP64 Eberly distance, bit-identical to the host
+// restatement).  The traversal prunes a box only when its squared distance
+// exceeds eps_2 by a margin larger than the rounding error of any facet
+// distance inside it, so the result does not depend on traversal order, on
+// the facet-hint shortcut (src/tetwild/LocalOperations.cpp:1078-1098), or on
+// FP32 boxes.  The traversal is stackless (a bit per tree level marks a
+// pending far child).
+#pragma once
+
+#include <vector>
+
+#include "twg_geom.hpp"
+
+struct BvhNode64 {
+    double lo[3];
+    double hi[3];
+    int32_t leaf;  // facet position for leaves, -1 internal
+    int32_t pad[3];
+};
+struct BvhNode32 {
+    float lo[3];
+    float hi[3];
+    int32_t leaf;
+    int32_t pad;
+};
+static_assert(sizeof(BvhNode64) == 64, "node64");
+static_assert(sizeof(BvhNode32) == 32, "node32");
+
+struct twg_bvh {
+    int device = 0;
+    int64_t nf = 0, nnode = 0;
+    int depth = 0;
+    double rmax = 0.0;
+    BvhNode64* node64 = nullptr;
+    BvhNode32* node32 = nullptr;
+    double* tri = nullptr;  // nf x 9, tree order
+    std::vector<int32_t> order;  // tree position -> caller's facet id
+};
+
+namespace twg {
+
+struct BvhView {
+    const BvhNode64* __restrict__ n64;
+    const BvhNode32* __restrict__ n32;
+    const double* __restrict__ tri;
+    int64_t nf;
+};
+
+__device__ __forceinline__ V3<double> ld3(const double* p) { return mk3(p[0], p[1], p[2]); }
+
+__device__ __forceinline__ double facet_d2(const BvhView& B, int f, V3<double> p)
+{
+    const double* t = B.tri + 9 * (int64_t)f;
+    V3<double> cp;
+    return point_tri_d2<double, false>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
+}
+
+struct Stats {
+    unsigned long long samples = 0, traversals = 0, boxes = 0, leaves = 0;
+};
+
+// Tf: FP32 pruning threshold, conservative for the rounding of p to FP32 and
+// of the FP32 box distance (see the header comment of this file).
+__device__ __forceinline__ float f32_threshold(V3<double> p, double sqrtT)
+{
+    double pm = fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
+    double ep = 2.0e-7 * pm;
+    double t = (sqrtT + ep) * (sqrtT + ep) * (1.0 + 0x1p-19);
+    float tf = (float)t;
+    if ((double)tf < t) tf = u2f32(f2u(tf) + 1u);  // t > 0: next float up
+    return tf;
+}
+
+template <bool F32, bool STATS>
+__device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double T, double sqrtT,
+                            int& hint, Stats& st)
+{
+    if (hint >= 0) {
+        if (STATS) st.leaves++;
+        if (facet_d2(B, hint, p) <= eps2) return true;
+    }
+    if (STATS) st.traversals++;
+    if (B.nf == 1) {
+        if (STATS) st.leaves++;
+        bool in = facet_d2(B, 0, p) <= eps2;
+        if (in) hint = 0;
+        return in;
+    }
+    V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
+    float Tf = F32 ? f32_threshold(p, sqrtT) : 0.f;
+    uint32_t trail = 0;
+    int depth = 0;
+    int64_t n = 1;
+    // the root box
+    bool ok;
+    if (F32) ok = box_d2(pf, B.n32[1].lo, B.n32[1].hi) <= Tf;
+    else ok = box_d2(p, B.n64[1].lo, B.n64[1].hi) <= T;
+    if (!ok) return false;
+    for (;;) {
+        // n is internal here: look at both children
+        int64_t c = 2 * n;
+        double dl, dr;
+        bool okl, okr;
+        int leafl, leafr;
+        if (F32) {
+            const BvhNode32 L = B.n32[c], R = B.n32[c + 1];
+            float fl = box_d2(pf, L.lo, L.hi), fr = box_d2(pf, R.lo, R.hi);
+            okl = fl <= Tf;
+            okr = fr <= Tf;
+            dl = fl;
+            dr = fr;
+            leafl = L.leaf;
+            leafr = R.leaf;
+        } else {
+            const BvhNode64 L = B.n64[c], R = B.n64[c + 1];
+            dl = box_d2(p, L.lo, L.hi);
+            dr = box_d2(p, R.lo, R.hi);
+            okl = dl <= T;
+            okr = dr <= T;
+            leafl = L.leaf;
+            leafr = R.leaf;
+        }
+        if (STATS) st.boxes += 2;
+        bool descend = okl || okr;
+        if (descend) {
+            bool left = okl && (!okr || dl <= dr);
+            if (okl && okr) trail |= 1u << (depth + 1);
+            n = left ? c : c + 1;
+            depth++;
+            int leaf = left ? leafl : leafr;
+            if (leaf < 0) continue;
+            if (STATS) st.leaves++;
+            if (facet_d2(B, leaf, p) <= eps2) {
+                hint = leaf;
+                return true;
+            }
+        }
+        // backtrack to the deepest pending far child
+        for (;;) {
+            if (trail == 0) return false;
+            int d = 31 - __clz(trail);
+            trail &= ~(1u << d);
+            n = (n >> (depth - d)) ^ 1;
+            depth = d;
+            int leaf = F32 ? B.n32[n].leaf : B.n64[n].leaf;
+            if (leaf < 0) break;
+            if (STATS) st.leaves++;
+            if (facet_d2(B, leaf, p) <= eps2) {
+                hint = leaf;
+                return true;
+            }
+        }
+    }
+}
+
+
+// nearest_facet with the reference's traversal order
+// (src/tetwild/geogram/mesh_AABB.h:130-141, mesh_AABB.cpp:381-480):
+// get_nearest_facet_hint, then depth-first search visiting the nearer child
+// first and pruning boxes whose signed distance is not below the best so far.
+__device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<double>& np,
+                                             double& sq)
+{
+    int64_t nd = 1;
+    while (B.n64[nd].leaf < 0) {
+        const BvhNode64& L = B.n64[2 * nd];
+        const BvhNode64& R = B.n64[2 * nd + 1];
+        nd = box_center_d2(p, L.lo, L.hi) < box_center_d2(p, R.lo, R.hi) ? 2 * nd : 2 * nd + 1;
+    }
+    int best = B.n64[nd].leaf;
+    np = ld3(B.tri + 9 * (int64_t)best);
+    sq = len2(np - p);
+    int64_t stk_n[64];
+    double stk_d[64];
+    int sp = 0;
+    stk_n[sp] = 1;
+    stk_d[sp] = -__builtin_inf();
+    sp++;
+    while (sp > 0) {
+        --sp;
+        int64_t nn = stk_n[sp];
+        double dn = stk_d[sp];
+        if (!(dn < sq)) continue;
+        int leaf = B.n64[nn].leaf;
+        if (leaf >= 0) {
+            const double* t = B.tri + 9 * (int64_t)leaf;
+            V3<double> cp;
+            double dd = point_tri_d2<double, true>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
+            if (dd < sq) {
+                best = leaf;
+                np = cp;
+                sq = dd;
+            }
+            continue;
+        }
+        const BvhNode64& L = B.n64[2 * nn];
+        const BvhNode64& R = B.n64[2 * nn + 1];
+        double dl = box_signed_d2(p, L.lo, L.hi);
+        double dr = box_signed_d2(p, R.lo, R.hi);
+        if (sp + 2 > 64) break;  // unreachable: depth <= 30 is enforced at build
+        if (dl < dr) {
+            stk_n[sp] = 2 * nn + 1; stk_d[sp] = dr; sp++;
+            stk_n[sp] = 2 * nn; stk_d[sp] = dl; sp++;
+        } else {
+            stk_n[sp] = 2 * nn; stk_d[sp] = dl; sp++;
+            stk_n[sp] = 2 * nn + 1; stk_d[sp] = dr; sp++;
+        }
+    }
+    return best;
+}
+
+// isFaceOutEnvelop_sampling for one triangle (false for a collinear one)
+template <bool F32, bool STATS>
+__device__ bool face_out(const BvhView& B, const double* t9, double eps2, double d, double T,
+                         double sqrtT, Stats& st)
+{
+    if (collinear3(t9, t9 + 3, t9 + 6)) return false;
+    V3<double> vs[3] = {ld3(t9), ld3(t9 + 3), ld3(t9 + 6)};
+    int hint = -1;
+    return for_each_sample(vs, d, [&](V3<double> s) {
+        if (STATS) st.samples++;
+        return !in_envelope<F32, STATS>(B, s, eps2, T, sqrtT, hint, st);
+    });
+}
+
+inline BvhView bvh_view(const twg_bvh* b)
+{
+    BvhView v;
+    v.n64 = b->node64;
+    v.n32 = b->node32;
+    v.tri = b->tri;
+    v.nf = b->nf;
+    return v;
+}
+
+inline double bvh_prune_threshold(const twg_bvh* b, double eps2)
+{
+    double r = b->rmax + __builtin_sqrt(eps2);
+    return eps2 + eps2 * 0x1p-20 + 0x1p-44 * (r * r);
+}
+
+}  // namespace twg

@@ -223,3 +223,110 @@ class Surface:
             self.close()
         except Exception:
             pass
+
+
+class SmoothBatch:
+    """Device-resident batch of one-ring problems (see twg_smooth_desc).
+    arrays: dict of numpy arrays as produced by workloads.smoothing_batch,
+    plus optional old_q; sf / bt: Surface objects (or None)."""
+
+    def __init__(self, ctx, arrays, sf=None, bt=None, eps2=0.0, sampling_dist=1.0, old_q=None):
+        import numpy as np
+        self.ctx = ctx
+        dev = ctx.device
+
+        def T(a, dt):
+            return torch.from_numpy(np.ascontiguousarray(a).astype(dt, copy=False)).to(dev)
+        self.x = T(arrays["x"], np.float64)
+        self.y = T(arrays["y"], np.float64)
+        self.z = T(arrays["z"], np.float64)
+        self.center = T(arrays["center"], np.int32)
+        self.ring_off = T(arrays["ring_off"], np.int32)
+        self.ring_tets = T(arrays["ring_tets"], np.int32)
+        self.kind = T(arrays["kind"], np.uint8)
+        self.tri_off = T(arrays["tri_off"], np.int32)
+        tri_ab = arrays["tri_ab"]
+        self.tri_ab = T(tri_ab if len(tri_ab) else np.zeros((1, 2)), np.int32)
+        n = len(arrays["center"])
+        nr = len(arrays["ring_tets"])
+        self.n, self.n_ring, self.n_tri = n, nr, len(tri_ab)
+        if old_q is None:
+            old_q = np.zeros(nr)
+        self.old_q = T(old_q, np.float64)
+        self.p_out = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        self.status = torch.empty(n, dtype=torch.uint8, device=dev)
+        self.new_q = torch.zeros(max(nr, 1), dtype=torch.float64, device=dev)
+        d = C.SmoothDesc()
+        d.x, d.y, d.z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        d.nv = self.x.shape[0]
+        d.n = n
+        d.center = self.center.data_ptr()
+        d.ring_off = self.ring_off.data_ptr()
+        d.ring_tets = self.ring_tets.data_ptr()
+        d.n_ring = nr
+        d.kind = self.kind.data_ptr()
+        d.old_q = self.old_q.data_ptr()
+        d.tri_off = self.tri_off.data_ptr()
+        d.tri_ab = self.tri_ab.data_ptr()
+        d.n_tri = self.n_tri
+        d.sf_tree = sf.h.value if sf is not None else None
+        d.b_tree = bt.h.value if bt is not None else None
+        d.eps2 = float(eps2)
+        d.sampling_dist = float(sampling_dist)
+        d.p_out = self.p_out.data_ptr()
+        d.status = self.status.data_ptr()
+        d.new_q = self.new_q.data_ptr()
+        self.desc = d
+        self._keep = (sf, bt)
+
+    def run(self, precision=C.TWG_FP64, tile=0, chunk=0):
+        self.ctx._bind()
+        C.check(self.ctx.L.twg_smooth_batch(self.ctx.h, ctypes.byref(self.desc), precision, tile,
+                                            chunk), "twg_smooth_batch")
+        return self.p_out, self.status, self.new_q
+
+    def autotune(self, precision=C.TWG_FP64):
+        t = ctypes.c_int()
+        c = ctypes.c_int()
+        ms = ctypes.c_float()
+        self.ctx._bind()
+        C.check(self.ctx.L.twg_smooth_autotune(self.ctx.h, ctypes.byref(self.desc), precision,
+                                               ctypes.byref(t), ctypes.byref(c), ctypes.byref(ms)),
+                "twg_smooth_autotune")
+        return t.value, c.value, ms.value
+
+    def plan(self, precision=C.TWG_FP64, tile=0, chunk=0):
+        return SmoothPlan(self, precision, tile, chunk)
+
+
+class SmoothPlan:
+    """hipGraph of one persistent smoothing launch over a SmoothBatch."""
+
+    def __init__(self, batch, precision, tile, chunk):
+        self.batch = batch
+        L = batch.ctx.L
+        h = ctypes.c_void_p()
+        batch.ctx._bind()
+        C.check(L.twg_smooth_plan_create(batch.ctx.h, ctypes.byref(batch.desc), precision, tile,
+                                         chunk, ctypes.byref(h)), "twg_smooth_plan_create")
+        self.h = h
+        t = ctypes.c_int()
+        c = ctypes.c_int()
+        C.check(L.twg_smooth_plan_info(h, ctypes.byref(t), ctypes.byref(c)), "twg_smooth_plan_info")
+        self.tile, self.chunk = t.value, c.value
+
+    def run(self):
+        self.batch.ctx._bind()
+        C.check(self.batch.ctx.L.twg_smooth_plan_run(self.h), "twg_smooth_plan_run")
+        return self.batch.p_out, self.batch.status, self.batch.new_q
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.batch.ctx.L.twg_smooth_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -339,3 +339,140 @@ def envelope_samples(V, F, n, eps=1e-3, band=0.01, seed=0):
     d = eps * (1 + st.uniform(n, -band, band))
     sgn = np.where((st.u64(n) & np.uint64(1)) == 0, 1.0, -1.0)
     return np.ascontiguousarray(P + (sgn * d)[:, None] * N)
+
+
+# ------------------------------------------------------------ one-rings --
+_HULLS = {}
+
+
+def _hull_template(k):
+    """triangulated sphere (outward faces) over k seeded points on the unit
+    sphere; 2k - 4 faces"""
+    if k not in _HULLS:
+        from scipy.spatial import ConvexHull
+        st = Stream(1000 + k, 13)
+        P = st.normal(3 * k).reshape(k, 3)
+        P /= np.linalg.norm(P, axis=1, keepdims=True)
+        h = ConvexHull(P)
+        F = h.simplices.copy()
+        n = np.cross(P[F[:, 1]] - P[F[:, 0]], P[F[:, 2]] - P[F[:, 0]])
+        flip = np.einsum("ij,ij->i", n, P[F].mean(1)) < 0
+        F[flip] = F[flip][:, [0, 2, 1]]
+        _HULLS[k] = (P, F.astype(np.int64))
+    return _HULLS[k]
+
+
+_EVEN = np.array([[0, 1, 2, 3], [0, 2, 3, 1], [0, 3, 1, 2], [1, 0, 3, 2], [1, 2, 0, 3],
+                  [1, 3, 2, 0], [2, 0, 1, 3], [2, 1, 3, 0], [2, 3, 0, 1], [3, 0, 2, 1],
+                  [3, 1, 0, 2], [3, 2, 1, 0]])
+
+
+def ring_sizes(n, seed=0, median=24.0, sigma=0.6, lo=4, hi=200):
+    st = Stream(seed, 14)
+    s = np.exp(np.log(median) + sigma * st.normal(n))
+    s = np.clip(np.round(s), lo, hi).astype(np.int64)
+    return s + (s % 2)  # a closed one-ring of a vertex has an even tet count
+
+
+def smoothing_batch(n, seed=0, surface_frac=0.2, V=None, F=None, radius=0.01, boundary_frac=0.0):
+    """n frozen one-ring problems.  Interior rings: cone from the centre over a
+    triangulated sphere of k = (size + 4) / 2 jittered points; surface rings
+    (a fraction surface_frac, needs the surface V, F): a bipyramid whose
+    equator lies on the surface around a surface point, the equator faces
+    being the one-ring surface faces.  Centres are displaced from the ring's
+    middle so that Newton moves them; every ring tet is positively oriented.
+
+    Returns a dict of numpy arrays: x, y, z, center, ring_off, ring_tets (int32
+    [n_ring, 4], tet vertex order an even permutation with the centre at a
+    random slot), kind (uint8), tri_off, tri_ab."""
+    st = Stream(seed, 15)
+    sizes = ring_sizes(n, seed)
+    u = st.uniform(n)
+    kind = np.zeros(n, dtype=np.uint8)
+    if V is not None and surface_frac > 0:
+        kind[u < surface_frac] = 1
+        kind[u < surface_frac * boundary_frac] = 2
+    sizes[kind > 0] = np.maximum(sizes[kind > 0], 6)
+    xs, ys, zs = [], [], []
+    rings = [None] * n
+    tris = [None] * n
+    nvert = 0
+    centers = np.empty(n, dtype=np.int64)
+    order = np.argsort(sizes, kind="stable")
+    if V is not None and (kind > 0).any():
+        SP, Sf, SN = surface_points(V, F, int((kind > 0).sum()), seed=seed, tag=16)
+    si = 0
+    for p in range(n):
+        s = int(sizes[p])
+        R = _rotations(st, 1)[0]
+        if kind[p] == 0:
+            k = (s + 4) // 2
+            P, Fh = _hull_template(k)
+            rad = radius * (1 + 0.1 * st.uniform(k, -1, 1))
+            Q = (P * rad[:, None]) @ R.T
+            # largest displacement keeping the centre inside every face plane
+            nrm = np.cross(Q[Fh[:, 1]] - Q[Fh[:, 0]], Q[Fh[:, 2]] - Q[Fh[:, 0]])
+            nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+            h = np.einsum("ij,ij->i", nrm, Q[Fh[:, 0]]).min()
+            dirv = st.normal(3)
+            dirv /= np.linalg.norm(dirv)
+            ctr = dirv * h * 0.5 * st.uniform(1)[0]
+            base = st.uniform(3, -1, 1)
+            pts = np.vstack([ctr, Q]) + base
+            faces = Fh + 1
+            tri_ab = np.zeros((0, 2), dtype=np.int64)
+        else:
+            ke = s // 2
+            c0 = SP[si]
+            nn = SN[si]
+            si += 1
+            a = np.where(abs(nn[0]) < 0.9, np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
+            t1 = np.cross(nn, a)
+            t1 /= np.linalg.norm(t1)
+            t2 = np.cross(nn, t1)
+            ang = 2 * np.pi * (np.arange(ke) + 0.3 * st.uniform(ke, -1, 1)) / ke
+            rr = radius * (1 + 0.1 * st.uniform(ke, -1, 1))
+            eq = c0 + rr[:, None] * (np.cos(ang)[:, None] * t1 + np.sin(ang)[:, None] * t2)
+            north = c0 + nn * radius
+            south = c0 - nn * radius
+            off = st.uniform(2, -0.15, 0.15) * radius
+            ctr = c0 + off[0] * t1 + off[1] * t2 + nn * (0.02 * radius * st.uniform(1, -1, 1)[0])
+            pts = np.vstack([ctr, eq, north, south])
+            e = np.arange(ke) + 1
+            en = np.roll(e, -1)
+            Nn, Ss = ke + 1, ke + 2
+            faces = np.concatenate([np.stack([np.full(ke, Nn), e, en], 1),
+                                    np.stack([np.full(ke, Ss), en, e], 1)])
+            # outward orientation around the centre
+            fv = pts[faces]
+            nrm = np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0])
+            flip = np.einsum("ij,ij->i", nrm, fv.mean(1) - ctr) < 0
+            faces[flip] = faces[flip][:, [0, 2, 1]]
+            # one-ring surface faces (ctr, e_i, e_i+1) in the reference's order:
+            # sorted vertex ids, then rotated to start after the centre
+            tri_ab = []
+            for i in range(ke):
+                tri = sorted([nvert, nvert + int(e[i]), nvert + int(en[i])])
+                kk = tri.index(nvert)
+                tri_ab.append((tri[(kk + 1) % 3], tri[(kk + 2) % 3]))
+            tri_ab = np.array(sorted(tri_ab, key=lambda t: sorted([nvert, t[0], t[1]])),
+                              dtype=np.int64)
+        tets = np.concatenate([np.zeros((len(faces), 1), dtype=np.int64), faces], 1) + nvert
+        perm = _EVEN[st.integers(len(tets), 0, 11)]
+        tets = np.take_along_axis(tets, perm, axis=1)
+        rings[p] = tets
+        tris[p] = tri_ab
+        centers[p] = nvert
+        xs.append(pts[:, 0]); ys.append(pts[:, 1]); zs.append(pts[:, 2])
+        nvert += len(pts)
+    ring_off = np.zeros(n + 1, dtype=np.int64)
+    ring_off[1:] = np.cumsum([len(r) for r in rings])
+    tri_off = np.zeros(n + 1, dtype=np.int64)
+    tri_off[1:] = np.cumsum([len(t) for t in tris])
+    return {
+        "x": np.concatenate(xs), "y": np.concatenate(ys), "z": np.concatenate(zs),
+        "center": centers.astype(np.int32), "ring_off": ring_off.astype(np.int32),
+        "ring_tets": np.ascontiguousarray(np.concatenate(rings).astype(np.int32)),
+        "kind": kind, "tri_off": tri_off.astype(np.int32),
+        "tri_ab": np.ascontiguousarray(np.concatenate(tris).astype(np.int32).reshape(-1, 2)),
+    }
