@@ -135,7 +135,8 @@ int twg_envelope_points(twg_ctx* ctx, const twg_bvh* bvh, const double* p3, int6
  * (src/tetwild/LocalOperations.cpp:967-975, 1046-1109): out[i] = 0 for a
  * degenerate (exactly collinear) triangle, else 1 iff some sampleTriangle
  * sample (spacing sampling_dist) lies farther than sqrt(eps2) from the
- * surface. */
+ * surface.  Guard: a triangle whose longest edge exceeds 65536 * sampling_dist
+ * is reported out without sampling (its sample count would be ~10^9). */
 int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* bvh, const double* tri9, int64_t n,
                        double eps2, double sampling_dist, int precision, uint8_t* out,
                        unsigned long long* stats);

@@ -456,6 +456,14 @@ int orc_face_out(const orc_surface* s, const double* tri9, double eps2, double d
         if (n_samples) *n_samples = 0;
         return 0;
     }
+    {   /* guard shared with the device: longest edge > 65536 d -> out, unsampled */
+        v3 a = ld(tri9), b = ld(tri9 + 3), c = ld(tri9 + 6);
+        double l = fmax(len2(sub(a, b)), fmax(len2(sub(b, c)), len2(sub(c, a))));
+        if (!(sqrt(l) / d <= 65536.0)) {
+            if (n_samples) *n_samples = 0;
+            return 1;
+        }
+    }
     double local[3 * 256];
     int64_t n = orc_sample_triangle(tri9, tri9 + 3, tri9 + 6, d, local, 256);
     double* ps = local;

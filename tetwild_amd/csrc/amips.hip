@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void amips_eval_kernel(
     double* __restrict__ G, double* __restrict__ H)
 {
     __shared__ double lds[kBlock * 9];
-    const int64_t base = (int64_t)blockIdx.x * kBlock;
+    const int64_t base = xcd_block() * kBlock;
     const int64_t i = base + threadIdx.x;
     double e = 0.0, g[3] = {0.0, 0.0, 0.0}, h[9];
 #pragma unroll
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void amips_mixed_kernel(
     uint8_t* __restrict__ fallback)
 {
     __shared__ float lds[kBlock * 9];
-    const int64_t base = (int64_t)blockIdx.x * kBlock;
+    const int64_t base = xcd_block() * kBlock;
     const int64_t i = base + threadIdx.x;
     float g[3] = {0.f, 0.f, 0.f}, h[9];
 #pragma unroll

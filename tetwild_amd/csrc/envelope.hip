@@ -14,18 +14,6 @@
 namespace twg {
 namespace {
 
-// XCD-aware block order: consecutive logical blocks (neighbouring queries)
-// land on the same XCD, so each XCD's L2 sees a compact region of the tree.
-__device__ __forceinline__ int64_t logical_block()
-{
-    const unsigned g = gridDim.x, b = blockIdx.x;
-    const unsigned per = g / 8, rem = g % 8;
-    const unsigned xcd = b % 8, k = b / 8;
-    if (k >= per) return b;  // tail beyond the last full round
-    (void)rem;
-    return (int64_t)xcd * per + k;
-}
-
 __device__ __forceinline__ void flush_stats(const Stats& st, unsigned long long* out)
 {
     unsigned long long v[4] = {st.samples, st.traversals, st.boxes, st.leaves};
@@ -43,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
                                                         double sqrtT, uint8_t* __restrict__ out,
                                                         unsigned long long* stats)
 {
-    int64_t i = logical_block() * kBlock + threadIdx.x;
+    int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
         int hint = -1;
@@ -54,19 +42,47 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
     if (STATS) flush_stats(st, stats);
 }
 
+// Triangles the degeneracy filter cannot settle are queued (pending[],
+// *n_pending) for faces_exact_kernel, which keeps the exact predicate -- and its
+// register footprint -- out of this kernel.
 template <bool F32, bool STATS>
 __global__ __launch_bounds__(kBlock) void faces_kernel(BvhView B, const double* __restrict__ tri9,
                                                        int64_t n, double eps2, double d, double T,
                                                        double sqrtT, uint8_t* __restrict__ out,
-                                                       unsigned long long* stats)
+                                                       unsigned long long* stats,
+                                                       unsigned* n_pending, int64_t* pending)
 {
-    int64_t i = logical_block() * kBlock + threadIdx.x;
+    int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
-        out[i] = face_out<F32, STATS>(B, tri9 + 9 * i, eps2, d, T, sqrtT, st) ? 1 : 0;
+        const double* t = tri9 + 9 * i;
+        if (certainly_not_collinear(t, t + 3, t + 6)) {
+            out[i] = face_out_sampled<F32, STATS>(B, t, eps2, d, T, sqrtT, st) ? 1 : 0;
+        } else {
+            unsigned k = atomicAdd(n_pending, 1u);
+            pending[k] = i;
+        }
     }
     if (STATS) flush_stats(st, stats);
 }
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void faces_exact_kernel(BvhView B,
+                                                             const double* __restrict__ tri9,
+                                                             double eps2, double d, double T,
+                                                             double sqrtT, uint8_t* __restrict__ out,
+                                                             const unsigned* n_pending,
+                                                             const int64_t* pending)
+{
+    const unsigned cnt = *n_pending;
+    Stats st;
+    for (unsigned k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+        int64_t i = pending[k];
+        out[i] = face_out<F32, false>(B, tri9 + 9 * i, eps2, d, T, sqrtT, st) ? 1 : 0;
+    }
+}
+
+__global__ void zero_u32(unsigned* p) { *p = 0u; }
 
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
                                                          int64_t n, double* __restrict__ cp3,
@@ -366,16 +382,27 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
                 "twg_envelope_faces: sampling_dist must be positive");
     if (n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
+    if (int r = ensure_scratch(ctx, 256 + sizeof(int64_t) * (size_t)n)) return r;
+    unsigned* n_pending = (unsigned*)ctx->scratch;
+    int64_t* pending = (int64_t*)((char*)ctx->scratch + 256);
     double T = bvh_prune_threshold(B, eps2), sT = std::sqrt(T);
     dim3 g(grid_for(n)), b(kBlock);
     BvhView v = bvh_view(B);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->cur, n_pending);
+    TWG_LAUNCHED();
     if (precision == TWG_FP64) {
-        if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats);
-        else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats);
+        if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
+        else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
     } else {
-        if (stats) hipLaunchKernelGGL((faces_kernel<true, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats);
-        else hipLaunchKernelGGL((faces_kernel<true, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats);
+        if (stats) hipLaunchKernelGGL((faces_kernel<true, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
+        else hipLaunchKernelGGL((faces_kernel<true, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
     }
+    TWG_LAUNCHED();
+    unsigned eg = (unsigned)std::min<int64_t>(grid_for(n), 64);
+    if (precision == TWG_FP64)
+        hipLaunchKernelGGL((faces_exact_kernel<false>), dim3(eg), b, 0, ctx->cur, v, tri9, eps2, sampling_dist, T, sT, out, n_pending, pending);
+    else
+        hipLaunchKernelGGL((faces_exact_kernel<true>), dim3(eg), b, 0, ctx->cur, v, tri9, eps2, sampling_dist, T, sT, out, n_pending, pending);
     TWG_LAUNCHED();
     return TWG_OK;
 }

@@ -220,18 +220,36 @@ __device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<
     return best;
 }
 
+// Triangles whose longest edge exceeds kMaxEdgeSamples sampling distances are
+// reported out without sampling: their sample count (quadratic in that ratio)
+// would keep one lane busy for seconds.  TetWild's faces are orders of
+// magnitude below this.
+constexpr double kMaxEdgeSamples = 65536.0;
+
+// isFaceOutEnvelop_sampling for a triangle known not to be degenerate
+template <bool F32, bool STATS>
+__device__ bool face_out_sampled(const BvhView& B, const double* t9, double eps2, double d,
+                                 double T, double sqrtT, Stats& st)
+{
+    V3<double> vs[3] = {ld3(t9), ld3(t9 + 3), ld3(t9 + 6)};
+    {
+        double l = fmax(len2(vs[0] - vs[1]), fmax(len2(vs[1] - vs[2]), len2(vs[2] - vs[0])));
+        if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
+    }
+    int hint = -1;
+    return for_each_sample(vs, d, [&](V3<double> s) {
+        if (STATS) st.samples++;
+        return !in_envelope<F32, STATS>(B, s, eps2, T, sqrtT, hint, st);
+    });
+}
+
 // isFaceOutEnvelop_sampling for one triangle (false for a collinear one)
 template <bool F32, bool STATS>
 __device__ bool face_out(const BvhView& B, const double* t9, double eps2, double d, double T,
                          double sqrtT, Stats& st)
 {
     if (collinear3(t9, t9 + 3, t9 + 6)) return false;
-    V3<double> vs[3] = {ld3(t9), ld3(t9 + 3), ld3(t9 + 6)};
-    int hint = -1;
-    return for_each_sample(vs, d, [&](V3<double> s) {
-        if (STATS) st.samples++;
-        return !in_envelope<F32, STATS>(B, s, eps2, T, sqrtT, hint, st);
-    });
+    return face_out_sampled<F32, STATS>(B, t9, eps2, d, T, sqrtT, st);
 }
 
 inline BvhView bvh_view(const twg_bvh* b)

@@ -182,70 +182,168 @@ TWG_HD R point_tri_d2(V3<R> p, V3<R> V0, V3<R> V1, V3<R> V2, V3<R>& cp)
 }
 
 // ---- sampleTriangle (src/tetwild/Common.cpp:143-228) -------------------
-// Calls emit(p) for every sample in the reference's generation order; emit
-// returns true to stop early.  Returns true iff stopped early.
+// A resumable generator: next(p) yields the samples one at a time in the
+// reference's generation order (longest edge, v1, rows parallel to it, v2,
+// then the two other edges), so a caller has a single consumption site.
+struct SampleGen {
+    V3<double> v0, v1, v2, n01, n02, n12, ne, rowv;
+    double d, sqrt3_2, ls1, ls2, N, sin_v0, tan_v0, sin_v1;
+    int phase, k, m, M, N1;
+
+    __device__ __forceinline__ void init(const V3<double> in[3], double d_)
+    {
+        d = d_;
+        const V3<double>* vs = in;
+        sqrt3_2 = __builtin_sqrt(3.0) / 2;
+        double ls[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ls[i] = len2(vs[i] - vs[(i + 1) % 3]);
+        int max_i = 0;
+        for (int i = 1; i < 3; ++i)
+            if (!(ls[i] < ls[max_i])) max_i = i;  // last largest, as std::minmax_element
+        N = __builtin_sqrt(ls[max_i]) / d;
+        k = 0;
+        if (N <= 1) {
+            v0 = vs[0];
+            v1 = vs[1];
+            v2 = vs[2];
+            phase = 9;
+            return;
+        }
+        if (N == (int)N) N -= 1;
+        v0 = vs[max_i];
+        v1 = vs[(max_i + 1) % 3];
+        v2 = vs[(max_i + 2) % 3];
+        ls1 = ls[(max_i + 1) % 3];
+        ls2 = ls[(max_i + 2) % 3];
+        n01 = normalize(v1 - v0);
+        double lsm = ls[max_i];
+        V3<double> e01 = v1 - v0;
+        double h = dist(divv(mulv(e01, dot(v2 - v0, e01)), lsm) + v0, v2);
+        M = (int)(h / (sqrt3_2 * d));
+        phase = 0;
+    }
+
+    __device__ __forceinline__ bool next(V3<double>& p)
+    {
+        for (;;) {
+            switch (phase) {
+            case 9:  // short triangle: its three vertices
+                if (k < 3) {
+                    p = k == 0 ? v0 : (k == 1 ? v1 : v2);
+                    k++;
+                    return true;
+                }
+                return false;
+            case 0:  // longest edge
+                if (k <= N) {
+                    p = v0 + mulv(mulv(n01, d), (double)k);
+                    k++;
+                    return true;
+                }
+                phase = 1;
+                p = v1;
+                return true;
+            case 1:  // after v1: rows, or straight to v2
+                if (M < 1) {
+                    phase = 8;
+                    p = v2;
+                    return true;
+                }
+                n02 = normalize(v2 - v0);
+                n12 = normalize(v2 - v1);
+                sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
+                tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
+                sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
+                m = 1;
+                phase = 2;
+                break;
+            case 2: {  // row header
+                if (m > M) {
+                    phase = 4;
+                    break;
+                }
+                int n = (int)(sqrt3_2 / tan_v0 * m + 0.5);
+                int n1 = (int)(sqrt3_2 / tan_v0 * m);
+                if (m % 2 == 0 && n == n1) n += 1;
+                V3<double> v0_m = v0 + mulv(n02, m * sqrt3_2 * d / sin_v0);
+                V3<double> v1_m = v1 + mulv(n12, m * sqrt3_2 * d / sin_v1);
+                if (dist(v0_m, v1_m) <= d) {
+                    phase = 4;
+                    break;
+                }
+                double delta_d = ((n + (m % 2) / 2.0) - m * sqrt3_2 / tan_v0) * d;
+                rowv = v0_m + mulv(n01, delta_d);
+                N1 = (int)(dist(rowv, v1_m) / d);
+                k = 0;
+                phase = 3;
+                break;
+            }
+            case 3:  // row samples
+                if (k <= N1) {
+                    p = rowv + mulv(mulv(n01, (double)k), d);
+                    k++;
+                    return true;
+                }
+                m++;
+                phase = 2;
+                break;
+            case 4:  // v2, then the edge v1 -> v2
+                p = v2;
+                N = __builtin_sqrt(ls1) / d;
+                if (N > 1) {
+                    if (N == (int)N) N -= 1;
+                    ne = normalize(v2 - v1);
+                    k = 1;
+                    phase = 5;
+                } else {
+                    phase = 6;
+                }
+                return true;
+            case 5:
+                if (k <= N) {
+                    p = v1 + mulv(mulv(ne, d), (double)k);
+                    k++;
+                    return true;
+                }
+                phase = 6;
+                break;
+            case 6:  // the edge v2 -> v0
+                N = __builtin_sqrt(ls2) / d;
+                if (N > 1) {
+                    if (N == (int)N) N -= 1;
+                    ne = normalize(v0 - v2);
+                    k = 1;
+                    phase = 7;
+                } else {
+                    return false;
+                }
+                break;
+            case 7:
+                if (k <= N) {
+                    p = v2 + mulv(mulv(ne, d), (double)k);
+                    k++;
+                    return true;
+                }
+                return false;
+            case 8:  // done after v2 (no rows)
+            default:
+                return false;
+            }
+        }
+    }
+};
+
+// Calls emit(p) for every sample in order; emit returns true to stop early.
+// Returns true iff stopped early.
 template <class Emit>
 __device__ __forceinline__ bool for_each_sample(const V3<double> vs[3], double d, Emit&& emit)
 {
-    const double sqrt3_2 = __builtin_sqrt(3.0) / 2;
-    double ls[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ls[i] = len2(vs[i] - vs[(i + 1) % 3]);
-    int max_i = 0;
-    for (int i = 1; i < 3; ++i)
-        if (!(ls[i] < ls[max_i])) max_i = i;  // last largest, as std::minmax_element
-    double N = __builtin_sqrt(ls[max_i]) / d;
-    if (N <= 1) {
-        for (int i = 0; i < 3; ++i)
-            if (emit(vs[i])) return true;
-        return false;
-    }
-    if (N == (int)N) N -= 1;
-    V3<double> v0 = vs[max_i], v1 = vs[(max_i + 1) % 3], v2 = vs[(max_i + 2) % 3];
-    V3<double> n_v0v1 = normalize(v1 - v0);
-    for (int n = 0; n <= N; n++)
-        if (emit(v0 + mulv(mulv(n_v0v1, d), (double)n))) return true;
-    if (emit(v1)) return true;
-
-    V3<double> e01 = v1 - v0;
-    double h = dist(divv(mulv(e01, dot(v2 - v0, e01)), ls[max_i]) + v0, v2);
-    int M = (int)(h / (sqrt3_2 * d));
-    if (M < 1) return emit(v2);
-
-    V3<double> n_v0v2 = normalize(v2 - v0);
-    V3<double> n_v1v2 = normalize(v2 - v1);
-    double sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
-    double tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
-    double sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
-    for (int m = 1; m <= M; m++) {
-        int n = (int)(sqrt3_2 / tan_v0 * m + 0.5);
-        int n1 = (int)(sqrt3_2 / tan_v0 * m);
-        if (m % 2 == 0 && n == n1) n += 1;
-        V3<double> v0_m = v0 + mulv(n_v0v2, m * sqrt3_2 * d / sin_v0);
-        V3<double> v1_m = v1 + mulv(n_v1v2, m * sqrt3_2 * d / sin_v1);
-        if (dist(v0_m, v1_m) <= d) break;
-        double delta_d = ((n + (m % 2) / 2.0) - m * sqrt3_2 / tan_v0) * d;
-        V3<double> v = v0_m + mulv(n_v0v1, delta_d);
-        int N1 = (int)(dist(v, v1_m) / d);
-        for (int i = 0; i <= N1; i++)
-            if (emit(v + mulv(mulv(n_v0v1, (double)i), d))) return true;
-    }
-    if (emit(v2)) return true;
-
-    N = __builtin_sqrt(ls[(max_i + 1) % 3]) / d;
-    if (N > 1) {
-        if (N == (int)N) N -= 1;
-        V3<double> nn = normalize(v2 - v1);
-        for (int n = 1; n <= N; n++)
-            if (emit(v1 + mulv(mulv(nn, d), (double)n))) return true;
-    }
-    N = __builtin_sqrt(ls[(max_i + 2) % 3]) / d;
-    if (N > 1) {
-        if (N == (int)N) N -= 1;
-        V3<double> nn = normalize(v0 - v2);
-        for (int n = 1; n <= N; n++)
-            if (emit(v2 + mulv(mulv(nn, d), (double)n))) return true;
-    }
+    SampleGen g;
+    g.init(vs, d);
+    V3<double> p;
+    while (g.next(p))
+        if (emit(p)) return true;
     return false;
 }
 

@@ -403,4 +403,34 @@ TWG_HD bool collinear3(const double* p, const double* q, const double* r)
     return minor_zero(p, q, r, 0, 1) && minor_zero(p, q, r, 1, 2) && minor_zero(p, q, r, 2, 0);
 }
 
+// Filter only: true when some 2x2 minor of (q-p) x (r-p) is certainly nonzero,
+// i.e. the triangle is certainly not degenerate.  false = only the exact test
+// can tell (never for a triangle with a clearly nonzero area).
+TWG_HD bool certainly_not_collinear(const double* p, const double* q, const double* r)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int u = k, v = (k + 1) % 3;
+        double au = q[u] - p[u], av = q[v] - p[v], bu = r[u] - p[u], bv = r[v] - p[v];
+        double m = au * bv - av * bu;
+        double perm = fabs_(au * bv) + fabs_(av * bu);
+        if (fabs_(m) > 0x1p-51 * perm) return true;
+    }
+    return false;
+}
+
+// XCD-aware block order: blocks are dispatched round-robin over the 8 XCDs of
+// an MI355X, so block b runs on XCD b % 8.  Remapping b -> (b % 8) * (G / 8) +
+// b / 8 gives every XCD a contiguous range of the work, so neighbouring items
+// (which share vertices or tree nodes) meet in the same L2.  Blocks beyond the
+// last full round of 8 keep their index.
+__device__ __forceinline__ int64_t xcd_block()
+{
+    const unsigned g = gridDim.x, b = blockIdx.x;
+    const unsigned per = g / 8;
+    const unsigned k = b / 8;
+    if (k >= per) return b;
+    return (int64_t)(b % 8) * per + k;
+}
+
 }  // namespace twg

@@ -279,11 +279,25 @@ class SmoothBatch:
         self.desc = d
         self._keep = (sf, bt)
 
-    def run(self, precision=C.TWG_FP64, tile=0, chunk=0):
+    def run(self, precision=C.TWG_FP64, tile=0, chunk=0, desc=None):
         self.ctx._bind()
-        C.check(self.ctx.L.twg_smooth_batch(self.ctx.h, ctypes.byref(self.desc), precision, tile,
-                                            chunk), "twg_smooth_batch")
+        d = self.desc if desc is None else desc
+        C.check(self.ctx.L.twg_smooth_batch(self.ctx.h, ctypes.byref(d), precision, tile, chunk),
+                "twg_smooth_batch")
         return self.p_out, self.status, self.new_q
+
+    def problem_desc(self, p):
+        """descriptor of problem p alone (n = 1), sharing this batch's arrays"""
+        d = C.SmoothDesc()
+        ctypes.pointer(d)[0] = self.desc
+        d.n = 1
+        d.center = self.center.data_ptr() + 4 * p
+        d.ring_off = self.ring_off.data_ptr() + 4 * p
+        d.kind = self.kind.data_ptr() + p
+        d.tri_off = self.tri_off.data_ptr() + 4 * p
+        d.p_out = self.p_out.data_ptr() + 24 * p
+        d.status = self.status.data_ptr() + p
+        return d
 
     def autotune(self, precision=C.TWG_FP64):
         t = ctypes.c_int()

@@ -289,12 +289,15 @@ def facet_normals(V, F):
     return n / area2[:, None], 0.5 * area2
 
 
-def surface_points(V, F, n, seed=0, tag=6):
+def surface_points(V, F, n, seed=0, tag=6, _pre=None):
     """n points uniform by area on the surface: (points, facet ids, normals)"""
     st = Stream(seed, tag)
-    nrm, area = facet_normals(V, F)
-    cdf = np.cumsum(area)
-    cdf /= cdf[-1]
+    if _pre is None:
+        nrm, area = facet_normals(V, F)
+        cdf = np.cumsum(area)
+        cdf /= cdf[-1]
+    else:
+        nrm, cdf = _pre
     f = np.searchsorted(cdf, st.uniform(n), side="right")
     f = np.minimum(f, len(F) - 1)
     u = st.uniform(n)
@@ -310,9 +313,12 @@ def query_triangles(V, F, n, eps=1e-3, edge=0.002, seed=0, chunk=1 << 21):
     surface, in the facet plane, edge * (1 + U[-0.1, 0.1]) long, offset along
     the facet normal by +-U[0, 2 eps]."""
     out = np.empty((n, 9))
+    nrm, area = facet_normals(V, F)
+    cdf = np.cumsum(area)
+    cdf /= cdf[-1]
     for s in range(0, n, chunk):
         k = min(chunk, n - s)
-        C, f, N = surface_points(V, F, k, seed, tag=7 + s // chunk)
+        C, f, N = surface_points(V, F, k, seed, tag=7 + s // chunk, _pre=(nrm, cdf))
         st = Stream(seed, 1000 + s // chunk)
         # tangent frame
         a = np.where(np.abs(N[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
@@ -378,13 +384,14 @@ def smoothing_batch(n, seed=0, surface_frac=0.2, V=None, F=None, radius=0.01, bo
     """n frozen one-ring problems.  Interior rings: cone from the centre over a
     triangulated sphere of k = (size + 4) / 2 jittered points; surface rings
     (a fraction surface_frac, needs the surface V, F): a bipyramid whose
-    equator lies on the surface around a surface point, the equator faces
-    being the one-ring surface faces.  Centres are displaced from the ring's
-    middle so that Newton moves them; every ring tet is positively oriented.
+    equator lies on the surface's tangent plane around a surface point, the
+    equator faces being the one-ring surface faces.  Centres are displaced
+    from the ring's middle so that Newton moves them; every ring tet is
+    positively oriented.  Problems of equal ring size are generated together.
 
     Returns a dict of numpy arrays: x, y, z, center, ring_off, ring_tets (int32
-    [n_ring, 4], tet vertex order an even permutation with the centre at a
-    random slot), kind (uint8), tri_off, tri_ab."""
+    [n_ring, 4], each tet's vertex order an even permutation, so the centre
+    sits at a random slot), kind (uint8), tri_off, tri_ab."""
     st = Stream(seed, 15)
     sizes = ring_sizes(n, seed)
     u = st.uniform(n)
@@ -393,86 +400,88 @@ def smoothing_batch(n, seed=0, surface_frac=0.2, V=None, F=None, radius=0.01, bo
         kind[u < surface_frac] = 1
         kind[u < surface_frac * boundary_frac] = 2
     sizes[kind > 0] = np.maximum(sizes[kind > 0], 6)
-    xs, ys, zs = [], [], []
-    rings = [None] * n
-    tris = [None] * n
-    nvert = 0
-    centers = np.empty(n, dtype=np.int64)
-    order = np.argsort(sizes, kind="stable")
+    nverts = np.where(kind == 0, 1 + (sizes + 4) // 2, 1 + sizes // 2 + 2)
+    voff = np.zeros(n + 1, dtype=np.int64)
+    voff[1:] = np.cumsum(nverts)
+    roff = np.zeros(n + 1, dtype=np.int64)
+    roff[1:] = np.cumsum(sizes)
+    ntri = np.where(kind > 0, sizes // 2, 0)
+    toff = np.zeros(n + 1, dtype=np.int64)
+    toff[1:] = np.cumsum(ntri)
+    pts = np.empty((voff[-1], 3))
+    tets = np.empty((roff[-1], 4), dtype=np.int64)
+    tri_ab = np.empty((toff[-1], 2), dtype=np.int64)
     if V is not None and (kind > 0).any():
-        SP, Sf, SN = surface_points(V, F, int((kind > 0).sum()), seed=seed, tag=16)
-    si = 0
-    for p in range(n):
-        s = int(sizes[p])
-        R = _rotations(st, 1)[0]
-        if kind[p] == 0:
-            k = (s + 4) // 2
-            P, Fh = _hull_template(k)
-            rad = radius * (1 + 0.1 * st.uniform(k, -1, 1))
-            Q = (P * rad[:, None]) @ R.T
-            # largest displacement keeping the centre inside every face plane
-            nrm = np.cross(Q[Fh[:, 1]] - Q[Fh[:, 0]], Q[Fh[:, 2]] - Q[Fh[:, 0]])
-            nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
-            h = np.einsum("ij,ij->i", nrm, Q[Fh[:, 0]]).min()
-            dirv = st.normal(3)
-            dirv /= np.linalg.norm(dirv)
-            ctr = dirv * h * 0.5 * st.uniform(1)[0]
-            base = st.uniform(3, -1, 1)
-            pts = np.vstack([ctr, Q]) + base
-            faces = Fh + 1
-            tri_ab = np.zeros((0, 2), dtype=np.int64)
-        else:
-            ke = s // 2
-            c0 = SP[si]
-            nn = SN[si]
-            si += 1
-            a = np.where(abs(nn[0]) < 0.9, np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
-            t1 = np.cross(nn, a)
-            t1 /= np.linalg.norm(t1)
-            t2 = np.cross(nn, t1)
-            ang = 2 * np.pi * (np.arange(ke) + 0.3 * st.uniform(ke, -1, 1)) / ke
-            rr = radius * (1 + 0.1 * st.uniform(ke, -1, 1))
-            eq = c0 + rr[:, None] * (np.cos(ang)[:, None] * t1 + np.sin(ang)[:, None] * t2)
-            north = c0 + nn * radius
-            south = c0 - nn * radius
-            off = st.uniform(2, -0.15, 0.15) * radius
-            ctr = c0 + off[0] * t1 + off[1] * t2 + nn * (0.02 * radius * st.uniform(1, -1, 1)[0])
-            pts = np.vstack([ctr, eq, north, south])
-            e = np.arange(ke) + 1
-            en = np.roll(e, -1)
-            Nn, Ss = ke + 1, ke + 2
-            faces = np.concatenate([np.stack([np.full(ke, Nn), e, en], 1),
-                                    np.stack([np.full(ke, Ss), en, e], 1)])
-            # outward orientation around the centre
-            fv = pts[faces]
-            nrm = np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0])
-            flip = np.einsum("ij,ij->i", nrm, fv.mean(1) - ctr) < 0
-            faces[flip] = faces[flip][:, [0, 2, 1]]
-            # one-ring surface faces (ctr, e_i, e_i+1) in the reference's order:
-            # sorted vertex ids, then rotated to start after the centre
-            tri_ab = []
-            for i in range(ke):
-                tri = sorted([nvert, nvert + int(e[i]), nvert + int(en[i])])
-                kk = tri.index(nvert)
-                tri_ab.append((tri[(kk + 1) % 3], tri[(kk + 2) % 3]))
-            tri_ab = np.array(sorted(tri_ab, key=lambda t: sorted([nvert, t[0], t[1]])),
-                              dtype=np.int64)
-        tets = np.concatenate([np.zeros((len(faces), 1), dtype=np.int64), faces], 1) + nvert
-        perm = _EVEN[st.integers(len(tets), 0, 11)]
-        tets = np.take_along_axis(tets, perm, axis=1)
-        rings[p] = tets
-        tris[p] = tri_ab
-        centers[p] = nvert
-        xs.append(pts[:, 0]); ys.append(pts[:, 1]); zs.append(pts[:, 2])
-        nvert += len(pts)
-    ring_off = np.zeros(n + 1, dtype=np.int64)
-    ring_off[1:] = np.cumsum([len(r) for r in rings])
-    tri_off = np.zeros(n + 1, dtype=np.int64)
-    tri_off[1:] = np.cumsum([len(t) for t in tris])
+        SP, _, SN = surface_points(V, F, n, seed=seed, tag=16)
+    for k in np.unique(sizes[kind == 0]):
+        idx = np.nonzero((kind == 0) & (sizes == k))[0]
+        m = len(idx)
+        kv = (int(k) + 4) // 2
+        P, Fh = _hull_template(kv)
+        R = _rotations(st, m)
+        rad = radius * (1 + 0.1 * st.uniform(m * kv, -1, 1).reshape(m, kv))
+        Q = np.einsum("mij,mkj->mki", R, P[None] * rad[:, :, None])
+        f0, f1, f2 = Q[:, Fh[:, 0]], Q[:, Fh[:, 1]], Q[:, Fh[:, 2]]
+        nrm = np.cross(f1 - f0, f2 - f0)
+        nrm /= np.linalg.norm(nrm, axis=2, keepdims=True)
+        h = np.einsum("mfi,mfi->mf", nrm, f0).min(axis=1)
+        dirv = st.normal(3 * m).reshape(m, 3)
+        dirv /= np.linalg.norm(dirv, axis=1, keepdims=True)
+        ctr = dirv * (h * 0.5 * st.uniform(m))[:, None]
+        base = st.uniform(3 * m, -1, 1).reshape(m, 1, 3)
+        allp = np.concatenate([ctr[:, None, :], Q], 1) + base
+        for j, p in enumerate(idx):
+            pts[voff[p]:voff[p + 1]] = allp[j]
+        T = np.concatenate([np.zeros((len(Fh), 1), dtype=np.int64), Fh + 1], 1)
+        TT = T[None] + voff[idx][:, None, None]
+        for j, p in enumerate(idx):
+            tets[roff[p]:roff[p + 1]] = TT[j]
+    for s in np.unique(sizes[kind > 0]):
+        idx = np.nonzero((kind > 0) & (sizes == s))[0]
+        m = len(idx)
+        ke = int(s) // 2
+        c0, nn = SP[idx], SN[idx]
+        a = np.where(np.abs(nn[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
+        t1 = np.cross(nn, a)
+        t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+        t2 = np.cross(nn, t1)
+        ang = 2 * np.pi * (np.arange(ke)[None] + 0.3 * st.uniform(m * ke, -1, 1).reshape(m, ke)) / ke
+        rr = radius * (1 + 0.1 * st.uniform(m * ke, -1, 1).reshape(m, ke))
+        eq = c0[:, None] + rr[:, :, None] * (np.cos(ang)[:, :, None] * t1[:, None] +
+                                             np.sin(ang)[:, :, None] * t2[:, None])
+        north = c0 + nn * radius
+        south = c0 - nn * radius
+        off = st.uniform(2 * m, -0.15, 0.15).reshape(m, 2) * radius
+        ctr = c0 + off[:, :1] * t1 + off[:, 1:] * t2 + nn * (0.02 * radius * st.uniform(m, -1, 1))[:, None]
+        allp = np.concatenate([ctr[:, None], eq, north[:, None], south[:, None]], 1)
+        e = np.arange(ke) + 1
+        en = np.roll(e, -1)
+        faces = np.concatenate([np.stack([np.full(ke, ke + 1), e, en], 1),
+                                np.stack([np.full(ke, ke + 2), en, e], 1)])
+        fv = allp[:, faces]  # (m, 2ke, 3, 3)
+        nrm = np.cross(fv[:, :, 1] - fv[:, :, 0], fv[:, :, 2] - fv[:, :, 0])
+        flip = np.einsum("mfi,mfi->mf", nrm, fv.mean(2) - ctr[:, None]) < 0
+        Fm = np.broadcast_to(faces, (m,) + faces.shape).copy()
+        Fm[flip] = Fm[flip][:, [0, 2, 1]]
+        T = np.concatenate([np.zeros((m, len(faces), 1), dtype=np.int64), Fm], 2)
+        # one-ring surface faces (ctr, e_i, e_i+1): the centre has the smallest
+        # id, so the reference's sorted-and-rotated triple is (ctr, lo, hi);
+        # std::sort orders the faces by (lo, hi)
+        lo = np.minimum(e, en)
+        hi = np.maximum(e, en)
+        o = np.lexsort((hi, lo))
+        ab = np.stack([lo[o], hi[o]], 1)
+        for j, p in enumerate(idx):
+            pts[voff[p]:voff[p + 1]] = allp[j]
+            tets[roff[p]:roff[p + 1]] = T[j] + voff[p]
+            tri_ab[toff[p]:toff[p + 1]] = ab + voff[p]
+    perm = _EVEN[st.integers(len(tets), 0, 11)]
+    tets = np.take_along_axis(tets, perm, axis=1)
     return {
-        "x": np.concatenate(xs), "y": np.concatenate(ys), "z": np.concatenate(zs),
-        "center": centers.astype(np.int32), "ring_off": ring_off.astype(np.int32),
-        "ring_tets": np.ascontiguousarray(np.concatenate(rings).astype(np.int32)),
-        "kind": kind, "tri_off": tri_off.astype(np.int32),
-        "tri_ab": np.ascontiguousarray(np.concatenate(tris).astype(np.int32).reshape(-1, 2)),
+        "x": np.ascontiguousarray(pts[:, 0]), "y": np.ascontiguousarray(pts[:, 1]),
+        "z": np.ascontiguousarray(pts[:, 2]),
+        "center": voff[:-1].astype(np.int32), "ring_off": roff.astype(np.int32),
+        "ring_tets": np.ascontiguousarray(tets.astype(np.int32)),
+        "kind": kind, "tri_off": toff.astype(np.int32),
+        "tri_ab": np.ascontiguousarray(tri_ab.astype(np.int32)),
     }
