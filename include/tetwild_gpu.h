@@ -174,7 +174,8 @@ int twg_sample_write(twg_ctx* ctx, const double* tri9, int64_t n, double d,
  * gathers them, src/tetwild/VertexSmoother.cpp:145-150).  kind[p]: 0 = interior
  * vertex (smoothSingle body, :145-212), 1 = surface vertex (smoothSurface body,
  * :282-455, projection on sf_tree), 2 = surface vertex on the boundary
- * (projection on b_tree).  For kinds 1 and 2: old_q holds the stored
+ * (projection on b_tree), 3 = NewtonsMethod alone (:465-542: status MOVED /
+ * NOT_MOVED = its is_moved, p_out = its returned position).  For kinds 1 and 2: old_q holds the stored
  * slim_energy of each ring tet, tri_ab[tri_off[p] .. tri_off[p+1]) the other
  * two vertex ids of each one-ring surface face in the reference's order
  * (:395-402), and new_q receives the ring's new qualities when the vertex

@@ -301,6 +301,6 @@ def smooth_batch(B, old_q=None, sf=None, bt=None, eps2=0.0, d=1.0):
                                  _p(po), _p(nq))
         status[p] = s
         p_out[p] = po
-        if s == 0 and int(B["kind"][p]) != 0:
+        if s == 0 and int(B["kind"][p]) in (1, 2):
             new_q[r0:r1] = nq[:r1 - r0]
     return p_out, status, new_q

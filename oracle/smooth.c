@@ -269,6 +269,9 @@ int orc_smooth_problem(int kind, int32_t v, const int32_t* rt, int m, const doub
     view w = {x, y, z, v, {x[v], y[v], z[v]}};
     double p0[3] = {x[v], y[v], z[v]};
     p_out[0] = p0[0]; p_out[1] = p0[1]; p_out[2] = p0[2];
+    if (kind == 3)
+        return orc_newtons_method(rt, m, v, x, y, z, p_out) ? ORC_SMOOTH_MOVED
+                                                            : ORC_SMOOTH_NOT_MOVED;
     if (ring_flip(rt, m, &w)) return ORC_SMOOTH_INVALID;
     double p[3];
     if (!orc_newtons_method(rt, m, v, x, y, z, p)) return ORC_SMOOTH_NOT_MOVED;

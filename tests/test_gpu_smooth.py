@@ -129,6 +129,18 @@ def test_graph_plan_replays(ctx, surf_data):
     plan.close()
 
 
+def test_newtons_method_only(ctx):
+    """kind 3: NewtonsMethod alone (is_moved and its returned position)"""
+    B = W.smoothing_batch(300, seed=16)
+    B["kind"][:] = 3
+    ref = O.smooth_batch(B)
+    batch = SmoothBatch(ctx, B)
+    batch.run(tile=32)
+    torch.cuda.synchronize()
+    assert np.array_equal(batch.status.cpu().numpy(), ref[1])
+    assert np.array_equal(bits(batch.p_out.cpu().numpy()), bits(ref[0]))
+
+
 def test_autotune_and_empty(ctx):
     B = W.smoothing_batch(300, seed=15)
     ref = O.smooth_batch(B)

@@ -428,13 +428,13 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
     const int64_t r0 = A.ring_off[p], r1 = A.ring_off[p + 1];
     const int kind = A.kind ? A.kind[p] : 0;
     int64_t t0 = 0, t1 = 0;
-    if (kind != 0 && A.tri_off) {
+    if ((kind == 1 || kind == 2) && A.tri_off) {
         t0 = A.tri_off[p];
         t1 = A.tri_off[p + 1];
     }
     // input validation: every index read below stays inside its array
-    bool bad = v < 0 || v >= A.nv || r0 < 0 || r1 < r0 || r1 > A.n_ring || kind > 2 ||
-               (kind != 0 && (!A.has_sf || (kind == 2 && !A.has_bt) || !A.old_q || !A.new_q ||
+    bool bad = v < 0 || v >= A.nv || r0 < 0 || r1 < r0 || r1 > A.n_ring || kind > 3 ||
+               ((kind == 1 || kind == 2) && (!A.has_sf || (kind == 2 && !A.has_bt) || !A.old_q || !A.new_q ||
                               t0 < 0 || t1 < t0 || t1 > A.n_tri));
     const int m = bad ? 0 : (int)(r1 - r0);
     const int4* rt = A.ring + r0;
@@ -456,6 +456,12 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
     V3<double> pfin = p0;
     if (bad) {
         st = TWG_SMOOTH_BAD_INPUT;
+    } else if (kind == 3) {
+        // NewtonsMethod alone: is_moved and the loop's final position
+        V3<double> pn;
+        st = newtons_method<TILE, MIXED>(A, c, rt, m, v, p0, pn) ? TWG_SMOOTH_MOVED
+                                                                 : TWG_SMOOTH_NOT_MOVED;
+        pfin = pn;
     } else if (ring_flip<TILE>(A, c, rt, m, v, p0)) {
         st = TWG_SMOOTH_INVALID;
     } else {

@@ -1,0 +1,729 @@
+// tetwild_shim.cpp -- host side of the LocalOperations / VertexSmoother
+// mirrors.  Control flow follows src/tetwild/LocalOperations.cpp and
+// src/tetwild/VertexSmoother.cpp; all geometry runs in libtetwild_amd.so.
+#include "tetwild_shim.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+
+namespace tetwild_amd {
+
+namespace {
+void check(int status, const char* what)
+{
+    if (status != TWG_OK)
+        throw std::runtime_error(std::string(what) + ": " + twg_last_error());
+}
+}  // namespace
+
+// ---- Device --------------------------------------------------------------
+Device::Device(int device) { check(twg_ctx_create(device, &ctx_), "twg_ctx_create"); }
+
+Device::~Device()
+{
+    for (int i = 0; i < 8; ++i)
+        if (buf_[i]) twg_free(ctx_, buf_[i]);
+    twg_ctx_destroy(ctx_);
+}
+
+void* Device::buffer(int i, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (bytes > cap_[i]) {
+        if (buf_[i]) check(twg_free(ctx_, buf_[i]), "twg_free");
+        size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
+        check(twg_malloc(ctx_, want, &buf_[i]), "twg_malloc");
+        cap_[i] = want;
+    }
+    return buf_[i];
+}
+
+void Device::h2d(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) check(twg_memcpy_h2d(ctx_, dst, src, bytes), "twg_memcpy_h2d");
+}
+
+void Device::d2h(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) check(twg_memcpy_d2h(ctx_, dst, src, bytes), "twg_memcpy_d2h");
+}
+
+void Device::sync() { check(twg_ctx_sync(ctx_), "twg_ctx_sync"); }
+
+Device& default_device()
+{
+    static Device d(0);
+    return d;
+}
+
+// ---- Envelope ------------------------------------------------------------
+Envelope::Envelope(Device& dev, const std::vector<Point_3f>& V,
+                   const std::vector<std::array<int, 3>>& F)
+    : dev_(dev)
+{
+    check(twg_bvh_build(dev.ctx(), reinterpret_cast<const double*>(V.data()), (int64_t)V.size(),
+                        reinterpret_cast<const int32_t*>(F.data()), (int64_t)F.size(), &bvh_),
+          "twg_bvh_build");
+}
+
+Envelope::~Envelope() { twg_bvh_destroy(bvh_); }
+
+double Envelope::squared_distance(const Point_3f& p, Point_3f* nearest) const
+{
+    // device buffers 6/7 are reserved for the envelope helpers
+    double* dp = (double*)dev_.buffer(6, 64);
+    double* dout = (double*)dev_.buffer(7, 64);
+    dev_.h2d(dp, p.data(), 24);
+    check(twg_nearest_points(dev_.ctx(), bvh_, dp, 1, dout, dout + 3, (int32_t*)(dout + 4)),
+          "twg_nearest_points");
+    double h[5];
+    dev_.d2h(h, dout, 40);
+    if (nearest) *nearest = {{h[0], h[1], h[2]}};
+    return h[3];
+}
+
+// ---- LocalOperations -------------------------------------------------------
+void LocalOperations::gather(const std::vector<std::array<int, 4>>& ts, Gathered& g, int pin)
+{
+    if (stamp_.size() < tet_vertices.size()) stamp_.assign(tet_vertices.size(), -1);
+    g.tets.resize(4 * ts.size());
+    g.global.clear();
+    auto local = [&](int v) {
+        if (stamp_[v] < 0) {
+            stamp_[v] = (int)g.global.size();
+            g.global.push_back(v);
+        }
+        return stamp_[v];
+    };
+    if (pin >= 0) local(pin);
+    for (size_t i = 0; i < ts.size(); ++i)
+        for (int j = 0; j < 4; ++j) g.tets[4 * i + j] = local(ts[i][j]);
+    g.x.resize(g.global.size());
+    g.y.resize(g.global.size());
+    g.z.resize(g.global.size());
+    for (size_t k = 0; k < g.global.size(); ++k) {
+        const Point_3f& p = tet_vertices[g.global[k]].posf;
+        g.x[k] = p[0];
+        g.y[k] = p[1];
+        g.z[k] = p[2];
+        stamp_[g.global[k]] = -1;
+    }
+}
+
+void LocalOperations::calTetQualities(const std::vector<std::array<int, 4>>& new_tets,
+                                      std::vector<TetQuality>& tet_qs, bool)
+{
+    tet_qs.assign(new_tets.size(), TetQuality());
+    if (new_tets.empty()) return;
+    Gathered g;
+    gather(new_tets, g);
+    size_t n = new_tets.size(), nv = g.global.size();
+    int32_t* dt = (int32_t*)dev.buffer(0, 16 * n);
+    double* dx = (double*)dev.buffer(1, 8 * nv);
+    double* dy = (double*)dev.buffer(2, 8 * nv);
+    double* dz = (double*)dev.buffer(3, 8 * nv);
+    double* de = (double*)dev.buffer(4, 8 * n);
+    dev.h2d(dt, g.tets.data(), 16 * n);
+    dev.h2d(dx, g.x.data(), 8 * nv);
+    dev.h2d(dy, g.y.data(), 8 * nv);
+    dev.h2d(dz, g.z.data(), 8 * nv);
+    check(twg_tet_qualities(dev.ctx(), dt, (int64_t)n, dx, dy, dz, (int64_t)nv, TWG_FP64, 0.0, de,
+                            nullptr),
+          "twg_tet_qualities");
+    std::vector<double> e(n);
+    dev.d2h(e.data(), de, 8 * n);
+    for (size_t i = 0; i < n; ++i) tet_qs[i].slim_energy = e[i];
+}
+
+bool LocalOperations::isFlip(const std::vector<std::array<int, 4>>& new_tets)
+{
+    if (new_tets.empty()) return false;
+    Gathered g;
+    gather(new_tets, g);
+    size_t n = new_tets.size(), nv = g.global.size();
+    int32_t* dt = (int32_t*)dev.buffer(0, 16 * n);
+    double* dx = (double*)dev.buffer(1, 8 * nv);
+    double* dy = (double*)dev.buffer(2, 8 * nv);
+    double* dz = (double*)dev.buffer(3, 8 * nv);
+    uint8_t* df = (uint8_t*)dev.buffer(4, n);
+    dev.h2d(dt, g.tets.data(), 16 * n);
+    dev.h2d(dx, g.x.data(), 8 * nv);
+    dev.h2d(dy, g.y.data(), 8 * nv);
+    dev.h2d(dz, g.z.data(), 8 * nv);
+    check(twg_tet_flip(dev.ctx(), dt, (int64_t)n, dx, dy, dz, (int64_t)nv, df), "twg_tet_flip");
+    std::vector<uint8_t> f(n);
+    dev.d2h(f.data(), df, n);
+    return std::any_of(f.begin(), f.end(), [](uint8_t b) { return b != 0; });
+}
+
+bool LocalOperations::isTetFlip(const std::array<int, 4>& t)
+{
+    return isFlip(std::vector<std::array<int, 4>>{t});
+}
+
+// src/tetwild/LocalOperations.cpp:575-603
+void LocalOperations::getCheckQuality(const std::vector<TetQuality>& tet_qs, TetQuality& tq)
+{
+    double slim_sum = 0, slim_max = 0;
+    for (const auto& q : tet_qs) {
+        if (state.use_energy_max) {
+            if (q.slim_energy > slim_max) slim_max = q.slim_energy;
+        } else {
+            slim_sum += q.slim_energy * q.volume;
+        }
+    }
+    tq.slim_energy = state.use_energy_max ? slim_max : slim_sum;
+}
+
+void LocalOperations::getCheckQuality(const std::vector<int>& t_ids, TetQuality& tq)
+{
+    std::vector<TetQuality> qs;
+    qs.reserve(t_ids.size());
+    for (int t : t_ids) qs.push_back(tet_qualities[t]);
+    getCheckQuality(qs, tq);
+}
+
+// src/tetwild/LocalOperations.cpp:605-637 (no UI locks in this model)
+void LocalOperations::getAvgMaxEnergy(double& avg_tq, double& max_tq)
+{
+    avg_tq = 0;
+    max_tq = 0;
+    int cnt = 0;
+    for (size_t i = 0; i < tet_qualities.size(); ++i) {
+        if (t_is_removed[i]) continue;
+        if (tet_qualities[i].slim_energy > max_tq) max_tq = tet_qualities[i].slim_energy;
+        avg_tq += tet_qualities[i].slim_energy;
+        cnt++;
+    }
+    avg_tq /= cnt;
+    if (std::isinf(avg_tq)) avg_tq = State::MAX_ENERGY;
+}
+
+double LocalOperations::getMaxEnergy()
+{
+    double max_tq = 0;
+    for (size_t i = 0; i < tet_qualities.size(); ++i) {
+        if (t_is_removed[i]) continue;
+        if (tet_qualities[i].slim_energy > max_tq) max_tq = tet_qualities[i].slim_energy;
+    }
+    return max_tq;
+}
+
+void LocalOperations::areFacesOutEnvelop(const std::vector<Triangle_3f>& tris,
+                                         std::vector<uint8_t>& out)
+{
+    out.assign(tris.size(), 0);
+    if (tris.empty()) return;
+    double* dt = (double*)dev.buffer(6, 72 * tris.size());
+    uint8_t* dout = (uint8_t*)dev.buffer(7, tris.size());
+    dev.h2d(dt, tris.data(), 72 * tris.size());
+    check(twg_envelope_faces(dev.ctx(), geo_sf_tree.bvh(), dt, (int64_t)tris.size(), state.eps_2,
+                             state.sampling_dist, TWG_FP64, dout, nullptr),
+          "twg_envelope_faces");
+    dev.d2h(out.data(), dout, tris.size());
+}
+
+void LocalOperations::arePointsOutEnvelop(const std::vector<Point_3f>& ps,
+                                          std::vector<uint8_t>& out)
+{
+    out.assign(ps.size(), 0);
+    if (ps.empty()) return;
+    double* dp = (double*)dev.buffer(6, 24 * ps.size());
+    uint8_t* dout = (uint8_t*)dev.buffer(7, ps.size());
+    dev.h2d(dp, ps.data(), 24 * ps.size());
+    check(twg_envelope_points(dev.ctx(), geo_sf_tree.bvh(), dp, (int64_t)ps.size(), state.eps_2,
+                              TWG_FP64, dout, nullptr),
+          "twg_envelope_points");
+    dev.d2h(out.data(), dout, ps.size());
+}
+
+// src/tetwild/LocalOperations.cpp:967-975, 1046-1109
+bool LocalOperations::isFaceOutEnvelop(const Triangle_3f& tri)
+{
+    std::vector<uint8_t> out;
+    areFacesOutEnvelop({tri}, out);
+    return out[0] != 0;
+}
+
+// src/tetwild/LocalOperations.cpp:1034-1044
+bool LocalOperations::isPointOutEnvelop(const Point_3f& p)
+{
+    std::vector<uint8_t> out;
+    arePointsOutEnvelop({p}, out);
+    return out[0] != 0;
+}
+
+// src/tetwild/LocalOperations.cpp:1111-1121
+bool LocalOperations::isPointOutBoundaryEnvelop(const Point_3f& p)
+{
+    double* dp = (double*)dev.buffer(6, 24);
+    uint8_t* dout = (uint8_t*)dev.buffer(7, 1);
+    dev.h2d(dp, p.data(), 24);
+    check(twg_envelope_points(dev.ctx(), geo_b_tree.bvh(), dp, 1, state.eps_2, TWG_FP64, dout,
+                              nullptr),
+          "twg_envelope_points");
+    uint8_t o = 0;
+    dev.d2h(&o, dout, 1);
+    return o != 0;
+}
+
+// src/tetwild/LocalOperations.cpp:1278-1287
+bool LocalOperations::isIsolated(int v_id)
+{
+    for (int t : tet_vertices[v_id].conn_tets)
+        for (int j = 0; j < 4; ++j)
+            if (tets[t][j] != v_id && is_surface_fs[t][j] != State::NOT_SURFACE) return false;
+    return true;
+}
+
+// src/tetwild/LocalOperations.cpp:1289-1303
+bool LocalOperations::isBoundaryPoint(int v_id)
+{
+    if (state.is_mesh_closed) return false;
+    std::unordered_set<int> n_v_ids;
+    for (int t : tet_vertices[v_id].conn_tets)
+        for (int j = 0; j < 4; ++j)
+            if (tets[t][j] != v_id && tet_vertices[tets[t][j]].is_on_boundary)
+                n_v_ids.insert(tets[t][j]);
+    for (int n : n_v_ids)
+        if (isEdgeOnBoundary(n, v_id)) return true;
+    return false;
+}
+
+// src/tetwild/LocalOperations.cpp:931-965
+bool LocalOperations::isEdgeOnBoundary(int v1_id, int v2_id)
+{
+    if (state.is_mesh_closed) return false;
+    if (!tet_vertices[v1_id].is_on_boundary || !tet_vertices[v2_id].is_on_boundary) return false;
+    int cnt = 0;
+    for (int t : tet_vertices[v1_id].conn_tets) {
+        std::array<int, 4> opp{};
+        int ii = 0;
+        for (int j = 0; j < 4; ++j) {
+            if (tets[t][j] == v1_id || tets[t][j] == v2_id) continue;
+            opp[ii++] = j;
+        }
+        if (ii == 2) {
+            if (is_surface_fs[t][opp[0]] != State::NOT_SURFACE) cnt++;
+            if (is_surface_fs[t][opp[1]] != State::NOT_SURFACE) cnt++;
+            if (cnt > 2) return false;
+        }
+    }
+    return cnt == 2;
+}
+
+// ---- static AMIPS functions (one tet, on the default device) --------------
+namespace {
+void amips_one(const double* T, int mode, double* E, double* G, double* H)
+{
+    Device& d = default_device();
+    double xyz[12];
+    for (int j = 0; j < 4; ++j) {
+        xyz[j] = T[3 * j];
+        xyz[4 + j] = T[3 * j + 1];
+        xyz[8 + j] = T[3 * j + 2];
+    }
+    int32_t tet[4] = {0, 1, 2, 3};
+    double* dx = (double*)d.buffer(0, 12 * 8);
+    int32_t* dt = (int32_t*)d.buffer(1, 16);
+    double* dout = (double*)d.buffer(2, 13 * 8);
+    d.h2d(dx, xyz, 96);
+    d.h2d(dt, tet, 16);
+    check(twg_amips_eval(d.ctx(), dt, 1, dx, dx + 4, dx + 8, 4, mode, dout, dout + 1, dout + 4),
+          "twg_amips_eval");
+    double h[13];
+    d.d2h(h, dout, 104);
+    if (E) *E = h[0];
+    if (G) std::memcpy(G, h + 1, 24);
+    if (H) std::memcpy(H, h + 4, 72);
+}
+}  // namespace
+
+double LocalOperations::comformalAMIPSEnergy_new(const double* T)
+{
+    double e;
+    amips_one(T, TWG_AMIPS_E, &e, nullptr, nullptr);
+    return e;
+}
+
+void LocalOperations::comformalAMIPSJacobian_new(const double* T, double* result_0)
+{
+    amips_one(T, TWG_AMIPS_EG, nullptr, result_0, nullptr);
+}
+
+void LocalOperations::comformalAMIPSHessian_new(const double* T, double* result_0)
+{
+    amips_one(T, TWG_AMIPS_EGH, nullptr, nullptr, result_0);
+}
+
+// ---- VertexSmoother --------------------------------------------------------
+void VertexSmoother::oneRing(int v_id, std::vector<int>& t_ids) const
+{
+    t_ids.assign(tet_vertices[v_id].conn_tets.begin(), tet_vertices[v_id].conn_tets.end());
+    std::sort(t_ids.begin(), t_ids.end());
+}
+
+void VertexSmoother::runBatch(const std::vector<Problem>& probs, std::vector<uint8_t>& status)
+{
+    size_t n = probs.size();
+    status.assign(n, 0);
+    if (n == 0) return;
+    // gather every ring into one compact vertex set; centres first
+    if (stamp_.size() < tet_vertices.size()) stamp_.assign(tet_vertices.size(), -1);
+    std::vector<int> global;
+    auto local = [&](int v) {
+        if (stamp_[v] < 0) {
+            stamp_[v] = (int)global.size();
+            global.push_back(v);
+        }
+        return stamp_[v];
+    };
+    std::vector<int32_t> center(n), ring_off(n + 1), tri_off(n + 1);
+    std::vector<uint8_t> kind(n);
+    std::vector<std::vector<int>> rings(n);
+    for (size_t p = 0; p < n; ++p) center[p] = local(probs[p].v);
+    std::vector<int32_t> ring_tets, tri_ab;
+    std::vector<double> old_q;
+    ring_off[0] = 0;
+    tri_off[0] = 0;
+    for (size_t p = 0; p < n; ++p) {
+        int v = probs[p].v;
+        kind[p] = (uint8_t)probs[p].kind;
+        oneRing(v, rings[p]);
+        for (int t : rings[p]) {
+            for (int j = 0; j < 4; ++j) ring_tets.push_back(local(tets[t][j]));
+            old_q.push_back(tet_qualities.size() > (size_t)t ? tet_qualities[t].slim_energy : 0.0);
+        }
+        ring_off[p + 1] = (int32_t)(ring_tets.size() / 4);
+        if (probs[p].kind == 1 || probs[p].kind == 2) {
+            // one-ring surface faces, src/tetwild/VertexSmoother.cpp:316-327, 395-402
+            std::vector<std::array<int, 3>> tri_ids;
+            for (int t : rings[p])
+                for (int j = 0; j < 4; ++j)
+                    if (tets[t][j] != v && is_surface_fs[t][j] != State::NOT_SURFACE) {
+                        std::array<int, 3> tri = {{tets[t][(j + 1) % 4], tets[t][(j + 2) % 4],
+                                                   tets[t][(j + 3) % 4]}};
+                        std::sort(tri.begin(), tri.end());
+                        tri_ids.push_back(tri);
+                    }
+            std::sort(tri_ids.begin(), tri_ids.end());
+            tri_ids.erase(std::unique(tri_ids.begin(), tri_ids.end()), tri_ids.end());
+            for (const auto& tri : tri_ids) {
+                int k = (int)(std::find(tri.begin(), tri.end(), v) - tri.begin());
+                tri_ab.push_back(local(tri[(k + 1) % 3]));
+                tri_ab.push_back(local(tri[(k + 2) % 3]));
+            }
+        }
+        tri_off[p + 1] = (int32_t)(tri_ab.size() / 2);
+    }
+    size_t nv = global.size(), nr = ring_tets.size() / 4, nt = tri_ab.size() / 2;
+    std::vector<double> x(nv), y(nv), z(nv);
+    for (size_t k = 0; k < nv; ++k) {
+        const Point_3f& p = tet_vertices[global[k]].posf;
+        x[k] = p[0];
+        y[k] = p[1];
+        z[k] = p[2];
+        stamp_[global[k]] = -1;
+    }
+    // device copies (buffer slots 0..5; 6/7 are the envelope helpers')
+    char* b0 = (char*)dev.buffer(0, 24 * nv);
+    double *dx = (double*)b0, *dy = dx + nv, *dz = dy + nv;
+    int32_t* dring = (int32_t*)dev.buffer(1, 16 * nr + 16);
+    char* b2 = (char*)dev.buffer(2, 4 * (n + (n + 1) + (n + 1)) + 2 * 4 * nt + 64);
+    int32_t* dcenter = (int32_t*)b2;
+    int32_t* droff = dcenter + n;
+    int32_t* dtoff = droff + (n + 1);
+    int32_t* dtri = dtoff + (n + 1);
+    uint8_t* dkind = (uint8_t*)dev.buffer(3, n + n);
+    uint8_t* dstatus = dkind + n;
+    double* dq = (double*)dev.buffer(4, 16 * nr + 16);
+    double* dnewq = dq + nr;
+    double* dpout = (double*)dev.buffer(5, 24 * n);
+    dev.h2d(dx, x.data(), 8 * nv);
+    dev.h2d(dy, y.data(), 8 * nv);
+    dev.h2d(dz, z.data(), 8 * nv);
+    dev.h2d(dring, ring_tets.data(), 16 * nr);
+    dev.h2d(dcenter, center.data(), 4 * n);
+    dev.h2d(droff, ring_off.data(), 4 * (n + 1));
+    dev.h2d(dtoff, tri_off.data(), 4 * (n + 1));
+    dev.h2d(dtri, tri_ab.data(), 8 * nt);
+    dev.h2d(dkind, kind.data(), n);
+    dev.h2d(dq, old_q.data(), 8 * nr);
+    twg_smooth_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.x = dx;
+    d.y = dy;
+    d.z = dz;
+    d.nv = (int64_t)nv;
+    d.n = (int64_t)n;
+    d.center = dcenter;
+    d.ring_off = droff;
+    d.ring_tets = dring;
+    d.n_ring = (int64_t)nr;
+    d.kind = dkind;
+    d.old_q = dq;
+    d.tri_off = dtoff;
+    d.tri_ab = dtri;
+    d.n_tri = (int64_t)nt;
+    d.sf_tree = geo_sf_tree.bvh();
+    d.b_tree = geo_b_tree.bvh();
+    d.eps2 = state.eps_2;
+    d.sampling_dist = state.sampling_dist > 0 ? state.sampling_dist : 1.0;
+    d.p_out = dpout;
+    d.status = dstatus;
+    d.new_q = dnewq;
+    check(twg_smooth_batch(dev.ctx(), &d, precision, 0, 0), "twg_smooth_batch");
+    std::vector<double> pout(3 * n), newq(nr);
+    dev.d2h(status.data(), dstatus, n);
+    dev.d2h(pout.data(), dpout, 24 * n);
+    dev.d2h(newq.data(), dnewq, 8 * nr);
+    for (size_t p = 0; p < n; ++p) {
+        if (status[p] == TWG_SMOOTH_BAD_INPUT)
+            throw std::runtime_error("twg_smooth_batch: bad input for vertex " +
+                                     std::to_string(probs[p].v));
+        if (probs[p].kind == 3) continue;  // NewtonsMethod only: caller reads the position
+        if (status[p] != TWG_SMOOTH_MOVED) continue;
+        tet_vertices[probs[p].v].posf = {{pout[3 * p], pout[3 * p + 1], pout[3 * p + 2]}};
+        if (probs[p].kind == 1 || probs[p].kind == 2) {
+            int r0 = ring_off[p];
+            for (size_t i = 0; i < rings[p].size(); ++i) {
+                TetQuality q;
+                q.slim_energy = newq[r0 + i];
+                tet_qualities[rings[p][i]] = q;
+            }
+        }
+    }
+    last_pout_.assign(pout.begin(), pout.end());
+}
+
+// src/tetwild/VertexSmoother.cpp:465-542
+bool VertexSmoother::NewtonsMethod(const std::vector<int>& t_ids,
+                                   const std::vector<std::array<int, 4>>&, int v_id, Point_3f& p)
+{
+    (void)t_ids;  // the ring is the vertex's conn_tets (sorted), as oneRing() gathers it
+    std::vector<uint8_t> st;
+    runBatch({Problem{v_id, 3}}, st);
+    p = {{last_pout_[0], last_pout_[1], last_pout_[2]}};
+    return st[0] == TWG_SMOOTH_MOVED;
+}
+
+// src/tetwild/VertexSmoother.cpp:627-702
+bool VertexSmoother::NewtonsUpdate(const std::vector<int>& t_ids, int v_id, double& energy,
+                                   std::array<double, 3>& J, std::array<double, 9>& H,
+                                   std::array<double, 3>& X0)
+{
+    std::vector<std::array<int, 4>> rot;
+    for (int t : t_ids) {
+        int start = 0;
+        for (int j = 0; j < 4; ++j)
+            if (tets[t][j] == v_id) {
+                start = j;
+                break;
+            }
+        rot.push_back({{tets[t][start], tets[t][(start + 1) % 4], tets[t][(start + 2) % 4],
+                        tets[t][(start + 3) % 4]}});
+    }
+    Gathered g;
+    gather(rot, g);
+    size_t n = rot.size(), nv = g.global.size();
+    int32_t* dt = (int32_t*)dev.buffer(0, 16 * n);
+    double* dx = (double*)dev.buffer(1, 24 * nv);
+    double* dout = (double*)dev.buffer(4, 104 * n);
+    dev.h2d(dt, g.tets.data(), 16 * n);
+    dev.h2d(dx, g.x.data(), 8 * nv);
+    dev.h2d(dx + nv, g.y.data(), 8 * nv);
+    dev.h2d(dx + 2 * nv, g.z.data(), 8 * nv);
+    check(twg_amips_eval(dev.ctx(), dt, (int64_t)n, dx, dx + nv, dx + 2 * nv, (int64_t)nv,
+                         TWG_AMIPS_EGH, dout, dout + n, dout + 4 * n),
+          "twg_amips_eval");
+    std::vector<double> h(13 * n);
+    dev.d2h(h.data(), dout, 104 * n);
+    energy = 0;
+    J = {{0, 0, 0}};
+    H = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    X0 = tet_vertices[v_id].posf;
+    for (size_t i = 0; i < n; ++i) {
+        energy += h[i];
+        for (int k = 0; k < 3; ++k) J[k] += h[n + 3 * i + k];
+        for (int k = 0; k < 9; ++k) H[k] += h[4 * n + 9 * i + k];
+    }
+    if (std::isinf(energy)) energy = State::MAX_ENERGY;
+    if (std::isnan(energy) || energy <= 0) return false;
+    for (double v : J)
+        if (!std::isfinite(v)) return false;
+    for (double v : H)
+        if (!std::isfinite(v)) return false;
+    return true;
+}
+
+// src/tetwild/VertexSmoother.cpp:544-625 (scalar branch)
+double VertexSmoother::getNewEnergy(const std::vector<int>& t_ids)
+{
+    std::vector<std::array<int, 4>> ts;
+    for (int t : t_ids) ts.push_back(tets[t]);
+    if (ts.empty()) return State::MAX_ENERGY;
+    Gathered g;
+    gather(ts, g);
+    size_t n = ts.size(), nv = g.global.size();
+    int32_t* dt = (int32_t*)dev.buffer(0, 16 * n);
+    double* dx = (double*)dev.buffer(1, 24 * nv);
+    double* de = (double*)dev.buffer(4, 8 * n);
+    dev.h2d(dt, g.tets.data(), 16 * n);
+    dev.h2d(dx, g.x.data(), 8 * nv);
+    dev.h2d(dx + nv, g.y.data(), 8 * nv);
+    dev.h2d(dx + 2 * nv, g.z.data(), 8 * nv);
+    check(twg_amips_eval(dev.ctx(), dt, (int64_t)n, dx, dx + nv, dx + 2 * nv, (int64_t)nv,
+                         TWG_AMIPS_E, de, nullptr, nullptr),
+          "twg_amips_eval");
+    std::vector<double> e(n);
+    dev.d2h(e.data(), de, 8 * n);
+    double s = 0;
+    for (double v : e) s += v;
+    if (std::isinf(s) || std::isnan(s) || s <= 0 || s > State::MAX_ENERGY) s = State::MAX_ENERGY;
+    return s;
+}
+
+// src/tetwild/VertexSmoother.cpp:114-242 in vertex order (Gauss-Seidel)
+void VertexSmoother::smoothSingle()
+{
+    counter = 0;
+    suc_counter = 0;
+    std::vector<uint8_t> st;
+    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
+        if (v_is_removed[v]) continue;
+        const TetVertex& tv = tet_vertices[v];
+        if (tv.is_on_bbox) continue;
+        if (state.eps != State::EPSILON_INFINITE && tv.is_on_surface) continue;
+        if (tv.is_locked) continue;
+        if (!tv.is_rounded) continue;  // rational vertex: the caller's exact path
+        counter++;
+        runBatch({Problem{v, 0}}, st);
+        if (st[0] == TWG_SMOOTH_MOVED || st[0] == TWG_SMOOTH_FLIP_END) suc_counter++;
+    }
+    std::vector<std::array<int, 4>> live;
+    std::vector<int> ids;
+    for (size_t i = 0; i < tets.size(); ++i)
+        if (!t_is_removed[i]) {
+            live.push_back(tets[i]);
+            ids.push_back((int)i);
+        }
+    std::vector<TetQuality> qs;
+    calTetQualities(live, qs);
+    for (size_t k = 0; k < ids.size(); ++k) tet_qualities[ids[k]] = qs[k];
+}
+
+// src/tetwild/VertexSmoother.cpp:244-463 in vertex order
+void VertexSmoother::smoothSurface()
+{
+    std::vector<uint8_t> st;
+    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
+        if (v_is_removed[v]) continue;
+        TetVertex& tv = tet_vertices[v];
+        if (!tv.is_on_surface) continue;
+        if (tv.is_locked) continue;
+        if (isIsolated(v)) {
+            tv.is_on_surface = false;
+            tv.is_on_boundary = false;
+            continue;
+        }
+        if (!isBoundaryPoint(v)) tv.is_on_boundary = false;
+        if (!tv.is_rounded) continue;
+        counter++;
+        runBatch({Problem{v, tv.is_on_boundary ? 2 : 1}}, st);
+        if (st[0] == TWG_SMOOTH_MOVED) suc_counter++;
+    }
+}
+
+// src/tetwild/VertexSmoother.cpp:19-46
+void VertexSmoother::smooth()
+{
+    smoothSingle();
+    if (state.eps >= 0) smoothSurface();
+}
+
+std::vector<std::vector<int>> VertexSmoother::colourClasses(const std::vector<int>& verts) const
+{
+    std::vector<int> colour(tet_vertices.size(), -1);
+    std::vector<std::vector<int>> classes;
+    std::vector<char> used;
+    for (int v : verts) {
+        used.assign(classes.size() + 1, 0);
+        for (int t : tet_vertices[v].conn_tets)
+            for (int j = 0; j < 4; ++j) {
+                int u = tets[t][j];
+                if (u != v && colour[u] >= 0) used[colour[u]] = 1;
+            }
+        int c = 0;
+        while (used[c]) ++c;
+        colour[v] = c;
+        if (c == (int)classes.size()) classes.emplace_back();
+        classes[c].push_back(v);
+    }
+    return classes;
+}
+
+void VertexSmoother::smoothSingleBatched()
+{
+    counter = 0;
+    suc_counter = 0;
+    std::vector<int> verts;
+    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
+        if (v_is_removed[v]) continue;
+        const TetVertex& tv = tet_vertices[v];
+        if (tv.is_on_bbox || tv.is_locked || !tv.is_rounded) continue;
+        if (state.eps != State::EPSILON_INFINITE && tv.is_on_surface) continue;
+        verts.push_back(v);
+    }
+    std::vector<uint8_t> st;
+    for (const auto& cls : colourClasses(verts)) {
+        std::vector<Problem> probs;
+        for (int v : cls) probs.push_back(Problem{v, 0});
+        runBatch(probs, st);
+        counter += (int)cls.size();
+        for (uint8_t s : st)
+            if (s == TWG_SMOOTH_MOVED || s == TWG_SMOOTH_FLIP_END) suc_counter++;
+    }
+    std::vector<std::array<int, 4>> live;
+    std::vector<int> ids;
+    for (size_t i = 0; i < tets.size(); ++i)
+        if (!t_is_removed[i]) {
+            live.push_back(tets[i]);
+            ids.push_back((int)i);
+        }
+    std::vector<TetQuality> qs;
+    calTetQualities(live, qs);
+    for (size_t k = 0; k < ids.size(); ++k) tet_qualities[ids[k]] = qs[k];
+}
+
+void VertexSmoother::smoothSurfaceBatched()
+{
+    std::vector<int> verts;
+    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
+        if (v_is_removed[v]) continue;
+        TetVertex& tv = tet_vertices[v];
+        if (!tv.is_on_surface || tv.is_locked) continue;
+        if (isIsolated(v)) {
+            tv.is_on_surface = false;
+            tv.is_on_boundary = false;
+            continue;
+        }
+        if (!isBoundaryPoint(v)) tv.is_on_boundary = false;
+        if (!tv.is_rounded) continue;
+        verts.push_back(v);
+    }
+    std::vector<uint8_t> st;
+    for (const auto& cls : colourClasses(verts)) {
+        std::vector<Problem> probs;
+        for (int v : cls) probs.push_back(Problem{v, tet_vertices[v].is_on_boundary ? 2 : 1});
+        runBatch(probs, st);
+        counter += (int)cls.size();
+        for (uint8_t s : st)
+            if (s == TWG_SMOOTH_MOVED) suc_counter++;
+    }
+}
+
+}  // namespace tetwild_amd

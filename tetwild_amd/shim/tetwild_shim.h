@@ -1,0 +1,209 @@
+// tetwild_shim.h -- C++ host classes with TetWild's LocalOperations /
+// VertexSmoother call surface, backed by libtetwild_amd.so (the MI355X
+// kernels, include/tetwild_gpu.h).
+//
+// The reference classes (src/tetwild/LocalOperations.h:33-125,
+// src/tetwild/VertexSmoother.h:19-51) hold references to the mesh arrays
+// owned by MeshRefinement and use CGAL/geogram types.  These mirrors keep the
+// method names, argument meaning and error conventions (MAX_ENERGY sentinels,
+// bool results) over std:: types that are layout-compatible with the
+// reference's double-precision points:
+//   Point_3f     -> std::array<double, 3>       (CGAL Epick Point_3)
+//   Triangle_3f  -> std::array<Point_3f, 3>
+//   GEO::MeshFacetsAABBWithEps -> tetwild_amd::Envelope (a device AABB tree)
+// Every call gathers the vertices it touches into compact device arrays, so
+// the GPU always sees the host mesh as it is at the time of the call (the
+// driver may mutate the mesh freely between calls).  Vertices must be rounded
+// (is_rounded): rational positions stay on the caller's exact path.
+#pragma once
+
+#include <array>
+#include <climits>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/tetwild_gpu.h"
+
+namespace tetwild_amd {
+
+using Point_3f = std::array<double, 3>;
+using Triangle_3f = std::array<Point_3f, 3>;
+
+// TetVertex (src/tetwild/TetmeshElements.h:24-64), double-precision part
+struct TetVertex {
+    Point_3f posf{{0, 0, 0}};
+    bool is_rounded = true;
+    bool is_on_surface = false;
+    bool is_on_bbox = false;
+    bool is_on_boundary = false;
+    bool is_locked = false;
+    std::unordered_set<int> conn_tets;
+};
+
+// TetQuality (src/tetwild/TetmeshElements.h:66-111)
+struct TetQuality {
+    double min_d_angle = 0;
+    double max_d_angle = 0;
+    double slim_energy = 0;
+    double volume = 0;
+    bool isBetterThan(const TetQuality& tq) const { return slim_energy < tq.slim_energy; }
+    bool isBetterOrEqualThan(const TetQuality& tq) const { return slim_energy <= tq.slim_energy; }
+};
+
+// The parts of State (src/tetwild/State.h:22-60) the path reads
+struct State {
+    static constexpr double MAX_ENERGY = 1e50;
+    static constexpr int NOT_SURFACE = INT_MAX;
+    static constexpr int EPSILON_INFINITE = -2;
+    double eps = 0;
+    double eps_2 = 0;
+    double sampling_dist = 0;
+    bool is_mesh_closed = true;
+    bool use_energy_max = true;
+};
+
+// GPU context shared by the objects of one mesh (one device, one stream).
+class Device {
+public:
+    explicit Device(int device = 0);
+    ~Device();
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    twg_ctx* ctx() const { return ctx_; }
+    // growable device buffer i (0..7), at least `bytes` long
+    void* buffer(int i, size_t bytes);
+    void h2d(void* dst, const void* src, size_t bytes);
+    void d2h(void* dst, const void* src, size_t bytes);
+    void sync();
+
+private:
+    twg_ctx* ctx_ = nullptr;
+    void* buf_[8] = {nullptr};
+    size_t cap_[8] = {0};
+};
+
+// Process-wide device used by the static AMIPS functions.
+Device& default_device();
+
+// Stand-in for GEO::MeshFacetsAABBWithEps over a triangle surface: a device
+// tree built once (MeshRefinement builds its trees once per refine,
+// src/tetwild/MeshRefinement.cpp:209-213).
+class Envelope {
+public:
+    Envelope(Device& dev, const std::vector<Point_3f>& V, const std::vector<std::array<int, 3>>& F);
+    ~Envelope();
+    Envelope(const Envelope&) = delete;
+    Envelope& operator=(const Envelope&) = delete;
+    const twg_bvh* bvh() const { return bvh_; }
+    // squared distance to the surface and the nearest point
+    double squared_distance(const Point_3f& p, Point_3f* nearest = nullptr) const;
+
+private:
+    Device& dev_;
+    twg_bvh* bvh_ = nullptr;
+};
+
+class LocalOperations {
+public:
+    std::vector<TetVertex>& tet_vertices;
+    std::vector<std::array<int, 4>>& tets;
+    std::vector<std::array<int, 4>>& is_surface_fs;
+    std::vector<bool>& v_is_removed;
+    std::vector<bool>& t_is_removed;
+    std::vector<TetQuality>& tet_qualities;
+    const Envelope& geo_sf_tree;
+    const Envelope& geo_b_tree;
+    State& state;
+    Device& dev;
+    int counter = 0;
+    int suc_counter = 0;
+    // TWG_FP64 (default) or TWG_MIXED: results are identical, mixed screens in FP32
+    int precision = TWG_FP64;
+
+    LocalOperations(std::vector<TetVertex>& t_vs, std::vector<std::array<int, 4>>& ts,
+                    std::vector<std::array<int, 4>>& is_sf_fs, std::vector<bool>& v_is_rm,
+                    std::vector<bool>& t_is_rm, std::vector<TetQuality>& tet_qs,
+                    const Envelope& sf_tree, const Envelope& b_tree, State& st, Device& device)
+        : tet_vertices(t_vs), tets(ts), is_surface_fs(is_sf_fs), v_is_removed(v_is_rm),
+          t_is_removed(t_is_rm), tet_qualities(tet_qs), geo_sf_tree(sf_tree), geo_b_tree(b_tree),
+          state(st), dev(device)
+    {
+    }
+
+    // src/tetwild/LocalOperations.h:69-111
+    void calTetQualities(const std::vector<std::array<int, 4>>& new_tets,
+                         std::vector<TetQuality>& tet_qs, bool all_measure = false);
+    bool isFlip(const std::vector<std::array<int, 4>>& new_tets);
+    bool isTetFlip(const std::array<int, 4>& t);
+    bool isTetFlip(int t_id) { return isTetFlip(tets[t_id]); }
+    void getCheckQuality(const std::vector<TetQuality>& tet_qs, TetQuality& tq);
+    void getCheckQuality(const std::vector<int>& t_ids, TetQuality& tq);
+    void getAvgMaxEnergy(double& avg_tq, double& max_tq);
+    double getMaxEnergy();
+    bool isFaceOutEnvelop(const Triangle_3f& tri);
+    bool isFaceOutEnvelop_sampling(const Triangle_3f& tri) { return isFaceOutEnvelop(tri); }
+    bool isPointOutEnvelop(const Point_3f& p);
+    bool isPointOutBoundaryEnvelop(const Point_3f& p);
+    bool isIsolated(int v_id);
+    bool isBoundaryPoint(int v_id);
+    bool isEdgeOnBoundary(int v1_id, int v2_id);
+    // batched forms of the envelope tests (one launch for many queries)
+    void areFacesOutEnvelop(const std::vector<Triangle_3f>& tris, std::vector<uint8_t>& out);
+    void arePointsOutEnvelop(const std::vector<Point_3f>& ps, std::vector<uint8_t>& out);
+
+    static double comformalAMIPSEnergy_new(const double* T);
+    static void comformalAMIPSJacobian_new(const double* T, double* result_0);
+    static void comformalAMIPSHessian_new(const double* T, double* result_0);
+
+protected:
+    // compact gather: tets -> local ids, x/y/z of the touched vertices
+    struct Gathered {
+        std::vector<int32_t> tets;
+        std::vector<double> x, y, z;
+        std::vector<int> global;  // local -> global vertex id
+    };
+    void gather(const std::vector<std::array<int, 4>>& ts, Gathered& g, int pin = -1);
+    std::vector<int> stamp_;  // per-vertex local id (or -1), sized lazily
+};
+
+class VertexSmoother : public LocalOperations {
+public:
+    using LocalOperations::LocalOperations;
+
+    // src/tetwild/VertexSmoother.h:23-30
+    void smooth();
+    void smoothSingle();
+    void smoothSurface();
+    bool NewtonsMethod(const std::vector<int>& t_ids,
+                       const std::vector<std::array<int, 4>>& new_tets, int v_id, Point_3f& p);
+    bool NewtonsUpdate(const std::vector<int>& t_ids, int v_id, double& energy,
+                       std::array<double, 3>& J, std::array<double, 9>& H,
+                       std::array<double, 3>& X0);
+    double getNewEnergy(const std::vector<int>& t_ids);
+
+    // Batched variants: vertices of one colour class (no two share a tet) are
+    // smoothed in one persistent launch; classes run one after another, each
+    // seeing the positions the previous ones produced.  Same per-vertex rule
+    // as smoothSingle / smoothSurface, different (colour-major) order.
+    void smoothSingleBatched();
+    void smoothSurfaceBatched();
+
+    // one-ring of v in the order the mirrors use (sorted tet ids)
+    void oneRing(int v_id, std::vector<int>& t_ids) const;
+
+private:
+    struct Problem {
+        int v;
+        int kind;
+    };
+    // runs the problems (all independent) as one batch; returns statuses and
+    // writes moved positions / qualities back into the mesh
+    void runBatch(const std::vector<Problem>& probs, std::vector<uint8_t>& status);
+    std::vector<std::vector<int>> colourClasses(const std::vector<int>& verts) const;
+    std::vector<double> last_pout_;  // p_out of the last batch
+};
+
+}  // namespace tetwild_amd
