@@ -45,6 +45,21 @@ int ensure_scratch(twg_ctx* ctx, size_t bytes)
     return TWG_OK;
 }
 
+int ensure_slot(twg_ctx* ctx, int i, size_t bytes)
+{
+    if (bytes <= ctx->slot_bytes[i]) return TWG_OK;
+    if (ctx->slot[i]) {
+        TWG_HIP(hipStreamSynchronize(ctx->cur));
+        TWG_HIP(hipFree(ctx->slot[i]));
+        ctx->slot[i] = nullptr;
+        ctx->slot_bytes[i] = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    TWG_HIP(hipMalloc(&ctx->slot[i], want));
+    ctx->slot_bytes[i] = want;
+    return TWG_OK;
+}
+
 }  // namespace twg
 
 extern "C" {
@@ -79,6 +94,8 @@ int twg_ctx_destroy(twg_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->cur) (void)hipStreamSynchronize(ctx->cur);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    for (int i = 0; i < 4; ++i)
+        if (ctx->slot[i]) (void)hipFree(ctx->slot[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return TWG_OK;

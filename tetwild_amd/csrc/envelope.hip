@@ -29,11 +29,13 @@ template <bool F32, bool STATS>
 __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double* __restrict__ p3,
                                                         int64_t n, double eps2, double T,
                                                         double sqrtT, uint8_t* __restrict__ out,
-                                                        unsigned long long* stats)
+                                                        unsigned long long* stats,
+                                                        const int32_t* __restrict__ order)
 {
     int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
+        if (order) i = order[i];
         int hint = -1;
         V3<double> p = ld3(p3 + 3 * i);
         if (STATS) st.samples++;
@@ -50,11 +52,13 @@ __global__ __launch_bounds__(kBlock) void faces_kernel(BvhView B, const double* 
                                                        int64_t n, double eps2, double d, double T,
                                                        double sqrtT, uint8_t* __restrict__ out,
                                                        unsigned long long* stats,
-                                                       unsigned* n_pending, int64_t* pending)
+                                                       unsigned* n_pending, int64_t* pending,
+                                                       const int32_t* __restrict__ order)
 {
     int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
+        if (order) i = order[i];
         const double* t = tri9 + 9 * i;
         if (certainly_not_collinear(t, t + 3, t + 6)) {
             out[i] = face_out_sampled<F32, STATS>(B, t, eps2, d, T, sqrtT, st) ? 1 : 0;
@@ -83,6 +87,9 @@ __global__ __launch_bounds__(kBlock) void faces_exact_kernel(BvhView B,
 }
 
 __global__ void zero_u32(unsigned* p) { *p = 0u; }
+
+// batches at least this large are walked in Morton order of their queries
+constexpr int64_t kSortMin = 1 << 16;
 
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
                                                          int64_t n, double* __restrict__ cp3,
@@ -274,6 +281,10 @@ int twg_bvh_build(twg_ctx* ctx, const double* V, int64_t nv, const int32_t* F, i
     int depth = 0;
     fill(nodes, tri, 1, 0, nf, 0, depth);
     B->depth = depth;
+    for (int c = 0; c < 3; ++c) {
+        B->lo[c] = nodes[1].lo[c];
+        B->hi[c] = nodes[1].hi[c];
+    }
     if (depth > 30) {
         delete B;
         set_error("twg_bvh_build: tree depth %d exceeds 30", depth);
@@ -362,12 +373,15 @@ int twg_envelope_points(twg_ctx* ctx, const twg_bvh* B, const double* p3, int64_
     double T = bvh_prune_threshold(B, eps2), sT = std::sqrt(T);
     dim3 g(grid_for(n)), b(kBlock);
     BvhView v = bvh_view(B);
+    const int32_t* order = nullptr;
+    if (n >= kSortMin)
+        if (int r = morton_order(ctx, p3, n, 3, B->lo, B->hi, &order)) return r;
     if (precision == TWG_FP64) {
-        if (stats) hipLaunchKernelGGL((points_kernel<false, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats);
-        else hipLaunchKernelGGL((points_kernel<false, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats);
+        if (stats) hipLaunchKernelGGL((points_kernel<false, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
+        else hipLaunchKernelGGL((points_kernel<false, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
     } else {
-        if (stats) hipLaunchKernelGGL((points_kernel<true, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats);
-        else hipLaunchKernelGGL((points_kernel<true, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats);
+        if (stats) hipLaunchKernelGGL((points_kernel<true, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
+        else hipLaunchKernelGGL((points_kernel<true, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
     }
     TWG_LAUNCHED();
     return TWG_OK;
@@ -388,14 +402,17 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
     double T = bvh_prune_threshold(B, eps2), sT = std::sqrt(T);
     dim3 g(grid_for(n)), b(kBlock);
     BvhView v = bvh_view(B);
+    const int32_t* order = nullptr;
+    if (n >= kSortMin)
+        if (int r = morton_order(ctx, tri9, n, 9, B->lo, B->hi, &order)) return r;
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->cur, n_pending);
     TWG_LAUNCHED();
     if (precision == TWG_FP64) {
-        if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
-        else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
+        if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
+        else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
     } else {
-        if (stats) hipLaunchKernelGGL((faces_kernel<true, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
-        else hipLaunchKernelGGL((faces_kernel<true, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending);
+        if (stats) hipLaunchKernelGGL((faces_kernel<true, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
+        else hipLaunchKernelGGL((faces_kernel<true, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
     }
     TWG_LAUNCHED();
     unsigned eg = (unsigned)std::min<int64_t>(grid_for(n), 64);

@@ -39,6 +39,7 @@ static_assert(sizeof(BvhNode32) == 32, "node32");
 
 struct twg_bvh {
     int device = 0;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // root box
     int64_t nf = 0, nnode = 0;
     int depth = 0;
     double rmax = 0.0;

@@ -18,6 +18,9 @@ struct twg_ctx {
     // growable device scratch owned by the context
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // further growable slots (query ordering keys, values, sort temp storage)
+    void* slot[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t slot_bytes[4] = {0, 0, 0, 0};
 };
 
 namespace twg {
@@ -28,6 +31,14 @@ void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 int ensure_scratch(twg_ctx* ctx, size_t bytes);
+int ensure_slot(twg_ctx* ctx, int i, size_t bytes);
+
+// Morton order of n query items (points: stride 3, triangles: stride 9,
+// keyed by their first three coordinates' centroid) inside the box [lo, hi]:
+// writes a permutation (int32, device) into *order (a context slot).  The
+// order only affects speed: every query's result is independent of it.
+int morton_order(twg_ctx* ctx, const double* items, int64_t n, int stride, const double* lo,
+                 const double* hi, const int32_t** order);
 
 // Activates ctx->device for the calling thread.
 int bind(twg_ctx* ctx);

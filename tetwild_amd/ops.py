@@ -91,6 +91,23 @@ class Context:
                                       _ptr(E), _ptr(G), _ptr(H)), "twg_amips_eval")
         return E, G, H
 
+    def amips_launcher(self, tets, x, y, z, mode, out):
+        """A zero-argument callable that enqueues twg_amips_eval with arguments
+        checked once (for timing loops: no per-call Python validation)."""
+        n = self._tets(tets)
+        nv = self._verts(x, y, z)
+        E, G, H = out
+        self._bind()
+        f = self.L.twg_amips_eval
+        args = (self.h, _ptr(tets), n, _ptr(x), _ptr(y), _ptr(z), nv, mode, _ptr(E), _ptr(G),
+                _ptr(H))
+
+        def launch():
+            r = f(*args)
+            if r:
+                C.check(r, "twg_amips_eval")
+        return launch
+
     def amips_eval_mixed(self, tets, x, y, z, mode=C.TWG_AMIPS_EGH, tol=2.0**-16, out=None):
         """E (n) float64, G (n, 3) / H (n, 9) float32, fallback (n) uint8."""
         n = self._tets(tets)
