@@ -16,11 +16,15 @@ Secondary measurements in the same JSON line ("secondary"):
                  20% surface vertices on the bumpy sphere), autotuned tiles,
                  one persistent launch replayed from a hipGraph, FP64 and mixed,
                  against one launch per problem
+  collapse       collapseAnEdge's checks for 1M directed-edge candidates of a
+                 jittered 80^3 Kuhn lattice carrying an open surface sheet,
+                 autotuned tiles, the batch replayed from a hipGraph, FP64 and
+                 mixed
 Each carries its CPU baseline (the CPU restatement, one host core, bounded
 sample) and its parity counts against that restatement.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
-                  [--only amips,envelope,slivers,smoothing]
+                  [--only amips,envelope,slivers,smoothing,collapse]
 
 Prints ONE JSON line.  The roofline object covers the headline kernel:
 algorithmic bytes per launch / average launch time from HIP events on the
@@ -369,6 +373,79 @@ def bench_smoothing(ctx, args, result, surf_cache):
     result["secondary"]["smoothing"] = sec
 
 
+# ------------------------------------------------------------ (f) --------
+def bench_collapse(ctx, args, result):
+    """collapseAnEdge's checks for 1M directed-edge candidates of a jittered
+    80^3 Kuhn lattice (3.07M tets) carrying an open surface sheet at mid-height (eps =
+    sampling_dist = 0.05 h): autotuned, one hipGraph replay of the batch per step"""
+    import numpy as np
+    import torch
+    from tetwild_amd import workloads as W
+    from tetwild_amd.ops import CollapseBatch
+
+    cubes = 20 if args.quick else 80
+    n = 100_000 if args.quick else 1_000_000
+    t0 = time.time()
+    M = W.sheet_mesh(cubes, seed=0)
+    tx, ty, tz = (torch.from_numpy(M[k]).to(ctx.device) for k in ("x", "y", "z"))
+    tt = torch.from_numpy(M["tets"]).to(ctx.device)
+    q = ctx.tet_qualities(tt, tx, ty, tz)[0].cpu().numpy()
+    B = W.collapse_batch(M, q, n_cand=n, seed=0)
+    n = len(B["v1"])
+    eps = 0.05 / cubes
+    sf = ctx.surface(M["sf_V"], M["sf_F"])
+    bt = ctx.surface(M["b_V"], M["b_F"])
+    log(f"[collapse] {n} candidates on {len(M['tets'])} tets, {len(B['ring_tets'])} ring tets, "
+        f"{len(B['tri_ids'])} epsilon triangles (gen {time.time() - t0:.1f}s)")
+    batch = CollapseBatch(ctx, B, sf, bt, eps * eps, eps)
+    sec = {"candidates": n, "tets": int(len(M["tets"])), "ring_tets": int(len(B["ring_tets"])),
+           "eps_triangles": int(len(B["tri_ids"])),
+           "v1_surface_fraction": float((B["flags"] & 1).astype(bool).mean())}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        tile, chunk, _ = batch.autotune(precision=prec)
+        plan = batch.plan(precision=prec)
+        wall, kern = events_time(plan.run, max(2, args.steps // 4), 1)
+        sec[f"candidates_per_s_{name}"] = n / wall
+        sec[f"ring_tets_per_s_{name}"] = len(B["ring_tets"]) / wall
+        sec[f"graph_ms_{name}"] = kern * 1e3
+        sec[f"tile_{name}"] = tile
+        sec[f"chunk_{name}"] = chunk
+        plan.close()
+        if prec == 0:
+            st0 = batch.status[:n].clone()
+            nq0 = batch.new_q[:len(B["ring_tets"])].clone()
+    torch.cuda.synchronize()
+    st = batch.status[:n].cpu().numpy()
+    nq = batch.new_q[:len(B["ring_tets"])].cpu().numpy()
+    sec["status_mismatch_mixed_vs_fp64"] = int(np.count_nonzero(st != st0.cpu().numpy()))
+    sec["status_counts"] = {s: int(c) for s, c in
+                            zip(["success", "flip", "quality", "envelop", "envelop_suc", "-", "-",
+                                 "bad_input"], np.bincount(st, minlength=8)) if s != "-"}
+    del nq0
+    if not args.no_cpu:
+        from oracle import oracle as O
+        k = 20000 if args.quick else 200_000
+        sub = dict(B)
+        for key in ("v1", "v2", "flags"):
+            sub[key] = B[key][:k]
+        sub["ring_off"] = B["ring_off"][:k + 1]
+        sub["tri_off"] = B["tri_off"][:k + 1]
+        S, Sb = O.Surface(M["sf_V"], M["sf_F"]), O.Surface(M["b_V"], M["b_F"])
+        t0 = time.perf_counter()
+        so, nqo = O.collapse_batch(sub, S, Sb, eps * eps, eps)
+        el = time.perf_counter() - t0
+        sec["cpu_baseline"] = {"value": k / el, "unit": "candidates/s", "cores": 1,
+                               "kind": "port", "sample": f"first {k} candidates, CPU restatement"}
+        sec["status_mismatch_vs_cpu"] = int(np.count_nonzero(so != st[:k]))
+        ok = np.isin(so, [0, 4])
+        rows = np.repeat(ok, np.diff(B["ring_off"][:k + 1]))
+        r1 = int(B["ring_off"][k])
+        sec["new_q_bit_mismatch_vs_cpu"] = bits_mismatch(nq[:r1][rows], nqo[:r1][rows])
+    sf.close()
+    bt.close()
+    result["secondary"]["collapse"] = sec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +454,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller secondary workloads")
     ap.add_argument("--cubes", type=int, default=119)
-    ap.add_argument("--only", default="amips,envelope,slivers,smoothing")
+    ap.add_argument("--only", default="amips,envelope,slivers,smoothing,collapse")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -425,8 +502,11 @@ def main():
         torch.cuda.empty_cache()
     if "smoothing" in only:
         bench_smoothing(ctx, args, result, surf_cache)
+        torch.cuda.empty_cache()
     if "surf" in surf_cache:
         surf_cache["surf"].close()
+    if "collapse" in only:
+        bench_collapse(ctx, args, result)
     ctx.close()
     print(json.dumps(result), flush=True)
 

@@ -230,6 +230,127 @@ int twg_smooth_plan_run(twg_smooth_plan* plan);
 int twg_smooth_plan_info(const twg_smooth_plan* plan, int* tile, int* chunk);
 int twg_smooth_plan_destroy(twg_smooth_plan* plan);
 
+/* ---- edge-collapse candidates -------------------------------------------- */
+/* Return codes of EdgeCollapser::collapseAnEdge (src/tetwild/EdgeCollapser.h:52-56) */
+#define TWG_COLLAPSE_SUCCESS 0
+#define TWG_COLLAPSE_FLIP 1
+#define TWG_COLLAPSE_QUALITY 2
+#define TWG_COLLAPSE_ENVELOP 3
+#define TWG_COLLAPSE_ENVELOP_SUC 4
+#define TWG_COLLAPSE_BAD_INPUT 7 /* an index of this candidate is out of range */
+/* Candidate flags: the vertices' state after collapseAnEdge's isolated and
+ * boundary-point updates (src/tetwild/EdgeCollapser.cpp:298-307). */
+#define TWG_CV1_SURFACE 1
+#define TWG_CV1_BOUNDARY 2
+#define TWG_CV2_SURFACE 4
+#define TWG_CV2_BOUNDARY 8
+#define TWG_CV1_UNROUNDED 16 /* v1 not rounded: its new quality counts as 0 (:408) */
+
+/* A batch of independent collapse candidates v1 -> v2 on a frozen mesh.
+ * ring_tets holds v1's one-ring (conn_tets, as 4 vertex ids); the tets that
+ * contain v2 are the ones the collapse removes, the others become new tets
+ * with v1 replaced by v2 in place.  tri_ids holds, per candidate, the
+ * triangles isCollapsable_epsilon tests (src/tetwild/EdgeCollapser.cpp:742-764:
+ * unique sorted one-ring surface faces of v1 without v2, v1 replaced by v2).
+ * Every vertex of the new tets and triangles must be rounded (posf). */
+typedef struct twg_collapse_desc {
+    const double* x;
+    const double* y;
+    const double* z;
+    int64_t nv;
+    int64_t n;
+    const int32_t* v1;        /* [n] */
+    const int32_t* v2;        /* [n] */
+    const uint8_t* flags;     /* [n] TWG_CV* bits, or NULL (all interior, rounded) */
+    const int32_t* ring_off;  /* [n + 1] */
+    const int32_t* ring_tets; /* [n_ring][4], 16-byte aligned */
+    int64_t n_ring;
+    const double* old_q;      /* [n_ring] stored slim_energy of the ring tets */
+    const int32_t* tri_off;   /* [n + 1], or NULL (no epsilon triangles) */
+    const int32_t* tri_ids;   /* [n_tri][3] */
+    int64_t n_tri;
+    const twg_bvh* sf_tree;   /* needed when a candidate has TWG_CV1_SURFACE */
+    const twg_bvh* b_tree;    /* needed when a candidate has TWG_CV1_BOUNDARY */
+    double eps2;
+    double sampling_dist;
+    int check_quality;        /* energy_type != ENERGY_NA && is_check_quality */
+    int check_eps;            /* state.eps is neither EPSILON_NA nor EPSILON_INFINITE */
+    int is_soft;              /* EdgeCollapser::is_soft */
+    double soft_energy;       /* EdgeCollapser::soft_energy */
+    uint8_t* status;          /* [n] TWG_COLLAPSE_* */
+    double* new_q;            /* [n_ring] new slim_energy per ring tet (NaN for the
+                                 removed ones); what the caller stores on success */
+} twg_collapse_desc;
+
+/* Evaluates every candidate of the batch: the checks of collapseAnEdge up to
+ * its "real update" (src/tetwild/EdgeCollapser.cpp:284-446) -- boundary and
+ * surface point envelope of v2, isFlip and calTetQualities of the new tets,
+ * getCheckQuality / isBetterOrEqualThan, isCollapsable_epsilon -- with the
+ * reference's return code.  The one-ring checks run in one persistent launch
+ * (tile = 16, 32 or 64 lanes per candidate, chunk = candidates per queue pop;
+ * 0 = tuned or default).  Codes and new_q are identical to the CPU
+ * restatement for any tile and chunk.  precision TWG_MIXED walks the envelope
+ * trees with FP32 boxes (identical decisions).  new_q is complete when the
+ * status is SUCCESS or ENVELOP_SUC. */
+int twg_collapse_batch(twg_ctx* ctx, const twg_collapse_desc* desc, int precision, int tile,
+                       int chunk);
+/* Times every (tile, chunk) pair on this batch and remembers the fastest for
+ * batches of a similar size on this device; the outputs are written as
+ * twg_collapse_batch would write them. */
+int twg_collapse_autotune(twg_ctx* ctx, const twg_collapse_desc* desc, int precision, int* tile,
+                          int* chunk, float* ms);
+/* The batch's launches captured once into a hipGraph (queue reset, persistent
+ * ring pass, exact ring pass, point, face, exact face and final passes) and
+ * replayed by twg_collapse_plan_run on the context's stream.  The plan owns
+ * its scratch; the descriptor's arrays must stay valid while the plan lives. */
+typedef struct twg_collapse_plan twg_collapse_plan;
+int twg_collapse_plan_create(twg_ctx* ctx, const twg_collapse_desc* desc, int precision, int tile,
+                             int chunk, twg_collapse_plan** out);
+int twg_collapse_plan_run(twg_collapse_plan* plan);
+int twg_collapse_plan_info(const twg_collapse_plan* plan, int* tile, int* chunk);
+int twg_collapse_plan_destroy(twg_collapse_plan* plan);
+
+/* ---- configurations of a few tets (edge removal) ------------------------ */
+#define TWG_GROUP_BAD_INPUT 2 /* flipped[g]: an index of the group is out of range */
+
+/* For every group g of tets (tets[group_off[g] .. group_off[g+1]), int32 x4):
+ * flipped[g] = isFlip(group) (src/tetwild/LocalOperations.cpp:555-573, exact
+ * orientation on posf); when not flipped, q[k] = calTetQualities of each tet
+ * (:695-773, 862-884; q may be NULL) and max_q[g] = getCheckQuality's max
+ * (:575-588, max over the group starting from 0), else max_q[g] = NaN.  The
+ * evaluation step of EdgeRemover::removeAnEdge_32 / _44 / _56
+ * (src/tetwild/EdgeRemover.cpp:138-174, 327-368, 501-575): one group per
+ * configuration; the caller keeps the selection logic.  One thread per group:
+ * meant for small groups (a few tets).  Identical to the CPU restatement. */
+int twg_tet_groups(twg_ctx* ctx, const int32_t* tets, const int32_t* group_off, int64_t n_groups,
+                   int64_t n_tets, const double* x, const double* y, const double* z, int64_t nv,
+                   uint8_t* flipped, double* max_q, double* q);
+
+/* ---- energy reductions ---------------------------------------------------- */
+typedef struct twg_energy_summary {
+    double sum;          /* sum of slim_energy over counted tets */
+    double max;          /* getMaxEnergy: max over counted tets (from 0) */
+    double second_max;   /* getSecondMaxEnergy: same, excluding MAX_ENERGY */
+    int64_t count;       /* tets neither removed nor locked */
+    int64_t buckets[11]; /* getFilterEnergy's histogram (removed tets skipped,
+                            locks ignored, as in the reference) */
+} twg_energy_summary;
+
+/* One pass over the per-tet slim energies (device arrays): the quantities of
+ * LocalOperations::getAvgMaxEnergy, getMaxEnergy, getSecondMaxEnergy and
+ * getFilterEnergy (src/tetwild/LocalOperations.cpp:605-693) that drive pass
+ * termination in MeshRefinement (src/tetwild/MeshRefinement.cpp:185-206,
+ * 285-292).  t_is_removed (uint8, may be NULL), tets + v_is_locked (uint8 per
+ * vertex; both NULL = nothing locked) give isTetLocked_ui (:1377-1384).  The
+ * bucket limits are filter_energy_thres - 1 + 10^j as the reference forms
+ * them.  avg = sum / count (inf -> MAX_ENERGY) is the caller's.  Max, second
+ * max, count and buckets are exact; sum is a deterministic reduction (within
+ * n 2^-53 sum|e| of the reference's sequential sum).  Synchronous: *out is a
+ * HOST struct, filled when the call returns. */
+int twg_energy_stats(twg_ctx* ctx, const double* slim_energy, int64_t n,
+                     const uint8_t* t_is_removed, const int32_t* tets, const uint8_t* v_is_locked,
+                     int64_t nv, double filter_energy_thres, twg_energy_summary* out);
+
 #ifdef __cplusplus
 }
 #endif

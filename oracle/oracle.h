@@ -25,6 +25,21 @@
  *                         agreement with geogram vec3 arithmetic: parity unpinned.
  *   orc_envelope_*        brute force over all facets (exact-rational decisions
  *                         away from the eps^2 rounding band).
+ *   orc_collapse_check    composition of the pinned parts above (orient3d,
+ *                         qualities, point / face envelope) in the reference's
+ *                         check order; the order and the early exits are pinned
+ *                         by constructed candidates whose outcome geometry
+ *                         fixes (tests/test_oracle_collapse.py).
+ *   orc_energy_stats, orc_filter_energy  constructed histograms and values on
+ *                         the bucket limits (tests/test_oracle_reduce.py);
+ *                         the sum checked against exact rational sums.
+ *   orc_tet_groups        composition of orient3d and tet qualities, checked
+ *                         against them on constructed configurations.
+ *   orc_remove_edge       constructed rings of 3, 4 and 5 tets around an edge:
+ *                         the new tets must be the geometric re-triangulation
+ *                         (face / diagonal / pentagon fan coned to v1 and v2)
+ *                         and the decision the quality comparison
+ *                         (tests/test_oracle_remove.py).
  *   orc_newton_*          invariants (monotone energy, no flips, step bounds,
  *                         is_moved semantics, X = X0 - a H^-1 J).  Bit-level
  *                         agreement with Eigen's ColPivHouseholderQR:
@@ -132,6 +147,56 @@ int orc_smooth_problem(int kind, int32_t v, const int32_t* ring_tets, int m, con
                        const int32_t* tri_ab, int ntri, const orc_surface* sf,
                        const orc_surface* bt, double eps2, double d, double* p_out,
                        double* new_q);
+
+/* ---- edge-collapse candidates, src/tetwild/EdgeCollapser.cpp:284-446 ---- */
+/* return codes: the reference's (src/tetwild/EdgeCollapser.h:52-56) */
+#define ORC_COLLAPSE_SUCCESS 0
+#define ORC_COLLAPSE_FLIP 1
+#define ORC_COLLAPSE_QUALITY 2
+#define ORC_COLLAPSE_ENVELOP 3
+#define ORC_COLLAPSE_ENVELOP_SUC 4
+/* candidate flags (vertex state after the isolated / boundary-point updates) */
+#define ORC_CV1_SURFACE 1
+#define ORC_CV1_BOUNDARY 2
+#define ORC_CV2_SURFACE 4
+#define ORC_CV2_BOUNDARY 8
+#define ORC_CV1_UNROUNDED 16
+/* One candidate v1 -> v2.  ring_tets: v1's m one-ring tets (int32 x4);
+ * old_q: their stored slim energies; tri_ids: ntri epsilon-check triangles
+ * (vertex ids, v2 already substituted).  new_q (m) receives the new tets'
+ * qualities (NaN for ring tets containing v2) once the quality stage is
+ * reached.  Returns the reference's code. */
+int orc_collapse_check(int32_t v1, int32_t v2, int flags, const int32_t* ring_tets, int m,
+                       const double* x, const double* y, const double* z, const double* old_q,
+                       const int32_t* tri_ids, int ntri, const orc_surface* sf,
+                       const orc_surface* bt, double eps2, double d, int check_quality,
+                       int check_eps, int is_soft, double soft_energy, double* new_q);
+
+/* ---- energy reductions, src/tetwild/LocalOperations.cpp:605-693 ----------- */
+/* sum / max / second max / count over tets neither removed nor locked (sum in
+ * index order), and getFilterEnergy's 11-bucket histogram over non-removed
+ * tets.  removed, tets + v_locked may be NULL. */
+void orc_energy_stats(const double* e, int64_t n, const uint8_t* removed, const int32_t* tets,
+                      const uint8_t* v_locked, double thres, double* sum, double* max,
+                      double* second_max, int64_t* count, int64_t* buckets);
+/* getFilterEnergy's decision from the histogram; sets *is_clean_up only where
+ * the reference does */
+double orc_filter_energy(const int64_t* buckets, double thres, int* is_clean_up);
+
+/* ---- configurations (EdgeRemover), src/tetwild/EdgeRemover.cpp ------------ */
+/* per group: isFlip; when not flipped the qualities (q may be NULL) and their
+ * max from 0; NaN max when flipped */
+void orc_tet_groups(const int32_t* tets, const int32_t* off, int64_t n_groups, const double* x,
+                    const double* y, const double* z, uint8_t* flipped, double* max_q, double* q);
+
+/* EdgeRemover::swap for one edge (v1, v2): ring = the m tets around the edge
+ * (int32 x4, the reference's old_t_ids order), old_q their stored qualities.
+ * Returns 3, 4 or 5 when removeAnEdge_32 / _44 / _56 succeeds (0 when none
+ * does), with the new tets (up to 6) and their qualities in the order the
+ * reference writes them; *equal_buget is updated as the reference updates it. */
+int orc_remove_edge(int32_t v1, int32_t v2, const int32_t* ring, int m, const double* old_q,
+                    const double* x, const double* y, const double* z, int* equal_buget,
+                    int32_t* new_tets, double* new_q, int* n_new);
 
 #ifdef __cplusplus
 }

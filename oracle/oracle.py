@@ -59,6 +59,14 @@ def lib():
             "orc_solve3_qrcp": (c_int, [P, P, P]),
             "orc_smooth_problem": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P, c_int,
                                            P, P, c_dbl, c_dbl, P, P]),
+            "orc_energy_stats": (None, [P, c_i64, P, P, P, c_dbl, P, P, P, P, P]),
+            "orc_filter_energy": (c_dbl, [P, c_dbl, P]),
+            "orc_tet_groups": (None, [P, P, c_i64, P, P, P, P, P, P]),
+            "orc_remove_edge": (c_int, [ctypes.c_int32, ctypes.c_int32, P, c_int, P, P, P, P, P,
+                                        P, P, P]),
+            "orc_collapse_check": (c_int, [ctypes.c_int32, ctypes.c_int32, c_int, P, c_int, P, P,
+                                           P, P, P, c_int, P, P, c_dbl, c_dbl, c_int, c_int,
+                                           c_int, c_dbl, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -304,3 +312,84 @@ def smooth_batch(B, old_q=None, sf=None, bt=None, eps2=0.0, d=1.0):
         if s == 0 and int(B["kind"][p]) in (1, 2):
             new_q[r0:r1] = nq[:r1 - r0]
     return p_out, status, new_q
+
+
+def collapse_batch(B, sf=None, bt=None, eps2=0.0, d=1.0, check_quality=True, check_eps=True,
+                   soft_energy=None):
+    """Per-candidate oracle over a batch dict (workloads.collapse_batch layout):
+    returns status (n) and new_q (n_ring; NaN for removed tets, 0 where the
+    quality stage was not reached)."""
+    L = lib()
+    x, y, z = _f64(B["x"]), _f64(B["y"]), _f64(B["z"])
+    ro, to = B["ring_off"], B["tri_off"]
+    rt_all = np.ascontiguousarray(B["ring_tets"], dtype=np.int32).reshape(-1, 4)
+    tri_all = np.ascontiguousarray(B["tri_ids"], dtype=np.int32).reshape(-1, 3)
+    oq = _f64(B["old_q"]) if len(rt_all) else np.zeros(1)
+    fl = B["flags"]
+    n = len(B["v1"])
+    status = np.empty(n, dtype=np.uint8)
+    new_q = np.zeros(max(len(rt_all), 1))
+    for c in range(n):
+        r0, r1 = int(ro[c]), int(ro[c + 1])
+        t0, t1 = int(to[c]), int(to[c + 1])
+        rt = np.ascontiguousarray(rt_all[r0:r1]) if r1 > r0 else np.zeros((1, 4), np.int32)
+        tr = np.ascontiguousarray(tri_all[t0:t1]) if t1 > t0 else np.zeros((1, 3), np.int32)
+        q = np.ascontiguousarray(oq[r0:r1]) if r1 > r0 else np.zeros(1)
+        nq = np.zeros(max(r1 - r0, 1))
+        status[c] = L.orc_collapse_check(
+            int(B["v1"][c]), int(B["v2"][c]), int(fl[c]), _p(rt), r1 - r0, _p(x), _p(y), _p(z),
+            _p(q), _p(tr), t1 - t0, sf.h if sf is not None else None,
+            bt.h if bt is not None else None, float(eps2), float(d), int(bool(check_quality)),
+            int(bool(check_eps)), int(soft_energy is not None),
+            float(soft_energy) if soft_energy is not None else 0.0, _p(nq))
+        if r1 > r0:
+            new_q[r0:r1] = nq[:r1 - r0]
+    return status, new_q
+
+
+def energy_stats(e, removed=None, tets=None, locked=None, thres=10.0):
+    """(sum, max, second_max, count, buckets[11]) as the reference's loops"""
+    e = _f64(e)
+    out = [np.zeros(1) for _ in range(3)]
+    cnt = np.zeros(1, np.int64)
+    b = np.zeros(11, np.int64)
+    rm = None if removed is None else np.ascontiguousarray(removed, dtype=np.uint8)
+    tt = None if tets is None else np.ascontiguousarray(tets, dtype=np.int32)
+    lk = None if locked is None else np.ascontiguousarray(locked, dtype=np.uint8)
+    lib().orc_energy_stats(_p(e), len(e), None if rm is None else _p(rm),
+                           None if tt is None else _p(tt), None if lk is None else _p(lk),
+                           float(thres), _p(out[0]), _p(out[1]), _p(out[2]), _p(cnt), _p(b))
+    return out[0][0], out[1][0], out[2][0], int(cnt[0]), b
+
+
+def filter_energy(buckets, thres=10.0):
+    """getFilterEnergy's return value and is_clean_up (None where not set)"""
+    b = np.ascontiguousarray(buckets, dtype=np.int64)
+    flag = np.full(1, -1, dtype=np.int32)
+    v = lib().orc_filter_energy(_p(b), float(thres), _p(flag))
+    return v, (None if flag[0] < 0 else bool(flag[0]))
+
+
+def tet_groups(tets, off, x, y, z):
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    off = np.ascontiguousarray(off, dtype=np.int32)
+    n = len(off) - 1
+    fl = np.zeros(max(n, 1), np.uint8)
+    mq = np.zeros(max(n, 1))
+    q = np.zeros(max(len(tets), 1))
+    lib().orc_tet_groups(_p(tets), _p(off), n, _p(_f64(x)), _p(_f64(y)), _p(_f64(z)), _p(fl),
+                         _p(mq), _p(q))
+    return fl[:n], mq[:n], q[:len(tets)]
+
+
+def remove_edge(v1, v2, ring, old_q, x, y, z, equal_buget=0):
+    """(code 0/3/4/5, new tets (k, 4), new qualities (k,), equal_buget after)"""
+    ring = np.ascontiguousarray(ring, dtype=np.int32).reshape(-1, 4)
+    oq = _f64(old_q)
+    nt = np.zeros((6, 4), np.int32)
+    nq = np.zeros(6)
+    k = np.zeros(1, np.int32)
+    eb = np.array([equal_buget], np.int32)
+    code = lib().orc_remove_edge(int(v1), int(v2), _p(ring), len(ring), _p(oq), _p(_f64(x)),
+                                 _p(_f64(y)), _p(_f64(z)), _p(eb), _p(nt), _p(nq), _p(k))
+    return code, nt[:k[0]].copy(), nq[:k[0]].copy(), int(eb[0])

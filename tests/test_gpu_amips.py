@@ -173,3 +173,60 @@ def test_large_lattice_checksum(ctx, dev):
     Eo, Go, Ho = O.amips_batch(tets, x, y, z, 2)
     assert np.array_equal(bits(E.cpu().numpy()), bits(Eo))
     assert np.array_equal(bits(H.cpu().numpy()), bits(Ho))
+
+
+def _near_degenerate(n, seed):
+    """n tets of integer points, the fourth point on the plane of the first
+    three, then a third of them nudged by one ulp: the orientation filter
+    cannot decide any of them"""
+    st = W.Stream(seed, 0)
+    a = st.integers(9 * n, -1000, 1000).reshape(n, 3, 3).astype(float)
+    uv = st.integers(2 * n, -3, 3).reshape(n, 2).astype(float)
+    d = a[:, 0] + uv[:, :1] * (a[:, 1] - a[:, 0]) + uv[:, 1:] * (a[:, 2] - a[:, 0])
+    P = np.concatenate([a, d[:, None]], 1)
+    k = st.integers(n, 0, 3)
+    c = st.integers(n, 0, 2)
+    m = st.integers(n, 0, 2)
+    r = np.arange(n)
+    v = P[r, k, c]
+    P[r, k, c] = np.where(m == 1, np.nextafter(v, np.inf), np.where(m == 2, np.nextafter(v, -np.inf), v))
+    return P.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("case", ["queue", "rescan"])
+def test_split_orientation_passes(ctx, dev, case):
+    """batches of >= 64k tets decide orientation with the filter and finish the
+    undecided tets in a second pass: from the queue (a few thousand), or by
+    rescanning the outputs when more than 65536 are undecided"""
+    if case == "queue":
+        x, y, z, tets, _ = W.kuhn_lattice(40, seed=2)
+        P = _near_degenerate(3000, seed=21)
+        nv = len(x)
+        x = np.concatenate([x, P[:, 0]])
+        y = np.concatenate([y, P[:, 1]])
+        z = np.concatenate([z, P[:, 2]])
+        extra = (nv + np.arange(4 * 3000)).reshape(-1, 4)
+        allt = np.concatenate([tets, extra]).astype(np.int32)
+        perm = np.argsort(W.Stream(22, 0).uniform(len(allt)))
+        tets = np.ascontiguousarray(allt[perm])
+    else:
+        P = _near_degenerate(100_000, seed=23)
+        x, y, z = (np.ascontiguousarray(P[:, j]) for j in range(3))
+        tets = np.arange(len(P), dtype=np.int32).reshape(-1, 4)
+    assert len(tets) >= 1 << 16
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    q, _ = ctx.tet_qualities(tt, tx, ty, tz)
+    fl = ctx.tet_flip(tt, tx, ty, tz)
+    qm, fbm = ctx.tet_qualities(tt, tx, ty, tz, precision=1, tol=2.0**-16)
+    qo = O.tet_qualities(tets, x, y, z)
+    flo = O.tet_flip(tets, x, y, z)
+    assert np.array_equal(bits(q.cpu().numpy()), bits(qo))
+    assert np.array_equal(fl.cpu().numpy(), flo)
+    qm = qm.cpu().numpy()
+    fbm = fbm.cpu().numpy().astype(bool)
+    assert np.array_equal(qm == 1e50, qo == 1e50)
+    assert np.array_equal(bits(qm[fbm]), bits(qo[fbm]))
+    ok = qo < 1e50
+    assert np.all(np.abs(qm[ok] - qo[ok]) <= 2.0**-16 * qo[ok])
+    if case == "rescan":
+        assert 0 < flo.sum() < len(flo) and (qo == 1e50).sum() > 1 << 16

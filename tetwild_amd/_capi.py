@@ -55,6 +55,14 @@ SIGNATURES = {
     "twg_smooth_plan_run": [P],
     "twg_smooth_plan_info": [P, P, P],
     "twg_smooth_plan_destroy": [P],
+    "twg_collapse_batch": [P, P, i32, i32, i32],
+    "twg_collapse_autotune": [P, P, i32, P, P, P],
+    "twg_collapse_plan_create": [P, P, i32, i32, i32, ctypes.POINTER(P)],
+    "twg_collapse_plan_run": [P],
+    "twg_collapse_plan_info": [P, P, P],
+    "twg_collapse_plan_destroy": [P],
+    "twg_tet_groups": [P, P, P, i64, i64, P, P, P, i64, P, P, P],
+    "twg_energy_stats": [P, P, i64, P, P, P, i64, f64, P],
 }
 _RESTYPES = {"twg_last_error": ctypes.c_char_p, "twg_ctx_stream": P}
 
@@ -69,6 +77,39 @@ class SmoothDesc(ctypes.Structure):
                 ("tri_off", P), ("tri_ab", P), ("n_tri", i64), ("sf_tree", P), ("b_tree", P),
                 ("eps2", f64), ("sampling_dist", f64), ("p_out", P), ("status", P),
                 ("new_q", P)]
+
+# return codes of EdgeCollapser::collapseAnEdge, and BAD_INPUT
+TWG_COLLAPSE_SUCCESS = 0
+TWG_COLLAPSE_FLIP = 1
+TWG_COLLAPSE_QUALITY = 2
+TWG_COLLAPSE_ENVELOP = 3
+TWG_COLLAPSE_ENVELOP_SUC = 4
+TWG_COLLAPSE_BAD_INPUT = 7
+TWG_CV1_SURFACE = 1
+TWG_CV1_BOUNDARY = 2
+TWG_CV2_SURFACE = 4
+TWG_CV2_BOUNDARY = 8
+TWG_CV1_UNROUNDED = 16
+
+
+class CollapseDesc(ctypes.Structure):
+    """twg_collapse_desc"""
+    _fields_ = [("x", P), ("y", P), ("z", P), ("nv", i64), ("n", i64), ("v1", P), ("v2", P),
+                ("flags", P), ("ring_off", P), ("ring_tets", P), ("n_ring", i64),
+                ("old_q", P), ("tri_off", P), ("tri_ids", P), ("n_tri", i64),
+                ("sf_tree", P), ("b_tree", P), ("eps2", f64), ("sampling_dist", f64),
+                ("check_quality", i32), ("check_eps", i32), ("is_soft", i32),
+                ("soft_energy", f64), ("status", P), ("new_q", P)]
+
+
+TWG_GROUP_BAD_INPUT = 2
+
+
+class EnergySummary(ctypes.Structure):
+    """twg_energy_summary"""
+    _fields_ = [("sum", f64), ("max", f64), ("second_max", f64), ("count", i64),
+                ("buckets", i64 * 11)]
+
 
 _LIB = None
 

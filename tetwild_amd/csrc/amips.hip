@@ -193,59 +193,172 @@ __device__ __forceinline__ double quality_from(double e)
     return e;
 }
 
+// ---- calTetQualities and isTetFlip ----------------------------------------
+// Large batches run in two passes: the main kernel decides the orientation
+// with the floating-point filter only and queues the tets it cannot decide
+// (a marker goes to their output); a small kernel then finishes those with
+// the exact predicate.  Keeping the exact predicate out of the main kernel
+// keeps its register count (and stack) small.  Small batches use one kernel
+// that calls the exact predicate inline.  Outputs are identical either way.
+
+// 0x7ff8dead00000000: a NaN no quality result carries (bad input writes the
+// default NaN)
+constexpr uint64_t kPendingBits = 0x7ff8dead00000000ull;
+constexpr uint8_t kPendingFlip = 2;
+constexpr unsigned kPendingCap = 1u << 16;
+constexpr int64_t kSplitMin = 1 << 16;
+
+struct Pending {
+    unsigned* count;
+    int64_t* list;
+};
+
+// appends i (where pend) with one atomic per wave; every active lane calls it
+__device__ __forceinline__ void push_pending(const Pending& P, int64_t i, bool pend)
+{
+    const unsigned long long m = __ballot(pend);
+    if (m == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(P.count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (pend) {
+        const unsigned k = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        if (k < kPendingCap) P.list[k] = i;
+    }
+}
+
+// quality of tet i given its orientation o (+1, -1 or 0); calTetQuality_AMIPS
+// semantics, src/tetwild/LocalOperations.cpp:862-884
 template <bool MIXED>
+__device__ __forceinline__ double quality_oriented(const Tet& T, int o, float tol, uint8_t& fb)
+{
+    fb = 0;
+    if (o != 1) return kMaxEnergy;
+    if (!MIXED) return quality_from(amips_energy(T));
+    float d[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d[0][k] = (float)(T.p[1][k] - T.p[0][k]);
+        d[1][k] = (float)(T.p[2][k] - T.p[0][k]);
+        d[2][k] = (float)(T.p[3][k] - T.p[0][k]);
+    }
+    AmipsCore32 c = amips_core32(d);
+    if (c.rel <= tol && fabsf_(c.E) < __builtin_inff() && c.E > 0.0f) return (double)c.E;
+    fb = 1;
+    return quality_from(amips_energy(T));
+}
+
+// SPLIT: filter only, undecided tets queued; otherwise the exact predicate inline
+template <bool MIXED, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void tet_quality_kernel(
     const int32_t* __restrict__ tets, int64_t n, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, int64_t nv, float tol,
-    double* __restrict__ out, uint8_t* __restrict__ fallback)
+    double* __restrict__ out, uint8_t* __restrict__ fallback, Pending P)
 {
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     Tet T;
     uint8_t fb = 0;
     double e;
+    bool pend = false;
     if (!load_tet(tets, i, x, y, z, nv, T)) {
         e = __builtin_nan("");
         fb = 1;
-    } else if (orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1) {
-        e = kMaxEnergy;
-    } else if (!MIXED) {
-        e = quality_from(amips_energy(T));
     } else {
-        float d[3][3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            d[0][k] = (float)(T.p[1][k] - T.p[0][k]);
-            d[1][k] = (float)(T.p[2][k] - T.p[0][k]);
-            d[2][k] = (float)(T.p[3][k] - T.p[0][k]);
-        }
-        AmipsCore32 c = amips_core32(d);
-        if (c.rel <= tol && fabsf_(c.E) < __builtin_inff() && c.E > 0.0f) {
-            e = (double)c.E;
+        int o = SPLIT ? orient3d_filter(T.p[0], T.p[1], T.p[2], T.p[3])
+                      : orient3d(T.p[0], T.p[1], T.p[2], T.p[3]);
+        if (SPLIT && o == kUndecided) {
+            pend = true;
+            e = __builtin_bit_cast(double, kPendingBits);
         } else {
-            e = quality_from(amips_energy(T));
-            fb = 1;
+            e = quality_oriented<MIXED>(T, o, tol, fb);
         }
     }
     out[i] = e;
     if (fallback) fallback[i] = fb;
+    if (SPLIT) push_pending(P, i, pend);
 }
 
+// the queued tets with the exact predicate; when the queue overflowed, every
+// tet still carrying the marker
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void tet_quality_exact_kernel(
+    const int32_t* __restrict__ tets, int64_t n, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int64_t nv, float tol,
+    double* __restrict__ out, uint8_t* __restrict__ fallback, Pending P)
+{
+    const unsigned cnt = *P.count;
+    if (cnt == 0) return;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const bool listed = cnt <= kPendingCap;
+    const int64_t m = listed ? (int64_t)cnt : n;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
+        int64_t i = listed ? P.list[k] : k;
+        if (!listed && __builtin_bit_cast(uint64_t, out[i]) != kPendingBits) continue;
+        Tet T;
+        load_tet(tets, i, x, y, z, nv, T);  // in range: only loaded tets are queued
+        uint8_t fb;
+        out[i] = quality_oriented<MIXED>(T, orient3d(T.p[0], T.p[1], T.p[2], T.p[3]), tol, fb);
+        if (fallback) fallback[i] = fb;
+    }
+}
+
+template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void tet_flip_kernel(const int32_t* __restrict__ tets,
                                                           int64_t n, const double* __restrict__ x,
                                                           const double* __restrict__ y,
                                                           const double* __restrict__ z, int64_t nv,
-                                                          uint8_t* __restrict__ flipped)
+                                                          uint8_t* __restrict__ flipped, Pending P)
 {
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     Tet T;
+    bool pend = false;
     if (!load_tet(tets, i, x, y, z, nv, T)) {
         flipped[i] = 1;
-        return;
+    } else {
+        int o = SPLIT ? orient3d_filter(T.p[0], T.p[1], T.p[2], T.p[3])
+                      : orient3d(T.p[0], T.p[1], T.p[2], T.p[3]);
+        pend = SPLIT && o == kUndecided;
+        flipped[i] = pend ? kPendingFlip : (o != 1);
     }
-    flipped[i] = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
+    if (SPLIT) push_pending(P, i, pend);
 }
+
+__global__ __launch_bounds__(kBlock) void tet_flip_exact_kernel(
+    const int32_t* __restrict__ tets, int64_t n, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int64_t nv,
+    uint8_t* __restrict__ flipped, Pending P)
+{
+    const unsigned cnt = *P.count;
+    if (cnt == 0) return;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const bool listed = cnt <= kPendingCap;
+    const int64_t m = listed ? (int64_t)cnt : n;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
+        int64_t i = listed ? P.list[k] : k;
+        if (!listed && flipped[i] != kPendingFlip) continue;
+        Tet T;
+        load_tet(tets, i, x, y, z, nv, T);
+        flipped[i] = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
+    }
+}
+
+__global__ void zero_count(unsigned* c) { *c = 0u; }
+
+// the context slot holding the queue: count, then the list (256-byte aligned)
+int pending_slot(twg_ctx* ctx, Pending& P)
+{
+    if (int r = ensure_slot(ctx, 0, 256 + sizeof(int64_t) * kPendingCap)) return r;
+    P.count = static_cast<unsigned*>(ctx->slot[0]);
+    P.list = reinterpret_cast<int64_t*>(static_cast<char*>(ctx->slot[0]) + 256);
+    return TWG_OK;
+}
+
+// grid of the exact pass: enough for a full queue, bounded for the rescan
+constexpr unsigned kExactBlocks = 1024;
 
 int check_common(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x, const double* y,
                  const double* z, int64_t nv, const char* fn)
@@ -323,12 +436,36 @@ int twg_tet_qualities(twg_ctx* ctx, const int32_t* tets, int64_t n, const double
     if (n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
     dim3 g(grid_for(n)), b(kBlock);
-    if (precision == TWG_FP64)
-        hipLaunchKernelGGL(tet_quality_kernel<false>, g, b, 0, ctx->cur, tets, n, x, y, z, nv,
-                           (float)tol, slim_energy, fallback);
-    else
-        hipLaunchKernelGGL(tet_quality_kernel<true>, g, b, 0, ctx->cur, tets, n, x, y, z, nv,
-                           (float)tol, slim_energy, fallback);
+    const float tf = (float)tol;
+    const bool mixed = precision == TWG_MIXED;
+    hipStream_t s = ctx->cur;
+    Pending P = {nullptr, nullptr};
+    if (n < kSplitMin) {
+        if (mixed)
+            hipLaunchKernelGGL((tet_quality_kernel<true, false>), g, b, 0, s, tets, n, x, y, z, nv,
+                               tf, slim_energy, fallback, P);
+        else
+            hipLaunchKernelGGL((tet_quality_kernel<false, false>), g, b, 0, s, tets, n, x, y, z,
+                               nv, tf, slim_energy, fallback, P);
+        TWG_LAUNCHED();
+        return TWG_OK;
+    }
+    if (int r = pending_slot(ctx, P)) return r;
+    hipLaunchKernelGGL(zero_count, dim3(1), dim3(1), 0, s, P.count);
+    TWG_LAUNCHED();
+    if (mixed) {
+        hipLaunchKernelGGL((tet_quality_kernel<true, true>), g, b, 0, s, tets, n, x, y, z, nv, tf,
+                           slim_energy, fallback, P);
+        TWG_LAUNCHED();
+        hipLaunchKernelGGL(tet_quality_exact_kernel<true>, dim3(kExactBlocks), b, 0, s, tets, n,
+                           x, y, z, nv, tf, slim_energy, fallback, P);
+    } else {
+        hipLaunchKernelGGL((tet_quality_kernel<false, true>), g, b, 0, s, tets, n, x, y, z, nv, tf,
+                           slim_energy, fallback, P);
+        TWG_LAUNCHED();
+        hipLaunchKernelGGL(tet_quality_exact_kernel<false>, dim3(kExactBlocks), b, 0, s, tets, n,
+                           x, y, z, nv, tf, slim_energy, fallback, P);
+    }
     TWG_LAUNCHED();
     return TWG_OK;
 }
@@ -340,8 +477,21 @@ int twg_tet_flip(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x, 
     TWG_REQUIRE(n == 0 || flipped, "twg_tet_flip: NULL output");
     if (n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
-    hipLaunchKernelGGL(tet_flip_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->cur, tets, n, x,
-                       y, z, nv, flipped);
+    dim3 g(grid_for(n)), b(kBlock);
+    hipStream_t s = ctx->cur;
+    Pending P = {nullptr, nullptr};
+    if (n < kSplitMin) {
+        hipLaunchKernelGGL(tet_flip_kernel<false>, g, b, 0, s, tets, n, x, y, z, nv, flipped, P);
+        TWG_LAUNCHED();
+        return TWG_OK;
+    }
+    if (int r = pending_slot(ctx, P)) return r;
+    hipLaunchKernelGGL(zero_count, dim3(1), dim3(1), 0, s, P.count);
+    TWG_LAUNCHED();
+    hipLaunchKernelGGL(tet_flip_kernel<true>, g, b, 0, s, tets, n, x, y, z, nv, flipped, P);
+    TWG_LAUNCHED();
+    hipLaunchKernelGGL(tet_flip_exact_kernel, dim3(kExactBlocks), b, 0, s, tets, n, x, y, z, nv,
+                       flipped, P);
     TWG_LAUNCHED();
     return TWG_OK;
 }

@@ -115,9 +115,8 @@ __global__ __launch_bounds__(kBlock) void sample_count_kernel(const double* __re
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double* t = tri9 + 9 * i;
-    V3<double> vs[3] = {ld3(t), ld3(t + 3), ld3(t + 6)};
     int64_t k = 0;
-    for_each_sample(vs, d, [&](V3<double>) {
+    for_each_sample(ld3(t), ld3(t + 3), ld3(t + 6), d, [&](V3<double>) {
         ++k;
         return false;
     });
@@ -133,9 +132,8 @@ __global__ __launch_bounds__(kBlock) void sample_write_kernel(const double* __re
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double* t = tri9 + 9 * i;
-    V3<double> vs[3] = {ld3(t), ld3(t + 3), ld3(t + 6)};
     int64_t k = offsets[i];
-    for_each_sample(vs, d, [&](V3<double> s) {
+    for_each_sample(ld3(t), ld3(t + 3), ld3(t + 6), d, [&](V3<double> s) {
         if (k >= 0 && k < total) {
             pts[3 * k + 0] = s.x;
             pts[3 * k + 1] = s.y;

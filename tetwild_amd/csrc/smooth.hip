@@ -77,7 +77,7 @@ template <int TILE>
 __device__ __forceinline__ bool tile_any(const TileCtx<TILE>& c, bool pred)
 {
     unsigned long long b = __ballot(pred);
-    unsigned long long m = TILE == 64 ? ~0ull : ((1ull << TILE) - 1ull);
+    unsigned long long m = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
     return ((b >> c.wbase) & m) != 0ull;
 }
 

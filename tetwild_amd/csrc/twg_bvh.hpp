@@ -169,7 +169,10 @@ __device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double 
 // nearest_facet with the reference's traversal order
 // (src/tetwild/geogram/mesh_AABB.h:130-141, mesh_AABB.cpp:381-480):
 // get_nearest_facet_hint, then depth-first search visiting the nearer child
-// first and pruning boxes whose signed distance is not below the best so far.
+// first (the right one on ties) and pruning boxes whose signed distance is not
+// below the best so far.  Stackless: one bit per level marks a far child still
+// to visit; its box distance is recomputed on the way back (the same value the
+// recursion keeps on its stack), so the walk needs no per-lane memory.
 __device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<double>& np,
                                              double& sq)
 {
@@ -182,43 +185,52 @@ __device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<
     int best = B.n64[nd].leaf;
     np = ld3(B.tri + 9 * (int64_t)best);
     sq = len2(np - p);
-    int64_t stk_n[64];
-    double stk_d[64];
-    int sp = 0;
-    stk_n[sp] = 1;
-    stk_d[sp] = -__builtin_inf();
-    sp++;
-    while (sp > 0) {
-        --sp;
-        int64_t nn = stk_n[sp];
-        double dn = stk_d[sp];
-        if (!(dn < sq)) continue;
-        int leaf = B.n64[nn].leaf;
-        if (leaf >= 0) {
-            const double* t = B.tri + 9 * (int64_t)leaf;
-            V3<double> cp;
-            double dd = point_tri_d2<double, true>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
-            if (dd < sq) {
-                best = leaf;
-                np = cp;
-                sq = dd;
-            }
-            continue;
+    auto visit = [&](int leaf) {
+        const double* t = B.tri + 9 * (int64_t)leaf;
+        V3<double> cp;
+        double dd = point_tri_d2<double, true>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
+        if (dd < sq) {
+            best = leaf;
+            np = cp;
+            sq = dd;
         }
-        const BvhNode64& L = B.n64[2 * nn];
-        const BvhNode64& R = B.n64[2 * nn + 1];
-        double dl = box_signed_d2(p, L.lo, L.hi);
-        double dr = box_signed_d2(p, R.lo, R.hi);
-        if (sp + 2 > 64) break;  // unreachable: depth <= 30 is enforced at build
-        if (dl < dr) {
-            stk_n[sp] = 2 * nn + 1; stk_d[sp] = dr; sp++;
-            stk_n[sp] = 2 * nn; stk_d[sp] = dl; sp++;
-        } else {
-            stk_n[sp] = 2 * nn; stk_d[sp] = dl; sp++;
-            stk_n[sp] = 2 * nn + 1; stk_d[sp] = dr; sp++;
+    };
+    if (B.n64[1].leaf >= 0) {  // a one-facet tree: the root is the leaf
+        visit(B.n64[1].leaf);
+        return best;
+    }
+    int64_t n = 1;
+    int depth = 0;
+    uint32_t trail = 0;  // bit d: the far child at depth d is still to visit
+    for (;;) {
+        // n is internal: the nearer child first
+        const int64_t c = 2 * n;
+        const double dl = box_signed_d2(p, B.n64[c].lo, B.n64[c].hi);
+        const double dr = box_signed_d2(p, B.n64[c + 1].lo, B.n64[c + 1].hi);
+        const bool left = dl < dr;
+        // the far child is never nearer, so a pruned near child prunes both
+        if ((left ? dl : dr) < sq) {
+            trail |= 1u << (depth + 1);
+            n = left ? c : c + 1;
+            depth++;
+            const int leaf = B.n64[n].leaf;
+            if (leaf < 0) continue;
+            visit(leaf);
+        }
+        // back to the deepest pending far child that can still hold a nearer facet
+        for (;;) {
+            if (trail == 0) return best;
+            const int d = 31 - __clz(trail);
+            trail &= ~(1u << d);
+            const int64_t far = (n >> (depth - d)) ^ 1;
+            n = far;
+            depth = d;
+            if (!(box_signed_d2(p, B.n64[far].lo, B.n64[far].hi) < sq)) continue;
+            const int leaf = B.n64[far].leaf;
+            if (leaf < 0) break;
+            visit(leaf);
         }
     }
-    return best;
 }
 
 // Triangles whose longest edge exceeds kMaxEdgeSamples sampling distances are
@@ -232,13 +244,13 @@ template <bool F32, bool STATS>
 __device__ bool face_out_sampled(const BvhView& B, const double* t9, double eps2, double d,
                                  double T, double sqrtT, Stats& st)
 {
-    V3<double> vs[3] = {ld3(t9), ld3(t9 + 3), ld3(t9 + 6)};
+    const V3<double> a = ld3(t9), b = ld3(t9 + 3), c = ld3(t9 + 6);
     {
-        double l = fmax(len2(vs[0] - vs[1]), fmax(len2(vs[1] - vs[2]), len2(vs[2] - vs[0])));
+        double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
         if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
     }
     int hint = -1;
-    return for_each_sample(vs, d, [&](V3<double> s) {
+    return for_each_sample(a, b, c, d, [&](V3<double> s) {
         if (STATS) st.samples++;
         return !in_envelope<F32, STATS>(B, s, eps2, T, sqrtT, hint, st);
     });

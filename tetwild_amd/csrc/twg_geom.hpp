@@ -181,6 +181,13 @@ TWG_HD R point_tri_d2(V3<R> p, V3<R> V0, V3<R> V1, V3<R> V2, V3<R>& cp)
     return d2;
 }
 
+// x0, x1 or x2 by i, component-wise selects on values
+__device__ __forceinline__ V3<double> sel3(int i, V3<double> x0, V3<double> x1, V3<double> x2)
+{
+    return mk3(i == 0 ? x0.x : (i == 1 ? x1.x : x2.x), i == 0 ? x0.y : (i == 1 ? x1.y : x2.y),
+               i == 0 ? x0.z : (i == 1 ? x1.z : x2.z));
+}
+
 // ---- sampleTriangle (src/tetwild/Common.cpp:143-228) -------------------
 // A resumable generator: next(p) yields the samples one at a time in the
 // reference's generation order (longest edge, v1, rows parallel to it, v2,
@@ -190,34 +197,34 @@ struct SampleGen {
     double d, sqrt3_2, ls1, ls2, N, sin_v0, tan_v0, sin_v1;
     int phase, k, m, M, N1;
 
-    __device__ __forceinline__ void init(const V3<double> in[3], double d_)
+    // the triangle arrives as three values (not an array): rotating an array
+    // by a run-time index would place it in scratch memory
+    __device__ __forceinline__ void init(V3<double> a, V3<double> b, V3<double> c, double d_)
     {
         d = d_;
-        const V3<double>* vs = in;
         sqrt3_2 = __builtin_sqrt(3.0) / 2;
-        double ls[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ls[i] = len2(vs[i] - vs[(i + 1) % 3]);
+        // squared edge lengths |v_i - v_(i+1)|^2
+        const double l0 = len2(a - b), l1 = len2(b - c), l2 = len2(c - a);
         int max_i = 0;
-        for (int i = 1; i < 3; ++i)
-            if (!(ls[i] < ls[max_i])) max_i = i;  // last largest, as std::minmax_element
-        N = __builtin_sqrt(ls[max_i]) / d;
+        double lsm = l0;
+        if (!(l1 < lsm)) { max_i = 1; lsm = l1; }  // last largest, as std::minmax_element
+        if (!(l2 < lsm)) { max_i = 2; lsm = l2; }
+        N = __builtin_sqrt(lsm) / d;
         k = 0;
         if (N <= 1) {
-            v0 = vs[0];
-            v1 = vs[1];
-            v2 = vs[2];
+            v0 = a;
+            v1 = b;
+            v2 = c;
             phase = 9;
             return;
         }
         if (N == (int)N) N -= 1;
-        v0 = vs[max_i];
-        v1 = vs[(max_i + 1) % 3];
-        v2 = vs[(max_i + 2) % 3];
-        ls1 = ls[(max_i + 1) % 3];
-        ls2 = ls[(max_i + 2) % 3];
+        v0 = sel3(max_i, a, b, c);
+        v1 = sel3(max_i, b, c, a);
+        v2 = sel3(max_i, c, a, b);
+        ls1 = max_i == 0 ? l1 : (max_i == 1 ? l2 : l0);
+        ls2 = max_i == 0 ? l2 : (max_i == 1 ? l0 : l1);
         n01 = normalize(v1 - v0);
-        double lsm = ls[max_i];
         V3<double> e01 = v1 - v0;
         double h = dist(divv(mulv(e01, dot(v2 - v0, e01)), lsm) + v0, v2);
         M = (int)(h / (sqrt3_2 * d));
@@ -230,7 +237,7 @@ struct SampleGen {
             switch (phase) {
             case 9:  // short triangle: its three vertices
                 if (k < 3) {
-                    p = k == 0 ? v0 : (k == 1 ? v1 : v2);
+                    p = sel3(k, v0, v1, v2);
                     k++;
                     return true;
                 }
@@ -337,10 +344,11 @@ struct SampleGen {
 // Calls emit(p) for every sample in order; emit returns true to stop early.
 // Returns true iff stopped early.
 template <class Emit>
-__device__ __forceinline__ bool for_each_sample(const V3<double> vs[3], double d, Emit&& emit)
+__device__ __forceinline__ bool for_each_sample(V3<double> a, V3<double> b, V3<double> c, double d,
+                                                Emit&& emit)
 {
     SampleGen g;
-    g.init(vs, d);
+    g.init(a, b, c, d);
     V3<double> p;
     while (g.next(p))
         if (emit(p)) return true;

@@ -18,7 +18,8 @@ struct twg_ctx {
     // growable device scratch owned by the context
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
-    // further growable slots (query ordering keys, values, sort temp storage)
+    // further growable slots: 0 the queue of tets the orientation filter left
+    // undecided, 1 query ordering keys and values, 2 sort temporary storage
     void* slot[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t slot_bytes[4] = {0, 0, 0, 0};
 };

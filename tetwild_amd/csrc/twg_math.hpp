@@ -357,8 +357,12 @@ __host__ __device__ __attribute__((noinline)) inline int orient3d_exact(const do
     return exp_sign(nacc, acc);
 }
 
-// CGAL::orientation(p, q, r, s) on doubles: +1 POSITIVE, 0, -1
-TWG_HD int orient3d(const double* p, const double* q, const double* r, const double* s)
+// The floating-point filter of orient3d alone: +1 / -1 when it decides the
+// sign of det[q-p; r-p; s-p], kUndecided otherwise.  Kernels that call this
+// instead of orient3d keep the exact predicate (and its registers and stack)
+// out of their hot loop and finish the undecided cases in a separate pass.
+constexpr int kUndecided = 2;
+TWG_HD int orient3d_filter(const double* p, const double* q, const double* r, const double* s)
 {
     double ax = q[0] - p[0], ay = q[1] - p[1], az = q[2] - p[2];
     double bx = r[0] - p[0], by = r[1] - p[1], bz = r[2] - p[2];
@@ -373,7 +377,14 @@ TWG_HD int orient3d(const double* p, const double* q, const double* r, const dou
     double bound = 0x1p-50 * perm;  // > (7 + 56 u) u perm, u = 2^-53
     if (det > bound) return 1;
     if (-det > bound) return -1;
-    return orient3d_exact(p, q, r, s);
+    return kUndecided;
+}
+
+// CGAL::orientation(p, q, r, s) on doubles: +1 POSITIVE, 0, -1
+TWG_HD int orient3d(const double* p, const double* q, const double* r, const double* s)
+{
+    int f = orient3d_filter(p, q, r, s);
+    return f != kUndecided ? f : orient3d_exact(p, q, r, s);
 }
 
 __host__ __device__ __attribute__((noinline)) inline int minor_zero_exact(const double* p,

@@ -146,6 +146,44 @@ class Context:
                                     _ptr(out)), "twg_tet_flip")
         return out
 
+    def tet_groups(self, tets, group_off, x, y, z, want_q=True):
+        """isFlip / calTetQualities / getCheckQuality per group of tets:
+        returns (flipped[g] uint8, max_q[g], q[t] or None) as device tensors"""
+        n_tets = self._tets(tets)
+        nv = self._verts(x, y, z)
+        group_off = self._dev(group_off, torch.int32, "group_off")
+        ng = group_off.shape[0] - 1
+        fl = torch.empty(max(ng, 1), dtype=torch.uint8, device=self.device)
+        mq = torch.empty(max(ng, 1), dtype=torch.float64, device=self.device)
+        q = torch.empty(max(n_tets, 1), dtype=torch.float64, device=self.device) if want_q else None
+        self._bind()
+        C.check(self.L.twg_tet_groups(self.h, _ptr(tets), _ptr(group_off), ng, n_tets, _ptr(x),
+                                      _ptr(y), _ptr(z), nv, _ptr(fl), _ptr(mq),
+                                      _ptr(q) if q is not None else None), "twg_tet_groups")
+        return fl[:ng], mq[:ng], (q[:n_tets] if q is not None else None)
+
+    def energy_stats(self, slim_energy, removed=None, tets=None, locked=None, thres=10.0):
+        """(sum, max, second_max, count, buckets[11]) over device arrays"""
+        self._dev(slim_energy, torch.float64, "slim_energy", (None,))
+        n = slim_energy.shape[0]
+        if removed is not None:
+            self._dev(removed, torch.uint8, "removed", (n,))
+        if (tets is None) != (locked is None):
+            raise ValueError("tets and locked go together")
+        if tets is not None:
+            if self._tets(tets) != n:
+                raise ValueError("tets must have one row per energy")
+            self._dev(locked, torch.uint8, "locked", (None,))
+        nv = locked.shape[0] if locked is not None else 0
+        out = C.EnergySummary()
+        self._bind()
+        C.check(self.L.twg_energy_stats(self.h, _ptr(slim_energy), n,
+                                        _ptr(removed) if removed is not None else None,
+                                        _ptr(tets) if tets is not None else None,
+                                        _ptr(locked) if locked is not None else None, nv,
+                                        float(thres), ctypes.byref(out)), "twg_energy_stats")
+        return out.sum, out.max, out.second_max, out.count, list(out.buckets)
+
     # -- (b) -----------------------------------------------------------
     def surface(self, V, F):
         """Device AABB tree over a triangle surface (numpy V (nv,3) float64,
@@ -354,6 +392,116 @@ class SmoothPlan:
     def close(self):
         if getattr(self, "h", None) is not None:
             self.batch.ctx.L.twg_smooth_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CollapseBatch:
+    """Device-resident batch of edge-collapse candidates (see twg_collapse_desc).
+    arrays: dict of numpy arrays as produced by workloads.collapse_batch (x, y,
+    z, v1, v2, flags, ring_off, ring_tets, old_q, tri_off, tri_ids); sf / bt:
+    Surface objects (or None)."""
+
+    def __init__(self, ctx, arrays, sf=None, bt=None, eps2=0.0, sampling_dist=1.0,
+                 check_quality=True, check_eps=True, soft_energy=None):
+        import numpy as np
+        self.ctx = ctx
+        dev = ctx.device
+
+        def T(a, dt):
+            return torch.from_numpy(np.ascontiguousarray(a).astype(dt, copy=False)).to(dev)
+        self.x = T(arrays["x"], np.float64)
+        self.y = T(arrays["y"], np.float64)
+        self.z = T(arrays["z"], np.float64)
+        self.v1 = T(arrays["v1"], np.int32)
+        self.v2 = T(arrays["v2"], np.int32)
+        self.flags = T(arrays["flags"], np.uint8)
+        self.ring_off = T(arrays["ring_off"], np.int32)
+        rt = arrays["ring_tets"]
+        self.ring_tets = T(rt if len(rt) else np.zeros((1, 4)), np.int32)
+        self.old_q = T(arrays["old_q"] if len(rt) else np.zeros(1), np.float64)
+        self.tri_off = T(arrays["tri_off"], np.int32)
+        tri = arrays["tri_ids"]
+        self.tri_ids = T(tri if len(tri) else np.zeros((1, 3)), np.int32)
+        n = len(arrays["v1"])
+        self.n, self.n_ring, self.n_tri = n, len(rt), len(tri)
+        self.status = torch.empty(max(n, 1), dtype=torch.uint8, device=dev)
+        self.new_q = torch.zeros(max(len(rt), 1), dtype=torch.float64, device=dev)
+        d = C.CollapseDesc()
+        d.x, d.y, d.z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        d.nv = self.x.shape[0]
+        d.n = n
+        d.v1, d.v2, d.flags = self.v1.data_ptr(), self.v2.data_ptr(), self.flags.data_ptr()
+        d.ring_off = self.ring_off.data_ptr()
+        d.ring_tets = self.ring_tets.data_ptr()
+        d.n_ring = self.n_ring
+        d.old_q = self.old_q.data_ptr()
+        d.tri_off = self.tri_off.data_ptr()
+        d.tri_ids = self.tri_ids.data_ptr()
+        d.n_tri = self.n_tri
+        d.sf_tree = sf.h.value if sf is not None else None
+        d.b_tree = bt.h.value if bt is not None else None
+        d.eps2 = float(eps2)
+        d.sampling_dist = float(sampling_dist)
+        d.check_quality = int(bool(check_quality))
+        d.check_eps = int(bool(check_eps))
+        d.is_soft = int(soft_energy is not None)
+        d.soft_energy = float(soft_energy) if soft_energy is not None else 0.0
+        d.status = self.status.data_ptr()
+        d.new_q = self.new_q.data_ptr()
+        self.desc = d
+        self._keep = (sf, bt)
+
+    def run(self, precision=C.TWG_FP64, tile=0, chunk=0):
+        self.ctx._bind()
+        C.check(self.ctx.L.twg_collapse_batch(self.ctx.h, ctypes.byref(self.desc), precision,
+                                              tile, chunk), "twg_collapse_batch")
+        return self.status[:self.n], self.new_q[:self.n_ring]
+
+    def autotune(self, precision=C.TWG_FP64):
+        t = ctypes.c_int()
+        c = ctypes.c_int()
+        ms = ctypes.c_float()
+        self.ctx._bind()
+        C.check(self.ctx.L.twg_collapse_autotune(self.ctx.h, ctypes.byref(self.desc), precision,
+                                                 ctypes.byref(t), ctypes.byref(c),
+                                                 ctypes.byref(ms)), "twg_collapse_autotune")
+        return t.value, c.value, ms.value
+
+    def plan(self, precision=C.TWG_FP64, tile=0, chunk=0):
+        return CollapsePlan(self, precision, tile, chunk)
+
+
+class CollapsePlan:
+    """hipGraph of every launch of a CollapseBatch."""
+
+    def __init__(self, batch, precision, tile, chunk):
+        self.batch = batch
+        L = batch.ctx.L
+        h = ctypes.c_void_p()
+        batch.ctx._bind()
+        C.check(L.twg_collapse_plan_create(batch.ctx.h, ctypes.byref(batch.desc), precision, tile,
+                                           chunk, ctypes.byref(h)), "twg_collapse_plan_create")
+        self.h = h
+        t = ctypes.c_int()
+        c = ctypes.c_int()
+        C.check(L.twg_collapse_plan_info(h, ctypes.byref(t), ctypes.byref(c)),
+                "twg_collapse_plan_info")
+        self.tile, self.chunk = t.value, c.value
+
+    def run(self):
+        self.batch.ctx._bind()
+        C.check(self.batch.ctx.L.twg_collapse_plan_run(self.h), "twg_collapse_plan_run")
+        return self.batch.status[:self.batch.n], self.batch.new_q[:self.batch.n_ring]
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.batch.ctx.L.twg_collapse_plan_destroy(self.h)
             self.h = None
 
     def __del__(self):

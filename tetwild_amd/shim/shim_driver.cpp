@@ -3,8 +3,17 @@
 // reference's class surface can be exercised end to end.
 //
 //   shim_driver <mesh.bin> <out.bin> <mode>
-//   mode: seq   smooth() in vertex order (the reference's Gauss-Seidel order)
-//         batch smoothSingleBatched() + smoothSurfaceBatched()
+//   mode: seq       smooth() in vertex order (the reference's Gauss-Seidel order)
+//         batch     smoothSingleBatched() + smoothSurfaceBatched()
+//         collapse  EdgeCollapser::checkCollapses over every directed edge
+//                   (v1, v2 ascending); collapse_soft: is_soft, soft_energy 40
+//         remove    EdgeRemover::checkRemovals over every interior edge (closed
+//                   ring) of 3..5 tets (edges ascending, rings = ascending tet ids);
+//                   out.bin: per edge int32 kind, int32 k, k x (int32[4] tet,
+//                   double quality); then int32 equal_buget
+//         energy    getAvgMaxEnergy / getMaxEnergy / getSecondMaxEnergy /
+//                   getFilterEnergy with tets t % 7 == 3 marked removed; prints
+//                   them as JSON (out.bin unused)
 //
 // mesh.bin (little-endian): int64 nv, nt, nsf, nbf, is_mesh_closed;
 // double eps, eps_2, sampling_dist; double posf[nv][3]; uint8 flags[nv]
@@ -15,8 +24,12 @@
 // suc_counter; plus, for the first up-to-8 interior vertices, NewtonsUpdate
 // (E, J[3], H[9]) and getNewEnergy at the final positions, and the static
 // comformalAMIPSEnergy_new of tet 0.
+// collapse out.bin: int32 codes[n]; int32 counts[n] (new qualities per
+// candidate); double new_q[sum counts]; uint8 flags[nv] after the checks.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -33,7 +46,7 @@ static bool rd(FILE* f, T* p, size_t n)
 int main(int argc, char** argv)
 {
     if (argc != 4) {
-        fprintf(stderr, "usage: shim_driver mesh.bin out.bin seq|batch\n");
+        fprintf(stderr, "usage: shim_driver mesh.bin out.bin seq|batch|collapse|collapse_soft\n");
         return 2;
     }
     FILE* f = fopen(argv[1], "rb");
@@ -85,8 +98,104 @@ int main(int argc, char** argv)
         st.eps_2 = eps[1];
         st.sampling_dist = eps[2];
         st.is_mesh_closed = hdr[4] != 0;
-        VertexSmoother vs(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
         std::string mode = argv[3];
+        if (mode == "energy") {
+            for (int64_t t = 0; t < nt; ++t) t_rm[t] = t % 7 == 3;
+            LocalOperations lo(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+            double avg, mx;
+            lo.getAvgMaxEnergy(avg, mx);
+            bool clean = false;
+            double fe = lo.getFilterEnergy(clean);
+            printf("{\"avg\": %.17g, \"max\": %.17g, \"max2\": %.17g, \"second\": %.17g, "
+                   "\"filter\": %.17g, \"clean\": %d}\n",
+                   avg, mx, lo.getMaxEnergy(), lo.getSecondMaxEnergy(mx), fe, clean ? 1 : 0);
+            return 0;
+        }
+        if (mode == "remove") {
+            EdgeRemover er(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+            std::vector<std::array<int, 2>> edges;
+            for (const auto& t : tets)
+                for (int a = 0; a < 4; ++a)
+                    for (int b = a + 1; b < 4; ++b)
+                        edges.push_back({{std::min(t[a], t[b]), std::max(t[a], t[b])}});
+            std::sort(edges.begin(), edges.end());
+            edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+            std::vector<std::array<int, 2>> cand;
+            std::vector<std::vector<int>> rings;
+            for (const auto& e : edges) {
+                std::vector<int> r;
+                for (int t : tv[e[0]].conn_tets)
+                    if (tv[e[1]].conn_tets.count(t)) r.push_back(t);
+                std::sort(r.begin(), r.end());
+                if (r.size() < 3 || r.size() > 5) continue;
+                // closed rings only (interior edges: every other vertex of the
+                // ring appears in exactly two of its tets)
+                std::map<int, int> cnt;
+                for (int t : r)
+                    for (int v : tets[t])
+                        if (v != e[0] && v != e[1]) cnt[v]++;
+                bool closed = true;
+                for (const auto& kv : cnt) closed = closed && kv.second == 2;
+                if (!closed) continue;
+                cand.push_back(e);
+                rings.push_back(r);
+            }
+            std::vector<EdgeRemover::Removal> res;
+            er.checkRemovals(cand, rings, res);
+            FILE* o = fopen(argv[2], "wb");
+            if (!o) return 2;
+            int n_ok = 0;
+            for (const auto& r : res) {
+                int32_t hdr2[2] = {r.kind, (int32_t)r.new_tets.size()};
+                fwrite(hdr2, 4, 2, o);
+                for (size_t k = 0; k < r.new_tets.size(); ++k) {
+                    fwrite(r.new_tets[k].data(), 4, 4, o);
+                    fwrite(&r.new_qs[k].slim_energy, 8, 1, o);
+                }
+                n_ok += r.kind != 0;
+            }
+            int32_t eb = er.equal_buget;
+            fwrite(&eb, 4, 1, o);
+            fclose(o);
+            printf("{\"edges\": %zu, \"removable\": %d}\n", cand.size(), n_ok);
+            return 0;
+        }
+        if (mode == "collapse" || mode == "collapse_soft") {
+            EdgeCollapser ec(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+            if (mode == "collapse_soft") {
+                ec.is_soft = true;
+                ec.soft_energy = 40;
+            }
+            std::vector<std::array<int, 2>> edges;
+            for (const auto& t : tets)
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b)
+                        if (a != b) edges.push_back({{t[a], t[b]}});
+            std::sort(edges.begin(), edges.end());
+            edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+            std::vector<int> codes;
+            std::vector<std::vector<TetQuality>> qs;
+            ec.checkCollapses(edges, codes, &qs);
+            FILE* o = fopen(argv[2], "wb");
+            if (!o) return 2;
+            std::vector<int32_t> c32(codes.begin(), codes.end()), cnt;
+            for (const auto& q : qs) cnt.push_back((int32_t)q.size());
+            fwrite(c32.data(), 4, c32.size(), o);
+            fwrite(cnt.data(), 4, cnt.size(), o);
+            for (const auto& q : qs)
+                for (const auto& t : q) fwrite(&t.slim_energy, 8, 1, o);
+            for (int64_t v = 0; v < nv; ++v) {
+                uint8_t fl = (tv[v].is_on_surface ? 1 : 0) | (tv[v].is_on_bbox ? 2 : 0) |
+                             (tv[v].is_on_boundary ? 4 : 0) | (tv[v].is_locked ? 8 : 0);
+                fwrite(&fl, 1, 1, o);
+            }
+            fclose(o);
+            int ok = 0;
+            for (int c : codes) ok += c == EdgeCollapser::SUCCESS || c == EdgeCollapser::ENVELOP_SUC;
+            printf("{\"candidates\": %zu, \"collapsible\": %d}\n", edges.size(), ok);
+            return 0;
+        }
+        VertexSmoother vs(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
         int64_t counters[2];
         if (mode == "seq") {
             vs.smooth();

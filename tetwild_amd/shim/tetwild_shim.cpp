@@ -187,30 +187,69 @@ void LocalOperations::getCheckQuality(const std::vector<int>& t_ids, TetQuality&
     getCheckQuality(qs, tq);
 }
 
-// src/tetwild/LocalOperations.cpp:605-637 (no UI locks in this model)
+// One device pass for the energy reductions (src/tetwild/LocalOperations.cpp:
+// 605-693): slim energies, removal flags, tets and vertex locks uploaded, the
+// sums, maxima and the getFilterEnergy histogram computed by twg_energy_stats.
+twg_energy_summary LocalOperations::energySummary()
+{
+    const size_t n = tet_qualities.size(), nv = tet_vertices.size();
+    std::vector<double> e(n);
+    std::vector<uint8_t> rm(n), lk(nv);
+    std::vector<int32_t> ts(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+        e[i] = tet_qualities[i].slim_energy;
+        rm[i] = i < t_is_removed.size() && t_is_removed[i] ? 1 : 0;
+        for (int j = 0; j < 4; ++j) ts[4 * i + j] = tets[i][j];
+    }
+    for (size_t v = 0; v < nv; ++v) lk[v] = tet_vertices[v].is_locked ? 1 : 0;
+    double* de = (double*)dev.buffer(0, 8 * n + 8);
+    uint8_t* drm = (uint8_t*)dev.buffer(1, n + nv + 16);
+    uint8_t* dlk = drm + n;
+    int32_t* dts = (int32_t*)dev.buffer(2, 16 * n + 16);
+    dev.h2d(de, e.data(), 8 * n);
+    dev.h2d(drm, rm.data(), n);
+    dev.h2d(dlk, lk.data(), nv);
+    dev.h2d(dts, ts.data(), 16 * n);
+    twg_energy_summary out;
+    check(twg_energy_stats(dev.ctx(), de, (int64_t)n, drm, dts, dlk, (int64_t)nv,
+                           state.filter_energy_thres, &out),
+          "twg_energy_stats");
+    return out;
+}
+
+// src/tetwild/LocalOperations.cpp:605-622
 void LocalOperations::getAvgMaxEnergy(double& avg_tq, double& max_tq)
 {
-    avg_tq = 0;
-    max_tq = 0;
-    int cnt = 0;
-    for (size_t i = 0; i < tet_qualities.size(); ++i) {
-        if (t_is_removed[i]) continue;
-        if (tet_qualities[i].slim_energy > max_tq) max_tq = tet_qualities[i].slim_energy;
-        avg_tq += tet_qualities[i].slim_energy;
-        cnt++;
-    }
-    avg_tq /= cnt;
+    twg_energy_summary s = energySummary();
+    max_tq = s.max;
+    avg_tq = s.sum / (double)s.count;
     if (std::isinf(avg_tq)) avg_tq = State::MAX_ENERGY;
 }
 
-double LocalOperations::getMaxEnergy()
+// src/tetwild/LocalOperations.cpp:624-635
+double LocalOperations::getMaxEnergy() { return energySummary().max; }
+
+// src/tetwild/LocalOperations.cpp:637-650 (the argument is unused there too)
+double LocalOperations::getSecondMaxEnergy(double) { return energySummary().second_max; }
+
+// src/tetwild/LocalOperations.cpp:652-693: the decision from the histogram
+double LocalOperations::getFilterEnergy(bool& is_clean_up)
 {
-    double max_tq = 0;
-    for (size_t i = 0; i < tet_qualities.size(); ++i) {
-        if (t_is_removed[i]) continue;
-        if (tet_qualities[i].slim_energy > max_tq) max_tq = tet_qualities[i].slim_energy;
+    twg_energy_summary s = energySummary();
+    int64_t below[10], above[10];
+    for (int i = 0; i < 10; ++i) {
+        below[i] = above[i] = 0;
+        for (int k = 0; k < 11; ++k) (k <= i ? below[i] : above[i]) += s.buckets[k];
     }
-    return max_tq;
+    if (below[0] >= above[0]) {
+        is_clean_up = above[5] > 0;
+        return 8;
+    }
+    if (below[8] <= above[8]) return 1e11;
+    for (int i = 0; i < 8; ++i)
+        if (below[i] < above[i] && below[i + 1] > above[i + 1])
+            return state.filter_energy_thres - 1 + 5 * std::pow(10, i + 1);
+    return 8;
 }
 
 void LocalOperations::areFacesOutEnvelop(const std::vector<Triangle_3f>& tris,
@@ -723,6 +762,414 @@ void VertexSmoother::smoothSurfaceBatched()
         counter += (int)cls.size();
         for (uint8_t s : st)
             if (s == TWG_SMOOTH_MOVED) suc_counter++;
+    }
+}
+
+// ---- EdgeCollapser ---------------------------------------------------------
+void EdgeCollapser::oneRing(int v_id, std::vector<int>& t_ids) const
+{
+    t_ids.assign(tet_vertices[v_id].conn_tets.begin(), tet_vertices[v_id].conn_tets.end());
+    std::sort(t_ids.begin(), t_ids.end());
+}
+
+int EdgeCollapser::checkCollapse(int v1_id, int v2_id, std::vector<TetQuality>* new_qs)
+{
+    std::vector<int> codes;
+    std::vector<std::vector<TetQuality>> qs;
+    checkCollapses({{{v1_id, v2_id}}}, codes, new_qs ? &qs : nullptr);
+    if (new_qs) *new_qs = qs[0];
+    return codes[0];
+}
+
+void EdgeCollapser::checkCollapses(const std::vector<std::array<int, 2>>& edges,
+                                   std::vector<int>& codes,
+                                   std::vector<std::vector<TetQuality>>* new_qs)
+{
+    const size_t n = edges.size();
+    codes.assign(n, SUCCESS);
+    if (new_qs) new_qs->assign(n, {});
+    if (n == 0) return;
+    // the flag updates at the top of collapseAnEdge, in candidate order
+    for (const auto& e : edges) {
+        TetVertex& v1 = tet_vertices[e[0]];
+        if (v1.is_on_surface && isIsolated(e[0])) {
+            v1.is_on_surface = false;
+            v1.is_on_boundary = false;
+        }
+        if (!isBoundaryPoint(e[0])) v1.is_on_boundary = false;
+    }
+    if (stamp_.size() < tet_vertices.size()) stamp_.assign(tet_vertices.size(), -1);
+    std::vector<int> global;
+    auto local = [&](int v) {
+        if (stamp_[v] < 0) {
+            stamp_[v] = (int)global.size();
+            global.push_back(v);
+        }
+        return stamp_[v];
+    };
+    std::vector<int32_t> v1s(n), v2s(n), ring_off(n + 1), tri_off(n + 1), ring_tets, tri_ids;
+    std::vector<uint8_t> flags(n);
+    std::vector<double> old_q;
+    std::vector<std::vector<int>> rings(n);
+    ring_off[0] = tri_off[0] = 0;
+    for (size_t c = 0; c < n; ++c) {
+        const int v1 = edges[c][0], v2 = edges[c][1];
+        v1s[c] = local(v1);
+        v2s[c] = local(v2);
+        const TetVertex &a = tet_vertices[v1], &b = tet_vertices[v2];
+        flags[c] = (uint8_t)((a.is_on_surface ? TWG_CV1_SURFACE : 0) |
+                             (a.is_on_boundary ? TWG_CV1_BOUNDARY : 0) |
+                             (b.is_on_surface ? TWG_CV2_SURFACE : 0) |
+                             (b.is_on_boundary ? TWG_CV2_BOUNDARY : 0) |
+                             (a.is_rounded ? 0 : TWG_CV1_UNROUNDED));
+        oneRing(v1, rings[c]);
+        for (int t : rings[c]) {
+            for (int j = 0; j < 4; ++j) ring_tets.push_back(local(tets[t][j]));
+            old_q.push_back(tet_qualities[t].slim_energy);
+        }
+        ring_off[c + 1] = (int32_t)(ring_tets.size() / 4);
+        // isCollapsable_epsilon's triangles, src/tetwild/EdgeCollapser.cpp:742-764
+        std::vector<std::array<int, 3>> tri;
+        for (int t : rings[c])
+            for (int j = 0; j < 4; ++j)
+                if (tets[t][j] != v1 && is_surface_fs[t][j] != State::NOT_SURFACE) {
+                    std::array<int, 3> f = {{tets[t][(j + 1) % 4], tets[t][(j + 2) % 4],
+                                             tets[t][(j + 3) % 4]}};
+                    std::sort(f.begin(), f.end());
+                    tri.push_back(f);
+                }
+        std::sort(tri.begin(), tri.end());
+        tri.erase(std::unique(tri.begin(), tri.end()), tri.end());
+        for (auto f : tri) {
+            if (std::find(f.begin(), f.end(), v2) != f.end()) continue;
+            *std::find(f.begin(), f.end(), v1) = v2;
+            for (int k = 0; k < 3; ++k) tri_ids.push_back(local(f[k]));
+        }
+        tri_off[c + 1] = (int32_t)(tri_ids.size() / 3);
+    }
+    const size_t nv = global.size(), nr = ring_tets.size() / 4, nt = tri_ids.size() / 3;
+    std::vector<double> x(nv), y(nv), z(nv);
+    for (size_t k = 0; k < nv; ++k) {
+        const Point_3f& p = tet_vertices[global[k]].posf;
+        x[k] = p[0];
+        y[k] = p[1];
+        z[k] = p[2];
+        stamp_[global[k]] = -1;
+    }
+    char* b0 = (char*)dev.buffer(0, 24 * nv);
+    double *dx = (double*)b0, *dy = dx + nv, *dz = dy + nv;
+    int32_t* dring = (int32_t*)dev.buffer(1, 16 * nr + 16);
+    char* b2 = (char*)dev.buffer(2, 4 * (4 * n + 2) + 12 * nt + 64);
+    int32_t* dv1 = (int32_t*)b2;
+    int32_t* dv2 = dv1 + n;
+    int32_t* droff = dv2 + n;
+    int32_t* dtoff = droff + (n + 1);
+    int32_t* dtri = dtoff + (n + 1);
+    uint8_t* dflags = (uint8_t*)dev.buffer(3, n + n);
+    uint8_t* dstatus = dflags + n;
+    double* dq = (double*)dev.buffer(4, 16 * nr + 16);
+    double* dnewq = dq + nr;
+    dev.h2d(dx, x.data(), 8 * nv);
+    dev.h2d(dy, y.data(), 8 * nv);
+    dev.h2d(dz, z.data(), 8 * nv);
+    dev.h2d(dring, ring_tets.data(), 16 * nr);
+    dev.h2d(dv1, v1s.data(), 4 * n);
+    dev.h2d(dv2, v2s.data(), 4 * n);
+    dev.h2d(droff, ring_off.data(), 4 * (n + 1));
+    dev.h2d(dtoff, tri_off.data(), 4 * (n + 1));
+    dev.h2d(dtri, tri_ids.data(), 12 * nt);
+    dev.h2d(dflags, flags.data(), n);
+    dev.h2d(dq, old_q.data(), 8 * nr);
+    twg_collapse_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.x = dx;
+    d.y = dy;
+    d.z = dz;
+    d.nv = (int64_t)nv;
+    d.n = (int64_t)n;
+    d.v1 = dv1;
+    d.v2 = dv2;
+    d.flags = dflags;
+    d.ring_off = droff;
+    d.ring_tets = dring;
+    d.n_ring = (int64_t)nr;
+    d.old_q = dq;
+    d.tri_off = dtoff;
+    d.tri_ids = dtri;
+    d.n_tri = (int64_t)nt;
+    d.sf_tree = geo_sf_tree.bvh();
+    d.b_tree = geo_b_tree.bvh();
+    d.eps2 = state.eps_2;
+    d.sampling_dist = state.sampling_dist > 0 ? state.sampling_dist : 1.0;
+    d.check_quality = energy_type != State::ENERGY_NA && is_check_quality;
+    d.check_eps = state.eps != State::EPSILON_NA && state.eps != State::EPSILON_INFINITE;
+    d.is_soft = is_soft;
+    d.soft_energy = soft_energy;
+    d.status = dstatus;
+    d.new_q = dnewq;
+    check(twg_collapse_batch(dev.ctx(), &d, precision, 0, 0), "twg_collapse_batch");
+    std::vector<uint8_t> st(n);
+    std::vector<double> newq(nr);
+    dev.d2h(st.data(), dstatus, n);
+    dev.d2h(newq.data(), dnewq, 8 * nr);
+    for (size_t c = 0; c < n; ++c) {
+        if (st[c] == TWG_COLLAPSE_BAD_INPUT)
+            throw std::runtime_error("twg_collapse_batch: bad input for edge " +
+                                     std::to_string(edges[c][0]) + "-" + std::to_string(edges[c][1]));
+        codes[c] = st[c];
+        if (!new_qs || (st[c] != SUCCESS && st[c] != ENVELOP_SUC)) continue;
+        for (size_t i = 0; i < rings[c].size(); ++i) {
+            const auto& t = tets[rings[c][i]];
+            if (std::find(t.begin(), t.end(), edges[c][1]) != t.end()) continue;  // removed
+            TetQuality q;
+            q.slim_energy = newq[ring_off[c] + i];
+            (*new_qs)[c].push_back(q);
+        }
+    }
+}
+
+// ---- EdgeRemover -----------------------------------------------------------
+namespace {
+
+void replace_id(std::array<int, 4>& t, int from, int to)
+{
+    for (int& v : t)
+        if (v == from) {
+            v = to;
+            return;
+        }
+}
+
+bool contains(const std::array<int, 4>& t, int v) { return std::find(t.begin(), t.end(), v) != t.end(); }
+
+// the two vertices of every ring tet other than v1, v2, chained around the
+// edge (n12_v_ids) and the ring positions in chain order (n12_t_ids),
+// src/tetwild/EdgeRemover.cpp:276-318
+void chain_ring(const std::vector<std::array<int, 4>>& ring, int v1, int v2, std::vector<int>& nv,
+                std::vector<int>& nt)
+{
+    const int m = (int)ring.size();
+    std::vector<std::array<int, 2>> es(m);
+    for (int i = 0; i < m; ++i) {
+        int c = 0;
+        for (int v : ring[i])
+            if (v != v1 && v != v2 && c < 2) es[i][c++] = v;
+    }
+    std::vector<bool> seen(m, false);
+    seen[0] = true;
+    nv = {es[0][0], es[0][1]};
+    nt = {0};
+    for (int i = 0; i < m - 2; ++i)
+        for (int j = 0; j < m; ++j) {
+            if (seen[j]) continue;
+            if (es[j][0] == nv.back()) {
+                seen[j] = true;
+                nv.push_back(es[j][1]);
+            } else if (es[j][1] == nv.back()) {
+                seen[j] = true;
+                nv.push_back(es[j][0]);
+            }
+            if (seen[j]) {
+                nt.push_back(j);
+                break;
+            }
+        }
+    for (int j = 0; j < m; ++j)
+        if (!seen[j]) {
+            nt.push_back(j);
+            break;
+        }
+}
+
+}  // namespace
+
+EdgeRemover::Removal EdgeRemover::checkRemoval(int v1_id, int v2_id, const std::vector<int>& old_t_ids)
+{
+    std::vector<Removal> out;
+    checkRemovals({{{v1_id, v2_id}}}, {old_t_ids}, out);
+    return out[0];
+}
+
+void EdgeRemover::checkRemovals(const std::vector<std::array<int, 2>>& edges,
+                                const std::vector<std::vector<int>>& rings,
+                                std::vector<Removal>& out)
+{
+    const size_t n = edges.size();
+    out.assign(n, Removal());
+    // configurations of every edge, one group each
+    std::vector<std::array<int, 4>> gt;   // group tets (global ids)
+    std::vector<int32_t> goff{0};
+    std::vector<int> first_group(n, -1);
+    std::vector<std::vector<int>> chain_v(n), chain_t(n);
+    auto close_group = [&]() { goff.push_back((int32_t)gt.size()); };
+    for (size_t e = 0; e < n; ++e) {
+        const int v1 = edges[e][0], v2 = edges[e][1];
+        const auto& ids = rings[e];
+        const int m = (int)ids.size();
+        if (m < 3 || m > 5) continue;
+        std::vector<std::array<int, 4>> ring(m);
+        for (int i = 0; i < m; ++i) ring[i] = tets[ids[i]];
+        first_group[e] = (int)goff.size() - 1;
+        if (m == 3) {  // src/tetwild/EdgeRemover.cpp:127-157
+            std::array<int, 2> vi{};
+            int c = 0;
+            for (int v : ring[0])
+                if (v != v1 && v != v2) vi[c++] = v;
+            int a = contains(ring[1], vi[0]) ? 1 : 2, b = 3 - a;
+            std::array<int, 4> t0 = ring[a], t1 = ring[b];
+            replace_id(t0, v1, vi[1]);
+            replace_id(t1, v2, vi[0]);
+            gt.push_back(t0);
+            gt.push_back(t1);
+            close_group();
+        } else if (m == 4) {  // :320-345, two rotations
+            chain_ring(ring, v1, v2, chain_v[e], chain_t[e]);
+            for (int i = 0; i < 2; ++i) {
+                const int a = chain_v[e][i], b = chain_v[e][2 + i];
+                for (int j = 0; j < 4; ++j) {
+                    std::array<int, 4> t = ring[j];
+                    if (contains(t, a)) replace_id(t, v2, b);
+                    else replace_id(t, v1, a);
+                    gt.push_back(t);
+                }
+                close_group();
+            }
+        } else {  // :458-575: pair k (around n12 vertex k), then middle pair k
+            chain_ring(ring, v1, v2, chain_v[e], chain_t[e]);
+            const auto& nv = chain_v[e];
+            const auto& nt = chain_t[e];
+            for (int k = 0; k < 5; ++k) {
+                std::array<int, 4> t0 = ring[nt[k]], t1 = ring[nt[k]];
+                replace_id(t0, v1, nv[(k + 4) % 5]);
+                replace_id(t1, v2, nv[(k + 4) % 5]);
+                gt.push_back(t0);
+                gt.push_back(t1);
+                close_group();
+            }
+            for (int k = 0; k < 5; ++k) {
+                std::array<int, 4> t0 = ring[nt[(k + 2) % 5]], t1 = ring[nt[(k + 2) % 5]];
+                replace_id(t0, v1, nv[k]);
+                replace_id(t1, v2, nv[k]);
+                gt.push_back(t0);
+                gt.push_back(t1);
+                close_group();
+            }
+        }
+    }
+    const size_t ng = goff.size() - 1, ntt = gt.size();
+    std::vector<uint8_t> gflip(ng);
+    std::vector<double> gmax(ng), gq(ntt);
+    if (ng > 0) {
+        // compact vertex set, upload, one launch
+        if (stamp_.size() < tet_vertices.size()) stamp_.assign(tet_vertices.size(), -1);
+        std::vector<int> global;
+        std::vector<int32_t> lt(4 * ntt);
+        for (size_t k = 0; k < ntt; ++k)
+            for (int j = 0; j < 4; ++j) {
+                int v = gt[k][j];
+                if (stamp_[v] < 0) {
+                    stamp_[v] = (int)global.size();
+                    global.push_back(v);
+                }
+                lt[4 * k + j] = stamp_[v];
+            }
+        const size_t nv = global.size();
+        std::vector<double> x(nv), y(nv), z(nv);
+        for (size_t k = 0; k < nv; ++k) {
+            const Point_3f& p = tet_vertices[global[k]].posf;
+            x[k] = p[0];
+            y[k] = p[1];
+            z[k] = p[2];
+            stamp_[global[k]] = -1;
+        }
+        double* dx = (double*)dev.buffer(0, 24 * nv);
+        double *dy = dx + nv, *dz = dy + nv;
+        int32_t* dt = (int32_t*)dev.buffer(1, 16 * ntt + 16);
+        int32_t* doff = (int32_t*)dev.buffer(2, 4 * (ng + 1));
+        uint8_t* dfl = (uint8_t*)dev.buffer(3, ng);
+        double* dmax = (double*)dev.buffer(4, 8 * (ng + ntt));
+        double* dq = dmax + ng;
+        dev.h2d(dx, x.data(), 8 * nv);
+        dev.h2d(dy, y.data(), 8 * nv);
+        dev.h2d(dz, z.data(), 8 * nv);
+        dev.h2d(dt, lt.data(), 16 * ntt);
+        dev.h2d(doff, goff.data(), 4 * (ng + 1));
+        check(twg_tet_groups(dev.ctx(), dt, doff, (int64_t)ng, (int64_t)ntt, dx, dy, dz,
+                             (int64_t)nv, dfl, dmax, dq),
+              "twg_tet_groups");
+        dev.d2h(gflip.data(), dfl, ng);
+        dev.d2h(gmax.data(), dmax, 8 * ng);
+        dev.d2h(gq.data(), dq, 8 * ntt);
+        for (size_t g = 0; g < ng; ++g)
+            if (gflip[g] == TWG_GROUP_BAD_INPUT) throw std::runtime_error("twg_tet_groups: bad input");
+    }
+    // the reference's decisions, edge by edge
+    auto better = [&](double nq, double oq) {
+        if (equal_buget > 0) {
+            equal_buget--;
+            return nq <= oq;
+        }
+        return nq < oq;
+    };
+    auto take = [&](Removal& r, int g) {
+        for (int k = goff[g]; k < goff[g + 1]; ++k) {
+            r.new_tets.push_back(gt[k]);
+            TetQuality q;
+            q.slim_energy = gq[k];
+            r.new_qs.push_back(q);
+        }
+    };
+    for (size_t e = 0; e < n; ++e) {
+        if (first_group[e] < 0) continue;
+        const auto& ids = rings[e];
+        const int m = (int)ids.size(), g0 = first_group[e];
+        double old_max = 0;
+        for (int t : ids)
+            if (tet_qualities[t].slim_energy > old_max) old_max = tet_qualities[t].slim_energy;
+        Removal& r = out[e];
+        if (m == 3) {
+            if (gflip[g0] || !better(gmax[g0], old_max)) continue;
+            r.kind = 3;
+            take(r, g0);
+        } else if (m == 4) {
+            int best = -1;
+            bool fail = false;
+            for (int i = 0; i < 2 && !fail; ++i) {
+                if (gflip[g0 + i]) continue;
+                if (!better(gmax[g0 + i], old_max)) {
+                    fail = true;
+                    break;
+                }
+                old_max = gmax[g0 + i];
+                best = i;
+            }
+            if (fail || best < 0) continue;
+            r.kind = 4;
+            take(r, g0 + best);
+        } else {
+            bool ok[5] = {true, true, true, true, true};
+            for (int k = 0; k < 5; ++k) {
+                if (!ok[(k + 1) % 5] && !ok[(k + 4) % 5]) continue;
+                if (gflip[g0 + k]) {
+                    ok[(k + 1) % 5] = false;
+                    ok[(k + 4) % 5] = false;
+                }
+            }
+            int sel = -1;
+            for (int i = 0; i < 5; ++i) {
+                if (!ok[i] || gflip[g0 + 5 + i]) continue;
+                const int l = (i + 1) % 5, rr = (i + 4) % 5;
+                double mx = std::max(gmax[g0 + 5 + i], std::max(gmax[g0 + l], gmax[g0 + rr]));
+                if (!better(mx, old_max)) continue;
+                old_max = mx;
+                sel = i;
+            }
+            if (sel < 0) continue;
+            r.kind = 5;
+            take(r, g0 + (sel + 1) % 5);
+            take(r, g0 + (sel + 4) % 5);
+            take(r, g0 + 5 + sel);
+        }
     }
 }
 

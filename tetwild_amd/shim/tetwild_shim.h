@@ -58,11 +58,16 @@ struct State {
     static constexpr double MAX_ENERGY = 1e50;
     static constexpr int NOT_SURFACE = INT_MAX;
     static constexpr int EPSILON_INFINITE = -2;
+    static constexpr int EPSILON_NA = -1;
+    static constexpr int ENERGY_NA = 0;
+    static constexpr int ENERGY_AMIPS = 2;
     double eps = 0;
     double eps_2 = 0;
     double sampling_dist = 0;
     bool is_mesh_closed = true;
     bool use_energy_max = true;
+    // Args::filter_energy_thres (include/tetwild/Args.h:63), read by getFilterEnergy
+    double filter_energy_thres = 10;
 };
 
 // GPU context shared by the objects of one mesh (one device, one stream).
@@ -143,6 +148,8 @@ public:
     void getCheckQuality(const std::vector<int>& t_ids, TetQuality& tq);
     void getAvgMaxEnergy(double& avg_tq, double& max_tq);
     double getMaxEnergy();
+    double getSecondMaxEnergy(double max_energy);
+    double getFilterEnergy(bool& is_clean_up);
     bool isFaceOutEnvelop(const Triangle_3f& tri);
     bool isFaceOutEnvelop_sampling(const Triangle_3f& tri) { return isFaceOutEnvelop(tri); }
     bool isPointOutEnvelop(const Point_3f& p);
@@ -159,6 +166,8 @@ public:
     static void comformalAMIPSHessian_new(const double* T, double* result_0);
 
 protected:
+    // sums, maxima and histogram of the stored slim energies (one device pass)
+    twg_energy_summary energySummary();
     // compact gather: tets -> local ids, x/y/z of the touched vertices
     struct Gathered {
         std::vector<int32_t> tets;
@@ -204,6 +213,67 @@ private:
     void runBatch(const std::vector<Problem>& probs, std::vector<uint8_t>& status);
     std::vector<std::vector<int>> colourClasses(const std::vector<int>& verts) const;
     std::vector<double> last_pout_;  // p_out of the last batch
+};
+
+// The checks of EdgeCollapser::collapseAnEdge (src/tetwild/EdgeCollapser.h:
+// 23-92, EdgeCollapser.cpp:284-446) up to its "real update": the caller (the
+// collapse queue and the topology update) stays on the host.
+class EdgeCollapser : public LocalOperations {
+public:
+    using LocalOperations::LocalOperations;
+
+    // return codes, src/tetwild/EdgeCollapser.h:52-56
+    static constexpr int SUCCESS = 0;
+    static constexpr int FLIP = 1;
+    static constexpr int QUALITY = 2;
+    static constexpr int ENVELOP = 3;
+    static constexpr int ENVELOP_SUC = 4;
+
+    int energy_type = State::ENERGY_AMIPS;
+    bool is_check_quality = true;
+    bool is_soft = false;
+    double soft_energy = 6;
+
+    // collapseAnEdge's checks for v1 -> v2, including its updates of v1's
+    // surface / boundary flags (:298-307).  new_qs receives the new tets'
+    // qualities in one-ring order (the removed tets skipped), which the caller
+    // stores when the code is SUCCESS or ENVELOP_SUC.
+    int checkCollapse(int v1_id, int v2_id, std::vector<TetQuality>* new_qs = nullptr);
+    // Many candidates in one launch, each checked against the mesh as it is
+    // now (no collapse is applied in between); the flag updates run in
+    // candidate order first, as the sequential checks would run them.
+    void checkCollapses(const std::vector<std::array<int, 2>>& edges, std::vector<int>& codes,
+                        std::vector<std::vector<TetQuality>>* new_qs = nullptr);
+
+    // v1's one-ring in the order the mirrors use (sorted tet ids)
+    void oneRing(int v_id, std::vector<int>& t_ids) const;
+};
+
+// The decision of EdgeRemover::swap for an edge (src/tetwild/EdgeRemover.h,
+// EdgeRemover.cpp:97-617): which of removeAnEdge_32 / _44 / _56 succeeds and
+// the tets and qualities it would write; the queue and the topology update
+// stay with the caller.
+class EdgeRemover : public LocalOperations {
+public:
+    using LocalOperations::LocalOperations;
+
+    int energy_type = State::ENERGY_AMIPS;
+    int equal_buget = 100;  // src/tetwild/EdgeRemover.cpp:60
+
+    struct Removal {
+        int kind = 0;  // 3, 4 or 5: the operator that succeeds; 0: none
+        std::vector<std::array<int, 4>> new_tets;
+        std::vector<TetQuality> new_qs;
+    };
+
+    // the edge (v1, v2) with its tets old_t_ids (the reference's order);
+    // equal_buget is consumed as the reference consumes it
+    Removal checkRemoval(int v1_id, int v2_id, const std::vector<int>& old_t_ids);
+    // many edges: every configuration evaluated in one launch, the decisions
+    // taken in edge order (equal_buget carried through), each against the mesh
+    // as it is now
+    void checkRemovals(const std::vector<std::array<int, 2>>& edges,
+                       const std::vector<std::vector<int>>& rings, std::vector<Removal>& out);
 };
 
 }  // namespace tetwild_amd

@@ -485,3 +485,134 @@ def smoothing_batch(n, seed=0, surface_frac=0.2, V=None, F=None, radius=0.01, bo
         "kind": kind, "tri_off": toff.astype(np.int32),
         "tri_ab": np.ascontiguousarray(tri_ab.astype(np.int32)),
     }
+
+
+# ------------------------------------------------------- collapse batches --
+NOT_SURFACE = 2**31 - 1  # State::NOT_SURFACE
+
+
+def sheet_mesh(n, jitter=0.15, surf_jitter=0.02, seed=0, level=None):
+    """Jittered Kuhn lattice (n cubes per axis) carrying an open surface sheet:
+    the horizontal grid plane z = level / n (default level n // 2), inside the
+    volume like TetWild's embedded input surface, so every sheet face has a tet
+    on each side and the sheet's rim is its boundary in the reference's sense
+    (LocalOperations::isEdgeOnBoundary, src/tetwild/LocalOperations.cpp:931-965).
+    Sheet vertices move in the plane by N(0, jitter h) and off it by
+    N(0, surf_jitter h); the rest by N(0, jitter h) per axis.  Tets are
+    re-oriented (2 <-> 3) where the jitter made them non-positive.
+
+    Returns a dict: x, y, z, tets (int32), is_surface_fs (int32 [nt, 4]: 0 for
+    the face opposite vertex j when it lies in the sheet, NOT_SURFACE
+    otherwise), on_surface / on_boundary (bool per vertex), and the envelope
+    inputs: sf_V, sf_F (the sheet triangles, z exactly on the plane) and b_V,
+    b_F (the rim segments as degenerate (a, b, b) triangles), the layout TetWild
+    builds its boundary tree in (src/tetwild/Preprocess.cpp:191-196)."""
+    n1 = n + 1
+    h = 1.0 / n
+    lv = n // 2 if level is None else level
+    x, y, z, tets, _ = kuhn_lattice(n, jitter=jitter, seed=seed)
+    vid = np.arange(n1 ** 3)
+    gi, gj, gk = vid % n1, (vid // n1) % n1, vid // (n1 * n1)
+    on = gk == lv
+    st = Stream(seed, 40)
+    z[on] = lv * h + st.normal(int(on.sum())) * (surf_jitter * h)
+    o = orientation(tets, x, y, z)
+    bad = o <= 0
+    tmp = tets[bad, 2].copy()
+    tets[bad, 2] = tets[bad, 3]
+    tets[bad, 3] = tmp
+    rim = on & ((gi == 0) | (gi == n) | (gj == 0) | (gj == n))
+    fs = np.full(tets.shape, NOT_SURFACE, dtype=np.int32)
+    for j in range(4):
+        k = [m for m in range(4) if m != j]
+        fs[on[tets[:, k[0]]] & on[tets[:, k[1]]] & on[tets[:, k[2]]], j] = 0
+    tt, jj = np.nonzero(fs != NOT_SURFACE)
+    faces = np.sort(np.stack([tets[tt, (jj + k) % 4] for k in (1, 2, 3)], 1), 1)
+    faces = np.unique(faces, axis=0).astype(np.int32)
+    V = np.stack([x, y, np.where(on, lv * h, z)], 1)
+    segs = []
+    for a in range(n):
+        for (i0, j0, i1, j1) in ((a, 0, a + 1, 0), (a, n, a + 1, n), (0, a, 0, a + 1), (n, a, n, a + 1)):
+            segs.append((i0 + n1 * (j0 + n1 * lv), i1 + n1 * (j1 + n1 * lv)))
+    segs = np.array(segs, dtype=np.int32)
+    b_F = np.stack([segs[:, 0], segs[:, 1], segs[:, 1]], 1).astype(np.int32)
+    return {"x": x, "y": y, "z": z, "tets": np.ascontiguousarray(tets), "is_surface_fs": fs,
+            "on_surface": on, "on_boundary": rim, "sf_V": V, "sf_F": faces,
+            "b_V": V, "b_F": b_F}
+
+
+def vertex_tets(tets, nv):
+    """conn_tets as CSR (off [nv + 1], tet ids ascending per vertex)"""
+    inc_v = tets.reshape(-1).astype(np.int64)
+    inc_t = np.repeat(np.arange(len(tets), dtype=np.int64), 4)
+    o = np.lexsort((inc_t, inc_v))
+    off = np.zeros(nv + 1, dtype=np.int64)
+    off[1:] = np.cumsum(np.bincount(inc_v, minlength=nv))
+    return off, inc_t[o]
+
+
+def _csr_gather(off, vals, keys):
+    """concatenation of vals[off[k]:off[k+1]] for k in keys, and its CSR offsets"""
+    cnt = off[keys + 1] - off[keys]
+    out_off = np.zeros(len(keys) + 1, dtype=np.int64)
+    out_off[1:] = np.cumsum(cnt)
+    idx = np.repeat(off[keys] - out_off[:-1], cnt) + np.arange(out_off[-1])
+    return vals[idx], out_off, cnt
+
+
+def collapse_batch(mesh, q, n_cand=None, seed=0, unrounded_frac=0.0):
+    """Edge-collapse candidates v1 -> v2 over the directed edges of `mesh`
+    (sheet_mesh layout), all of them or n_cand drawn without replacement.
+    q: stored slim energy per tet.  Per candidate: v1's one-ring (conn_tets in
+    ascending tet id), the ring's stored qualities, the vertex flags, and the
+    triangles isCollapsable_epsilon tests (src/tetwild/EdgeCollapser.cpp:742-764:
+    v1's one-ring surface faces as sorted vertex triples, in lexicographic order,
+    without those containing v2, v1 replaced by v2 in place).
+
+    Returns a dict of numpy arrays: x, y, z, v1, v2, flags (uint8, TWG_CV*),
+    ring_off, ring_tets, old_q, tri_off, tri_ids."""
+    tets = mesh["tets"]
+    nv = len(mesh["x"])
+    t64 = tets.astype(np.int64)
+    key = np.unique(np.concatenate([t64[:, a] * nv + t64[:, b]
+                                    for a in range(4) for b in range(4) if a != b]))
+    st = Stream(seed, 41)
+    if n_cand is not None and n_cand < len(key):
+        key = key[np.argsort(st.u64(len(key)), kind="stable")[:n_cand]]
+    v1, v2 = key // nv, key % nv
+    off, vt = vertex_tets(tets, nv)
+    ring_ids, ring_off, _ = _csr_gather(off, vt, v1)
+    # one-ring surface faces per vertex, sorted triples in lexicographic order
+    fs = mesh["is_surface_fs"]
+    tt, jj = np.nonzero(fs != NOT_SURFACE)
+    faces = np.sort(np.stack([tets[tt, (jj + k) % 4] for k in (1, 2, 3)], 1), 1).astype(np.int64)
+    assert nv < 2**21, "face keys are packed into int64"
+    fkey = np.unique((faces[:, 0] * nv + faces[:, 1]) * nv + faces[:, 2]) if len(faces) else \
+        np.zeros(0, np.int64)
+    faces = np.stack([fkey // (nv * nv), (fkey // nv) % nv, fkey % nv], 1)
+    fv = faces.reshape(-1)
+    fi = np.repeat(np.arange(len(faces)), 3)
+    o = np.lexsort((fi, fv))  # faces are already lexicographic, so ids ascending = lex order
+    foff = np.zeros(nv + 1, dtype=np.int64)
+    foff[1:] = np.cumsum(np.bincount(fv, minlength=nv))
+    cand_f, f_off, f_cnt = _csr_gather(foff, fi[o], v1)
+    owner = np.repeat(np.arange(len(v1)), f_cnt)
+    tri = faces[cand_f]
+    keep = ~(tri == v2[owner][:, None]).any(1)
+    tri, owner = tri[keep], owner[keep]
+    tri = np.where(tri == v1[owner][:, None], v2[owner][:, None], tri)
+    tri_off = np.zeros(len(v1) + 1, dtype=np.int64)
+    tri_off[1:] = np.cumsum(np.bincount(owner, minlength=len(v1)))
+    sfv, bdv = mesh["on_surface"], mesh["on_boundary"]
+    flags = (sfv[v1] * 1 + bdv[v1] * 2 + sfv[v2] * 4 + bdv[v2] * 8).astype(np.uint8)
+    if unrounded_frac > 0:
+        flags[st.uniform(len(v1)) < unrounded_frac] |= 16
+    return {
+        "x": mesh["x"], "y": mesh["y"], "z": mesh["z"],
+        "v1": v1.astype(np.int32), "v2": v2.astype(np.int32), "flags": flags,
+        "ring_off": ring_off.astype(np.int32),
+        "ring_tets": np.ascontiguousarray(tets[ring_ids].astype(np.int32)),
+        "old_q": np.ascontiguousarray(np.asarray(q, dtype=np.float64)[ring_ids]),
+        "tri_off": tri_off.astype(np.int32),
+        "tri_ids": np.ascontiguousarray(tri.astype(np.int32)).reshape(-1, 3),
+    }
