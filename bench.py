@@ -20,11 +20,14 @@ Secondary measurements in the same JSON line ("secondary"):
                  jittered 80^3 Kuhn lattice carrying an open surface sheet,
                  autotuned tiles, the batch replayed from a hipGraph, FP64 and
                  mixed
+  reduce         the energy reductions (twg_energy_stats) over 10.1M stored
+                 qualities and the configuration check (twg_tet_groups) over
+                 2.5M groups of four tets
 Each carries its CPU baseline (the CPU restatement, one host core, bounded
 sample) and its parity counts against that restatement.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
-                  [--only amips,envelope,slivers,smoothing,collapse]
+                  [--only amips,envelope,slivers,smoothing,collapse,reduce]
 
 Prints ONE JSON line.  The roofline object covers the headline kernel:
 algorithmic bytes per launch / average launch time from HIP events on the
@@ -446,15 +449,64 @@ def bench_collapse(ctx, args, result):
     result["secondary"]["collapse"] = sec
 
 
+def bench_reduce(ctx, args, result):
+    """twg_energy_stats over the 119^3 lattice's qualities (10.1M tets, 10%
+    removed, 1% of vertices locked) and twg_tet_groups over the same tets in
+    groups of four"""
+    import numpy as np
+    import torch
+    from tetwild_amd import workloads as W
+
+    x, y, z, tets, _ = W.kuhn_lattice(args.cubes, jitter=0.15, seed=0)
+    n, nv = len(tets), len(x)
+    tx, ty, tz, tt = (torch.from_numpy(a).to(ctx.device) for a in (x, y, z, tets))
+    q = ctx.tet_qualities(tt, tx, ty, tz)[0]
+    st = W.Stream(9, 0)
+    removed = torch.from_numpy((st.uniform(n) < 0.1).astype(np.uint8)).to(ctx.device)
+    locked = torch.from_numpy((st.uniform(nv) < 0.01).astype(np.uint8)).to(ctx.device)
+    sec = {"tets": n}
+    k = max(4, args.steps // 4)
+    for _ in range(2):
+        ctx.energy_stats(q, removed, tt, locked)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        s = ctx.energy_stats(q, removed, tt, locked)
+    el = (time.perf_counter() - t0) / k
+    sec["energy_stats_ms"] = el * 1e3
+    sec["energy_stats_gbs"] = (n * (8 + 1 + 16) + nv) / el / 1e9
+    sec["energy_stats"] = {"sum": s[0], "max": s[1], "second_max": s[2], "count": s[3],
+                           "buckets": s[4]}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        so = O.energy_stats(q.cpu().numpy(), removed.cpu().numpy(), tets, locked.cpu().numpy())
+        el = time.perf_counter() - t0
+        sec["cpu_baseline"] = {"value": n / el, "unit": "tets/s", "cores": 1, "kind": "port",
+                               "sample": f"all {n} tets, CPU restatement"}
+        sec["max_bucket_count_mismatch_vs_cpu"] = int(
+            (s[1] != so[1]) + (s[2] != so[2]) + (s[3] != so[3]) + int(list(s[4]) != list(so[4])))
+        sec["sum_rel_diff_vs_cpu"] = abs(s[0] - so[0]) / abs(so[0])
+    g = n // 4
+    off = torch.arange(0, 4 * g + 1, 4, dtype=torch.int32, device=ctx.device)
+    groups = tt[:4 * g].contiguous()
+    wall, kern = events_time(lambda: ctx.tet_groups(groups, off, tx, ty, tz), k, 2)
+    fl = ctx.tet_groups(groups, off, tx, ty, tz)[0]
+    torch.cuda.synchronize()
+    sec["groups"] = g
+    sec["groups_per_s"] = g / wall
+    sec["groups_flipped"] = int(fl.sum().item())
+    result["secondary"]["reduce"] = sec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller secondary workloads")
     ap.add_argument("--cubes", type=int, default=119)
-    ap.add_argument("--only", default="amips,envelope,slivers,smoothing,collapse")
+    ap.add_argument("--only", default="amips,envelope,slivers,smoothing,collapse,reduce")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -507,6 +559,9 @@ def main():
         surf_cache["surf"].close()
     if "collapse" in only:
         bench_collapse(ctx, args, result)
+        torch.cuda.empty_cache()
+    if "reduce" in only:
+        bench_reduce(ctx, args, result)
     ctx.close()
     print(json.dumps(result), flush=True)
 

@@ -118,6 +118,27 @@ def test_degenerate_surface_facets_and_tiny_surfaces(ctx, dev):
         assert ctx.envelope_points(surf, empty, 0.01).numel() == 0
 
 
+@pytest.mark.parametrize("nf", [255, 256, 257, 511, 512, 513, 1024, 1025])
+def test_tree_top_in_lds_boundaries(ctx, dev, bumpy, nf):
+    """surfaces whose trees end just below, at and above the levels the
+    kernels copy into LDS (512 FP64 / 1024 FP32 node slots)"""
+    V, F = bumpy
+    F = F[:nf]
+    eps = 1e-3
+    S = O.Surface(V, F)
+    surf = ctx.surface(V, F)
+    P = W.envelope_samples(V, F, 3000, eps=eps, band=0.01, seed=nf)
+    Q = W.query_triangles(V, F, 2000, eps=eps, edge=0.004, seed=nf)
+    ref_p = S.point_out(P, eps * eps)
+    ref_f, _ = S.face_out(Q, eps * eps, eps)
+    for precision in (0, 1):
+        out = ctx.envelope_points(surf, dev_t(dev, P), eps * eps, precision=precision)
+        assert np.array_equal(out.cpu().numpy(), ref_p), precision
+        out = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision)
+        assert np.array_equal(out.cpu().numpy(), ref_f), precision
+    assert 0.05 < ref_p.mean() < 0.95 and 0.05 < ref_f.mean() < 0.95
+
+
 def test_bad_surface_rejected(ctx):
     V = np.zeros((3, 3))
     F = np.array([[0, 1, 5]], dtype=np.int32)

@@ -273,11 +273,11 @@ __global__ __launch_bounds__(kBlock) void collapse_point_kernel(CollapseArgs A)
     bool out = false;
     if (w & kNeedBoundaryPoint) {
         int hint = -1;
-        out = !in_envelope<F32, false>(A.bt, p2, A.eps2, A.T_bt, A.sqrtT_bt, hint, st);
+        out = !in_envelope<F32, false>(walk(A.bt), p2, A.eps2, A.T_bt, A.sqrtT_bt, hint, st);
     }
     if (!out && (w & kNeedSurfacePoint)) {
         int hint = -1;
-        out = !in_envelope<F32, false>(A.sf, p2, A.eps2, A.T_sf, A.sqrtT_sf, hint, st);
+        out = !in_envelope<F32, false>(walk(A.sf), p2, A.eps2, A.T_sf, A.sqrtT_sf, hint, st);
     }
     if (out) {
         A.status[c] = TWG_COLLAPSE_ENVELOP;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void collapse_face_kernel(CollapseArgs A)
         return;
     }
     Stats st;
-    if (face_out_sampled<F32, false>(A.sf, t9, A.eps2, A.d, A.T_sf, A.sqrtT_sf, st))
+    if (face_out_sampled<F32, false>(walk(A.sf), t9, A.eps2, A.d, A.T_sf, A.sqrtT_sf, st))
         atomicOr(A.work + c, kFaceOut);
 }
 
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void collapse_face_exact_kernel(CollapseArg
             t9[3 * j + 2] = p.z;
         }
         Stats st;
-        if (face_out<F32, false>(A.sf, t9, A.eps2, A.d, A.T_sf, A.sqrtT_sf, st))
+        if (face_out<F32, false>(walk(A.sf), t9, A.eps2, A.d, A.T_sf, A.sqrtT_sf, st))
             atomicOr(A.work + c, kFaceOut);
     }
 }
@@ -381,7 +381,7 @@ int make_args(twg_ctx* ctx, const twg_collapse_desc* d, CollapseArgs& A)
     A.n_tri = d->tri_off ? d->n_tri : 0;
     A.has_sf = d->sf_tree != nullptr;
     A.has_bt = d->b_tree != nullptr;
-    BvhView none = {nullptr, nullptr, nullptr, 0};
+    BvhView none = no_bvh();
     A.sf = d->sf_tree ? bvh_view(d->sf_tree) : none;
     A.bt = d->b_tree ? bvh_view(d->b_tree) : none;
     A.eps2 = d->eps2;

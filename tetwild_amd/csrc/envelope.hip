@@ -32,6 +32,8 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
                                                         unsigned long long* stats,
                                                         const int32_t* __restrict__ order)
 {
+    __shared__ __align__(16) unsigned char cache[kTreeCacheBytes];
+    const BvhWalk W = cache_tree_top<F32>(B, cache);
     int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
@@ -39,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
         int hint = -1;
         V3<double> p = ld3(p3 + 3 * i);
         if (STATS) st.samples++;
-        out[i] = in_envelope<F32, STATS>(B, p, eps2, T, sqrtT, hint, st) ? 0 : 1;
+        out[i] = in_envelope<F32, STATS>(W, p, eps2, T, sqrtT, hint, st) ? 0 : 1;
     }
     if (STATS) flush_stats(st, stats);
 }
@@ -55,13 +57,15 @@ __global__ __launch_bounds__(kBlock) void faces_kernel(BvhView B, const double* 
                                                        unsigned* n_pending, int64_t* pending,
                                                        const int32_t* __restrict__ order)
 {
+    __shared__ __align__(16) unsigned char cache[kTreeCacheBytes];
+    const BvhWalk W = cache_tree_top<F32>(B, cache);
     int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {
         if (order) i = order[i];
         const double* t = tri9 + 9 * i;
         if (certainly_not_collinear(t, t + 3, t + 6)) {
-            out[i] = face_out_sampled<F32, STATS>(B, t, eps2, d, T, sqrtT, st) ? 1 : 0;
+            out[i] = face_out_sampled<F32, STATS>(W, t, eps2, d, T, sqrtT, st) ? 1 : 0;
         } else {
             unsigned k = atomicAdd(n_pending, 1u);
             pending[k] = i;
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void faces_exact_kernel(BvhView B,
     Stats st;
     for (unsigned k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
         int64_t i = pending[k];
-        out[i] = face_out<F32, false>(B, tri9 + 9 * i, eps2, d, T, sqrtT, st) ? 1 : 0;
+        out[i] = face_out<F32, false>(walk(B), tri9 + 9 * i, eps2, d, T, sqrtT, st) ? 1 : 0;
     }
 }
 

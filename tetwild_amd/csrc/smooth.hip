@@ -500,7 +500,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
                         V3<double> qa = vtx(A, A.tri_ab[2 * k]), qb = vtx(A, A.tri_ab[2 * k + 1]);
                         t9[3] = qa.x; t9[4] = qa.y; t9[5] = qa.z;
                         t9[6] = qb.x; t9[7] = qb.y; t9[8] = qb.z;
-                        out = face_out<false, false>(A.sf, t9, A.eps2, A.d, A.T, A.sqrtT, sts);
+                        out = face_out<false, false>(walk(A.sf), t9, A.eps2, A.d, A.T, A.sqrtT, sts);
                     }
                     if (tile_any(c, out)) {
                         st = TWG_SMOOTH_REJECT_ENVELOPE;
@@ -585,7 +585,7 @@ int make_args(twg_ctx* ctx, const twg_smooth_desc* d, unsigned* counter, unsigne
     A.n_tri = d->tri_ab ? d->n_tri : 0;
     A.has_sf = d->sf_tree != nullptr;
     A.has_bt = d->b_tree != nullptr;
-    BvhView none = {nullptr, nullptr, nullptr, 0};
+    BvhView none = no_bvh();
     A.sf = d->sf_tree ? bvh_view(d->sf_tree) : none;
     A.bt = d->b_tree ? bvh_view(d->b_tree) : none;
     A.eps2 = d->eps2;

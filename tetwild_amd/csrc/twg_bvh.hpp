@@ -51,16 +51,128 @@ struct twg_bvh {
 
 namespace twg {
 
+// node records in LDS (address space 3): typed so that the compiler emits LDS
+// reads for them (a generic pointer would turn every node read into a flat load)
+typedef __attribute__((address_space(3))) const BvhNode64 LdsNode64;
+typedef __attribute__((address_space(3))) const BvhNode32 LdsNode32;
+
+// field-wise copies out of LDS (a whole-struct assignment across address
+// spaces does not compile)
+__device__ __forceinline__ BvhNode32 from_lds(LdsNode32* p)
+{
+    BvhNode32 r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = p->lo[k];
+        r.hi[k] = p->hi[k];
+    }
+    r.leaf = p->leaf;
+    return r;
+}
+__device__ __forceinline__ BvhNode64 from_lds(LdsNode64* p)
+{
+    BvhNode64 r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = p->lo[k];
+        r.hi[k] = p->hi[k];
+    }
+    r.leaf = p->leaf;
+    return r;
+}
+
+// What kernels receive (by value, as a kernel argument or inside one): global
+// pointers and sizes only, so that the host and device layouts agree.
 struct BvhView {
     const BvhNode64* __restrict__ n64;
     const BvhNode32* __restrict__ n32;
     const double* __restrict__ tri;
     int64_t nf;
+    int64_t nnode;  // node slots (array length)
 };
+static_assert(sizeof(BvhView) == 40, "BvhView: kernel-argument layout");
+
+// What the traversals read: the view plus an optional copy of the top of the
+// tree in LDS (node slots [0, ncache), the levels every traversal walks
+// first; ncache = 0: none).  LDS pointers are 32-bit in device code and 64-bit
+// in the host pass, so a walker is made inside a kernel and never crosses into
+// a kernel argument.
+struct BvhWalk {
+    const BvhNode64* __restrict__ n64;
+    const BvhNode32* __restrict__ n32;
+    const double* __restrict__ tri;
+    int64_t nf;
+    int64_t nnode;
+    LdsNode64* c64;
+    LdsNode32* c32;
+    int64_t ncache;
+};
+
+// a walker that reads every node from global memory
+__device__ __forceinline__ BvhWalk walk(const BvhView& B)
+{
+    BvhWalk w;
+    w.n64 = B.n64;
+    w.n32 = B.n32;
+    w.tri = B.tri;
+    w.nf = B.nf;
+    w.nnode = B.nnode;
+    w.c64 = nullptr;
+    w.c32 = nullptr;
+    w.ncache = 0;
+    return w;
+}
+
+// bytes of the per-block LDS copy of the top of the tree: 1024 FP32 nodes
+// (10 levels) or 512 FP64 nodes (9 levels)
+constexpr int kTreeCacheBytes = 32768;
+
+// Every thread of the block calls this once, before any traversal: copies
+// node slots [0, ncache) of the tree (FP32 or FP64 records) into cache and
+// returns a walker that reads them from there.
+template <bool F32>
+__device__ __forceinline__ BvhWalk cache_tree_top(const BvhView& B, void* cache)
+{
+    const int64_t per = kTreeCacheBytes / (F32 ? (int64_t)sizeof(BvhNode32) : (int64_t)sizeof(BvhNode64));
+    const int64_t nc = B.nnode < per ? B.nnode : per;
+    const uint4* src = F32 ? reinterpret_cast<const uint4*>(B.n32) : reinterpret_cast<const uint4*>(B.n64);
+    uint4* dst = static_cast<uint4*>(cache);
+    const int64_t chunks = nc * (F32 ? (int64_t)sizeof(BvhNode32) : (int64_t)sizeof(BvhNode64)) / 16;
+    for (int64_t k = threadIdx.x; k < chunks; k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
+    BvhWalk w = walk(B);
+    if (F32) w.c32 = (LdsNode32*)cache;
+    else w.c64 = (LdsNode64*)cache;
+    w.ncache = nc;
+    return w;
+}
+
+// the two children c, c + 1 (same level: both cached or neither)
+__device__ __forceinline__ void load_pair(const BvhWalk& B, int64_t c, BvhNode32& L, BvhNode32& R)
+{
+    if (c < B.ncache) {
+        L = from_lds(B.c32 + c);
+        R = from_lds(B.c32 + c + 1);
+    } else {
+        L = B.n32[c];
+        R = B.n32[c + 1];
+    }
+}
+
+__device__ __forceinline__ void load_pair(const BvhWalk& B, int64_t c, BvhNode64& L, BvhNode64& R)
+{
+    if (c < B.ncache) {
+        L = from_lds(B.c64 + c);
+        R = from_lds(B.c64 + c + 1);
+    } else {
+        L = B.n64[c];
+        R = B.n64[c + 1];
+    }
+}
 
 __device__ __forceinline__ V3<double> ld3(const double* p) { return mk3(p[0], p[1], p[2]); }
 
-__device__ __forceinline__ double facet_d2(const BvhView& B, int f, V3<double> p)
+__device__ __forceinline__ double facet_d2(const BvhWalk& B, int f, V3<double> p)
 {
     const double* t = B.tri + 9 * (int64_t)f;
     V3<double> cp;
@@ -84,7 +196,7 @@ __device__ __forceinline__ float f32_threshold(V3<double> p, double sqrtT)
 }
 
 template <bool F32, bool STATS>
-__device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double T, double sqrtT,
+__device__ bool in_envelope(const BvhWalk& B, V3<double> p, double eps2, double T, double sqrtT,
                             int& hint, Stats& st)
 {
     if (hint >= 0) {
@@ -105,8 +217,17 @@ __device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double 
     int64_t n = 1;
     // the root box
     bool ok;
-    if (F32) ok = box_d2(pf, B.n32[1].lo, B.n32[1].hi) <= Tf;
-    else ok = box_d2(p, B.n64[1].lo, B.n64[1].hi) <= T;
+    if (F32) {
+        BvhNode32 r;
+        if (B.ncache > 1) r = from_lds(B.c32 + 1);
+        else r = B.n32[1];
+        ok = box_d2(pf, r.lo, r.hi) <= Tf;
+    } else {
+        BvhNode64 r;
+        if (B.ncache > 1) r = from_lds(B.c64 + 1);
+        else r = B.n64[1];
+        ok = box_d2(p, r.lo, r.hi) <= T;
+    }
     if (!ok) return false;
     for (;;) {
         // n is internal here: look at both children
@@ -115,7 +236,8 @@ __device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double 
         bool okl, okr;
         int leafl, leafr;
         if (F32) {
-            const BvhNode32 L = B.n32[c], R = B.n32[c + 1];
+            BvhNode32 L, R;
+            load_pair(B, c, L, R);
             float fl = box_d2(pf, L.lo, L.hi), fr = box_d2(pf, R.lo, R.hi);
             okl = fl <= Tf;
             okr = fr <= Tf;
@@ -124,7 +246,8 @@ __device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double 
             leafl = L.leaf;
             leafr = R.leaf;
         } else {
-            const BvhNode64 L = B.n64[c], R = B.n64[c + 1];
+            BvhNode64 L, R;
+            load_pair(B, c, L, R);
             dl = box_d2(p, L.lo, L.hi);
             dr = box_d2(p, R.lo, R.hi);
             okl = dl <= T;
@@ -154,7 +277,9 @@ __device__ bool in_envelope(const BvhView& B, V3<double> p, double eps2, double 
             trail &= ~(1u << d);
             n = (n >> (depth - d)) ^ 1;
             depth = d;
-            int leaf = F32 ? B.n32[n].leaf : B.n64[n].leaf;
+            int leaf;
+            if (F32) leaf = n < B.ncache ? B.c32[n].leaf : B.n32[n].leaf;
+            else leaf = n < B.ncache ? B.c64[n].leaf : B.n64[n].leaf;
             if (leaf < 0) break;
             if (STATS) st.leaves++;
             if (facet_d2(B, leaf, p) <= eps2) {
@@ -241,7 +366,7 @@ constexpr double kMaxEdgeSamples = 65536.0;
 
 // isFaceOutEnvelop_sampling for a triangle known not to be degenerate
 template <bool F32, bool STATS>
-__device__ bool face_out_sampled(const BvhView& B, const double* t9, double eps2, double d,
+__device__ bool face_out_sampled(const BvhWalk& B, const double* t9, double eps2, double d,
                                  double T, double sqrtT, Stats& st)
 {
     const V3<double> a = ld3(t9), b = ld3(t9 + 3), c = ld3(t9 + 6);
@@ -258,7 +383,7 @@ __device__ bool face_out_sampled(const BvhView& B, const double* t9, double eps2
 
 // isFaceOutEnvelop_sampling for one triangle (false for a collinear one)
 template <bool F32, bool STATS>
-__device__ bool face_out(const BvhView& B, const double* t9, double eps2, double d, double T,
+__device__ bool face_out(const BvhWalk& B, const double* t9, double eps2, double d, double T,
                          double sqrtT, Stats& st)
 {
     if (collinear3(t9, t9 + 3, t9 + 6)) return false;
@@ -272,6 +397,13 @@ inline BvhView bvh_view(const twg_bvh* b)
     v.n32 = b->node32;
     v.tri = b->tri;
     v.nf = b->nf;
+    v.nnode = b->nnode;
+    return v;
+}
+
+inline BvhView no_bvh()
+{
+    BvhView v = {nullptr, nullptr, nullptr, 0, 0};
     return v;
 }
 

@@ -193,16 +193,23 @@ __device__ __forceinline__ V3<double> sel3(int i, V3<double> x0, V3<double> x1, 
 // reference's generation order (longest edge, v1, rows parallel to it, v2,
 // then the two other edges), so a caller has a single consumption site.
 struct SampleGen {
-    V3<double> v0, v1, v2, n01, n02, n12, ne, rowv;
-    double d, sqrt3_2, ls1, ls2, N, sin_v0, tan_v0, sin_v1;
+    // State kept between samples: the rotated triangle, the spacing, the
+    // current edge's sample count, the current row's start and the counters.
+    // The unit directions and the angles of the rows are recomputed where they
+    // are used, with the same expressions (so every sample is bit-identical):
+    // kept, they would hold ~40 more registers across the envelope traversal
+    // that runs between two samples.
+    V3<double> v0, v1, v2, rowv;
+    double d, N;
     int phase, k, m, M, N1;
+
+    __device__ __forceinline__ static double sqrt3_2() { return __builtin_sqrt(3.0) / 2; }
 
     // the triangle arrives as three values (not an array): rotating an array
     // by a run-time index would place it in scratch memory
     __device__ __forceinline__ void init(V3<double> a, V3<double> b, V3<double> c, double d_)
     {
         d = d_;
-        sqrt3_2 = __builtin_sqrt(3.0) / 2;
         // squared edge lengths |v_i - v_(i+1)|^2
         const double l0 = len2(a - b), l1 = len2(b - c), l2 = len2(c - a);
         int max_i = 0;
@@ -222,12 +229,9 @@ struct SampleGen {
         v0 = sel3(max_i, a, b, c);
         v1 = sel3(max_i, b, c, a);
         v2 = sel3(max_i, c, a, b);
-        ls1 = max_i == 0 ? l1 : (max_i == 1 ? l2 : l0);
-        ls2 = max_i == 0 ? l2 : (max_i == 1 ? l0 : l1);
-        n01 = normalize(v1 - v0);
         V3<double> e01 = v1 - v0;
         double h = dist(divv(mulv(e01, dot(v2 - v0, e01)), lsm) + v0, v2);
-        M = (int)(h / (sqrt3_2 * d));
+        M = (int)(h / (sqrt3_2() * d));
         phase = 0;
     }
 
@@ -244,7 +248,7 @@ struct SampleGen {
                 return false;
             case 0:  // longest edge
                 if (k <= N) {
-                    p = v0 + mulv(mulv(n01, d), (double)k);
+                    p = v0 + mulv(mulv(normalize(v1 - v0), d), (double)k);
                     k++;
                     return true;
                 }
@@ -257,11 +261,6 @@ struct SampleGen {
                     p = v2;
                     return true;
                 }
-                n02 = normalize(v2 - v0);
-                n12 = normalize(v2 - v1);
-                sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
-                tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
-                sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
                 m = 1;
                 phase = 2;
                 break;
@@ -270,17 +269,22 @@ struct SampleGen {
                     phase = 4;
                     break;
                 }
-                int n = (int)(sqrt3_2 / tan_v0 * m + 0.5);
-                int n1 = (int)(sqrt3_2 / tan_v0 * m);
+                const double s32 = sqrt3_2();
+                const V3<double> n02 = normalize(v2 - v0), n12 = normalize(v2 - v1);
+                const double sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
+                const double tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
+                const double sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
+                int n = (int)(s32 / tan_v0 * m + 0.5);
+                int n1 = (int)(s32 / tan_v0 * m);
                 if (m % 2 == 0 && n == n1) n += 1;
-                V3<double> v0_m = v0 + mulv(n02, m * sqrt3_2 * d / sin_v0);
-                V3<double> v1_m = v1 + mulv(n12, m * sqrt3_2 * d / sin_v1);
+                V3<double> v0_m = v0 + mulv(n02, m * s32 * d / sin_v0);
+                V3<double> v1_m = v1 + mulv(n12, m * s32 * d / sin_v1);
                 if (dist(v0_m, v1_m) <= d) {
                     phase = 4;
                     break;
                 }
-                double delta_d = ((n + (m % 2) / 2.0) - m * sqrt3_2 / tan_v0) * d;
-                rowv = v0_m + mulv(n01, delta_d);
+                double delta_d = ((n + (m % 2) / 2.0) - m * s32 / tan_v0) * d;
+                rowv = v0_m + mulv(normalize(v1 - v0), delta_d);
                 N1 = (int)(dist(rowv, v1_m) / d);
                 k = 0;
                 phase = 3;
@@ -288,19 +292,18 @@ struct SampleGen {
             }
             case 3:  // row samples
                 if (k <= N1) {
-                    p = rowv + mulv(mulv(n01, (double)k), d);
+                    p = rowv + mulv(mulv(normalize(v1 - v0), (double)k), d);
                     k++;
                     return true;
                 }
                 m++;
                 phase = 2;
                 break;
-            case 4:  // v2, then the edge v1 -> v2
+            case 4:  // v2, then the edge v1 -> v2 (|v1 - v2|^2 is the edge's squared length)
                 p = v2;
-                N = __builtin_sqrt(ls1) / d;
+                N = __builtin_sqrt(len2(v1 - v2)) / d;
                 if (N > 1) {
                     if (N == (int)N) N -= 1;
-                    ne = normalize(v2 - v1);
                     k = 1;
                     phase = 5;
                 } else {
@@ -309,17 +312,16 @@ struct SampleGen {
                 return true;
             case 5:
                 if (k <= N) {
-                    p = v1 + mulv(mulv(ne, d), (double)k);
+                    p = v1 + mulv(mulv(normalize(v2 - v1), d), (double)k);
                     k++;
                     return true;
                 }
                 phase = 6;
                 break;
             case 6:  // the edge v2 -> v0
-                N = __builtin_sqrt(ls2) / d;
+                N = __builtin_sqrt(len2(v2 - v0)) / d;
                 if (N > 1) {
                     if (N == (int)N) N -= 1;
-                    ne = normalize(v0 - v2);
                     k = 1;
                     phase = 7;
                 } else {
@@ -328,7 +330,7 @@ struct SampleGen {
                 break;
             case 7:
                 if (k <= N) {
-                    p = v2 + mulv(mulv(ne, d), (double)k);
+                    p = v2 + mulv(mulv(normalize(v0 - v2), d), (double)k);
                     k++;
                     return true;
                 }
