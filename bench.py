@@ -145,6 +145,7 @@ def bench_amips(ctx, args, result):
     }
 
     if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
         torch.cuda.synchronize()
         cpu, parity = cpu_baseline_amips(x, y, z, tets, E.cpu().numpy(), H.cpu().numpy())
         result["cpu_baseline"] = cpu
@@ -234,6 +235,7 @@ def bench_envelope(ctx, args, result, surf_cache):
         "alg_gbs_fp64": alg * nq / rates["fp64"][1] / 1e9,
     }
     if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         S = O.Surface(V, F)
         k = 20000 if args.quick else 200000
@@ -302,6 +304,7 @@ def bench_slivers(ctx, args, result, surf_cache):
         "point_decision_mismatch_mixed_vs_fp64": int(np.count_nonzero(o64 != om)),
     })
     if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         S = O.Surface(V, F)
         k = 200000
@@ -364,6 +367,7 @@ def bench_smoothing(ctx, args, result, surf_cache):
     sec["batched_mixed_vs_one_launch_each"] = sec["problems_per_s_mixed"] * per_call
     sec["batched_fp64_vs_one_launch_each"] = sec["problems_per_s_fp64"] * per_call
     if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         S = O.Surface(V, F[surf.order()])
         t0 = time.perf_counter()
@@ -426,6 +430,7 @@ def bench_collapse(ctx, args, result):
                                  "bad_input"], np.bincount(st, minlength=8)) if s != "-"}
     del nq0
     if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         k = 20000 if args.quick else 200_000
         sub = dict(B)
@@ -477,6 +482,7 @@ def bench_reduce(ctx, args, result):
     sec["energy_stats"] = {"sum": s[0], "max": s[1], "second_max": s[2], "count": s[3],
                            "buckets": s[4]}
     if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         t0 = time.perf_counter()
         so = O.energy_stats(q.cpu().numpy(), removed.cpu().numpy(), tets, locked.cpu().numpy())
@@ -535,9 +541,16 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic", "config": {}, "secondary": {},
     }
-    if "amips" in only:
-        bench_amips(ctx, args, result)
+    t_all = time.time()
+
+    def stage(name, fn, *a):
+        t0 = time.time()
+        fn(ctx, args, result, *a)
         torch.cuda.empty_cache()
+        log(f"[{name}] done in {time.time() - t0:.1f}s (total {time.time() - t_all:.1f}s)")
+
+    if "amips" in only:
+        stage("amips", bench_amips)
     surf_cache = {}
     if only & {"envelope", "slivers", "smoothing"}:
         t0 = time.time()
@@ -547,21 +560,17 @@ def main():
         log(f"[surface] {len(F)} facets, tree depth {surf_cache['surf'].depth} "
             f"({time.time() - t0:.1f}s)")
     if "envelope" in only:
-        bench_envelope(ctx, args, result, surf_cache)
-        torch.cuda.empty_cache()
+        stage("envelope", bench_envelope, surf_cache)
     if "slivers" in only:
-        bench_slivers(ctx, args, result, surf_cache)
-        torch.cuda.empty_cache()
+        stage("slivers", bench_slivers, surf_cache)
     if "smoothing" in only:
-        bench_smoothing(ctx, args, result, surf_cache)
-        torch.cuda.empty_cache()
+        stage("smoothing", bench_smoothing, surf_cache)
     if "surf" in surf_cache:
         surf_cache["surf"].close()
     if "collapse" in only:
-        bench_collapse(ctx, args, result)
-        torch.cuda.empty_cache()
+        stage("collapse", bench_collapse)
     if "reduce" in only:
-        bench_reduce(ctx, args, result)
+        stage("reduce", bench_reduce)
     ctx.close()
     print(json.dumps(result), flush=True)
 
