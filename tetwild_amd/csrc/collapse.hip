@@ -285,10 +285,15 @@ __global__ __launch_bounds__(kBlock) void collapse_point_kernel(CollapseArgs A)
     }
 }
 
+// lanes per triangle in the face pass (samples spread over the group)
+constexpr int kFaceGroup = 8;
+
 template <bool F32>
 __global__ __launch_bounds__(kBlock) void collapse_face_kernel(CollapseArgs A)
 {
-    const int64_t k = xcd_block() * (int64_t)kBlock + threadIdx.x;
+    const int r = threadIdx.x % kFaceGroup;
+    const int gbase = (threadIdx.x % 64) - r;
+    const int64_t k = (xcd_block() * (int64_t)kBlock + threadIdx.x) / kFaceGroup;
     if (k >= A.n_tri) return;
     const int64_t c = A.tri_cand[k];
     // stale entries (from earlier batches) fail the range test
@@ -305,11 +310,12 @@ __global__ __launch_bounds__(kBlock) void collapse_face_kernel(CollapseArgs A)
         t9[3 * j + 2] = p.z;
     }
     if (!certainly_not_collinear(t9, t9 + 3, t9 + 6)) {
-        A.tri_cand[k] = -2 - (int32_t)c;  // for the exact face pass
+        if (r == 0) A.tri_cand[k] = -2 - (int32_t)c;  // for the exact face pass
         return;
     }
-    Stats st;
-    if (face_out_sampled<F32, false>(walk(A.sf), t9, A.eps2, A.d, A.T_sf, A.sqrtT_sf, st))
+    if (face_out_group<F32, kFaceGroup>(walk(A.sf), ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.eps2,
+                                        A.d, A.T_sf, A.sqrtT_sf, r, gbase) &&
+        r == 0)
         atomicOr(A.work + c, kFaceOut);
 }
 
@@ -456,8 +462,8 @@ int enqueue(twg_ctx* ctx, const CollapseArgs& A, int tile, bool f32, hipStream_t
         TWG_LAUNCHED();
     }
     if (A.n_tri > 0 && A.check_eps) {
-        const unsigned gtri = grid_for(A.n_tri);
-        const unsigned gtri_exact = std::min<unsigned>(gtri, 1024u);
+        const unsigned gtri = grid_for(A.n_tri * kFaceGroup);
+        const unsigned gtri_exact = std::min<unsigned>(grid_for(A.n_tri), 1024u);
         if (f32) {
             hipLaunchKernelGGL(collapse_face_kernel<true>, dim3(gtri), dim3(kBlock), 0, s, A);
             TWG_LAUNCHED();

@@ -48,14 +48,14 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
 
 // Triangles the degeneracy filter cannot settle are queued (pending[],
 // *n_pending) for faces_exact_kernel, which keeps the exact predicate -- and its
-// register footprint -- out of this kernel.
+// register footprint -- out of this kernel.  Capped at 128 VGPRs (4 waves
+// per SIMD, a few spills): more waves in flight hide more of the traversal's
+// load latency (+18% checks/s on 20 M queries against 1 M facets).
 template <bool F32, bool STATS>
-__global__ __launch_bounds__(kBlock) void faces_kernel(BvhView B, const double* __restrict__ tri9,
-                                                       int64_t n, double eps2, double d, double T,
-                                                       double sqrtT, uint8_t* __restrict__ out,
-                                                       unsigned long long* stats,
-                                                       unsigned* n_pending, int64_t* pending,
-                                                       const int32_t* __restrict__ order)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void faces_kernel(
+    BvhView B, const double* __restrict__ tri9, int64_t n, double eps2, double d, double T,
+    double sqrtT, uint8_t* __restrict__ out, unsigned long long* stats, unsigned* n_pending,
+    int64_t* pending, const int32_t* __restrict__ order)
 {
     __shared__ __align__(16) unsigned char cache[kTreeCacheBytes];
     const BvhWalk W = cache_tree_top<F32>(B, cache);

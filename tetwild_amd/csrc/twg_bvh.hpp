@@ -381,6 +381,51 @@ __device__ bool face_out_sampled(const BvhWalk& B, const double* t9, double eps2
     });
 }
 
+// face_out_sampled's decision computed by a group of G lanes (G a power of
+// two below 64, groups aligned in the wave; every lane of a group passes the
+// same triangle).  Lane r of the group tests samples r, r + G, r + 2G, ... of
+// the sampler's sequence, one sample per lane per round; after each round a
+// ballot over the group stops it at the first sample found outside.  The
+// decision -- some sample has no facet within eps -- does not depend on which
+// lane tests which sample, so it equals face_out_sampled's.  For batches of
+// few triangles with many samples each (the collapse's epsilon test), where
+// one lane per triangle leaves most of the machine idle.
+template <bool F32, int G>
+__device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c,
+                               double eps2, double d, double T, double sqrtT, int r, int gbase)
+{
+    static_assert(G >= 2 && G < 64 && (G & (G - 1)) == 0, "group size");
+    {
+        double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
+        if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
+    }
+    SampleGen g;
+    g.init(a, b, c, d);
+    int hint = -1;
+    Stats st;
+    const unsigned long long gmask = (1ull << G) - 1ull;
+    for (;;) {
+        // every lane steps the generator through the round (the same sequence
+        // in every lane of the group) and keeps sample r of it
+        V3<double> mine = a;
+        bool have = false, more = true;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            V3<double> q;
+            if (more) {
+                more = g.next(q);
+                if (more && k == r) {
+                    mine = q;
+                    have = true;
+                }
+            }
+        }
+        const bool out = have && !in_envelope<F32, false>(B, mine, eps2, T, sqrtT, hint, st);
+        if ((__ballot(out) >> gbase) & gmask) return true;
+        if (!more) return false;
+    }
+}
+
 // isFaceOutEnvelop_sampling for one triangle (false for a collinear one)
 template <bool F32, bool STATS>
 __device__ bool face_out(const BvhWalk& B, const double* t9, double eps2, double d, double T,
