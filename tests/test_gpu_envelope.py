@@ -85,6 +85,9 @@ def test_envelope_faces_match_oracle(ctx, dev, bumpy, precision):
     ref, tot = S.face_out(Q, eps * eps, eps)
     assert np.array_equal(out, ref)
     assert 0.2 < ref.mean() < 0.8
+    # without statistics a batch this small runs a wave per triangle
+    out_w = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision)
+    assert np.array_equal(out_w.cpu().numpy(), ref)
     st = st.cpu().numpy()
     print(f"precision={precision} out-rate={ref.mean():.3f} samples={st[0]} (all {tot}) "
           f"traversals={st[1]} boxes={st[2]} leaves={st[3]}")

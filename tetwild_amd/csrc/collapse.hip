@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void collapse_point_kernel(CollapseArgs A)
 }
 
 // lanes per triangle in the face pass (samples spread over the group)
-constexpr int kFaceGroup = 8;
+constexpr int kFaceGroup = 64;
 
 template <bool F32>
 __global__ __launch_bounds__(kBlock) void collapse_face_kernel(CollapseArgs A)

@@ -382,7 +382,7 @@ __device__ bool face_out_sampled(const BvhWalk& B, const double* t9, double eps2
 }
 
 // face_out_sampled's decision computed by a group of G lanes (G a power of
-// two below 64, groups aligned in the wave; every lane of a group passes the
+// two up to 64, groups aligned in the wave; every lane of a group passes the
 // same triangle).  Lane r of the group tests samples r, r + G, r + 2G, ... of
 // the sampler's sequence, one sample per lane per round; after each round a
 // ballot over the group stops it at the first sample found outside.  The
@@ -394,7 +394,7 @@ template <bool F32, int G>
 __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c,
                                double eps2, double d, double T, double sqrtT, int r, int gbase)
 {
-    static_assert(G >= 2 && G < 64 && (G & (G - 1)) == 0, "group size");
+    static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
     {
         double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
         if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
@@ -403,7 +403,7 @@ __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<
     g.init(a, b, c, d);
     int hint = -1;
     Stats st;
-    const unsigned long long gmask = (1ull << G) - 1ull;
+    const unsigned long long gmask = G == 64 ? ~0ull : (1ull << (G & 63)) - 1ull;
     for (;;) {
         // every lane steps the generator through the round (the same sequence
         // in every lane of the group) and keeps sample r of it
