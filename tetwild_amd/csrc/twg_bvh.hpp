@@ -383,13 +383,15 @@ __device__ bool face_out_sampled(const BvhWalk& B, const double* t9, double eps2
 
 // face_out_sampled's decision computed by a group of G lanes (G a power of
 // two up to 64, groups aligned in the wave; every lane of a group passes the
-// same triangle).  Lane r of the group tests samples r, r + G, r + 2G, ... of
-// the sampler's sequence, one sample per lane per round; after each round a
-// ballot over the group stops it at the first sample found outside.  The
-// decision -- some sample has no facet within eps -- does not depend on which
-// lane tests which sample, so it equals face_out_sampled's.  For batches of
-// few triangles with many samples each (the collapse's epsilon test), where
-// one lane per triangle leaves most of the machine idle.
+// same triangle).  The sampler's sequence of `total` samples is cut into G
+// consecutive blocks of K = ceil(total / G); lane r moves to sample r K
+// (SampleGen::skip passes whole rows by their headers) and tests its block
+// in order, one sample per step, and after each step a ballot over the group
+// stops it at the first sample found outside.  The decision -- some sample
+// has no facet within eps -- does not depend on which lane tests which
+// sample, so it equals face_out_sampled's.  For batches of few triangles
+// with many samples each, where one lane per triangle leaves most of the
+// machine idle.
 template <bool F32, int G>
 __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c,
                                double eps2, double d, double T, double sqrtT, int r, int gbase)
@@ -399,31 +401,27 @@ __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<
         double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
         if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
     }
+    int64_t total;
+    {
+        SampleGen cnt;
+        cnt.init(a, b, c, d);
+        total = cnt.skip(INT64_MAX);
+    }
+    const int64_t K = (total + G - 1) / G;
+    const int64_t first = (int64_t)r * K;
     SampleGen g;
     g.init(a, b, c, d);
+    g.skip(first);
     int hint = -1;
     Stats st;
     const unsigned long long gmask = G == 64 ? ~0ull : (1ull << (G & 63)) - 1ull;
-    for (;;) {
-        // every lane steps the generator through the round (the same sequence
-        // in every lane of the group) and keeps sample r of it
-        V3<double> mine = a;
-        bool have = false, more = true;
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            V3<double> q;
-            if (more) {
-                more = g.next(q);
-                if (more && k == r) {
-                    mine = q;
-                    have = true;
-                }
-            }
-        }
-        const bool out = have && !in_envelope<F32, false>(B, mine, eps2, T, sqrtT, hint, st);
+    for (int64_t j = 0; j < K; ++j) {
+        V3<double> q = a;
+        const bool have = first + j < total && g.next(q);
+        const bool out = have && !in_envelope<F32, false>(B, q, eps2, T, sqrtT, hint, st);
         if ((__ballot(out) >> gbase) & gmask) return true;
-        if (!more) return false;
     }
+    return false;
 }
 
 // isFaceOutEnvelop_sampling for one triangle (false for a collinear one)

@@ -235,6 +235,149 @@ struct SampleGen {
         phase = 0;
     }
 
+    // phase 2: the header of row m -- its start and sample count (phase 3),
+    // or the end of the rows (phase 4)
+    __device__ __forceinline__ void row_header()
+    {
+        if (m > M) {
+            phase = 4;
+            return;
+        }
+        const double s32 = sqrt3_2();
+        const V3<double> n02 = normalize(v2 - v0), n12 = normalize(v2 - v1);
+        const double sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
+        const double tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
+        const double sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
+        int n = (int)(s32 / tan_v0 * m + 0.5);
+        int n1 = (int)(s32 / tan_v0 * m);
+        if (m % 2 == 0 && n == n1) n += 1;
+        V3<double> v0_m = v0 + mulv(n02, m * s32 * d / sin_v0);
+        V3<double> v1_m = v1 + mulv(n12, m * s32 * d / sin_v1);
+        if (dist(v0_m, v1_m) <= d) {
+            phase = 4;
+            return;
+        }
+        double delta_d = ((n + (m % 2) / 2.0) - m * s32 / tan_v0) * d;
+        rowv = v0_m + mulv(normalize(v1 - v0), delta_d);
+        N1 = (int)(dist(rowv, v1_m) / d);
+        k = 0;
+        phase = 3;
+    }
+
+    // after v2: the edge v1 -> v2 (|v1 - v2|^2 is the edge's squared length)
+    __device__ __forceinline__ void start_edge12()
+    {
+        N = __builtin_sqrt(len2(v1 - v2)) / d;
+        if (N > 1) {
+            if (N == (int)N) N -= 1;
+            k = 1;
+            phase = 5;
+        } else {
+            phase = 6;
+        }
+    }
+
+    // the edge v2 -> v0; false when it has no interior samples (the end)
+    __device__ __forceinline__ bool start_edge20()
+    {
+        N = __builtin_sqrt(len2(v2 - v0)) / d;
+        if (!(N > 1)) return false;
+        if (N == (int)N) N -= 1;
+        k = 1;
+        phase = 7;
+        return true;
+    }
+
+    // Moves past the next s samples without producing them (a row is passed
+    // over from its header alone) and returns how many it passed (fewer than
+    // s at the end of the sequence).  next() then yields the sample after
+    // them, bit-identical to stepping through one at a time.
+    __device__ __forceinline__ int64_t skip(int64_t s)
+    {
+        int64_t done = 0;
+        for (;;) {
+            if (done >= s) return done;
+            const int64_t want = s - done;
+            switch (phase) {
+            case 9:
+                if (k >= 3) return done;
+                {
+                    const int64_t take = want < 3 - k ? want : 3 - k;
+                    k += (int)take;
+                    done += take;
+                }
+                break;
+            case 0:
+                if (k <= N) {
+                    const int64_t avail = (int64_t)N - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    phase = 1;  // v1
+                    done += 1;
+                }
+                break;
+            case 1:
+                if (M < 1) {
+                    phase = 8;  // v2, the last sample
+                    done += 1;
+                } else {
+                    m = 1;
+                    phase = 2;
+                }
+                break;
+            case 2:
+                row_header();
+                break;
+            case 3:
+                if (k <= N1) {
+                    const int64_t avail = (int64_t)N1 - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    m++;
+                    phase = 2;
+                }
+                break;
+            case 4:  // v2
+                start_edge12();
+                done += 1;
+                break;
+            case 5:
+                if (k <= N) {
+                    const int64_t avail = (int64_t)N - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    phase = 6;
+                }
+                break;
+            case 6:
+                if (!start_edge20()) {
+                    phase = 8;
+                    return done;
+                }
+                break;
+            case 7:
+                if (k <= N) {
+                    const int64_t avail = (int64_t)N - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    phase = 8;
+                    return done;
+                }
+                break;
+            default:
+                return done;
+            }
+        }
+    }
+
     __device__ __forceinline__ bool next(V3<double>& p)
     {
         for (;;) {
@@ -264,32 +407,9 @@ struct SampleGen {
                 m = 1;
                 phase = 2;
                 break;
-            case 2: {  // row header
-                if (m > M) {
-                    phase = 4;
-                    break;
-                }
-                const double s32 = sqrt3_2();
-                const V3<double> n02 = normalize(v2 - v0), n12 = normalize(v2 - v1);
-                const double sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
-                const double tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
-                const double sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
-                int n = (int)(s32 / tan_v0 * m + 0.5);
-                int n1 = (int)(s32 / tan_v0 * m);
-                if (m % 2 == 0 && n == n1) n += 1;
-                V3<double> v0_m = v0 + mulv(n02, m * s32 * d / sin_v0);
-                V3<double> v1_m = v1 + mulv(n12, m * s32 * d / sin_v1);
-                if (dist(v0_m, v1_m) <= d) {
-                    phase = 4;
-                    break;
-                }
-                double delta_d = ((n + (m % 2) / 2.0) - m * s32 / tan_v0) * d;
-                rowv = v0_m + mulv(normalize(v1 - v0), delta_d);
-                N1 = (int)(dist(rowv, v1_m) / d);
-                k = 0;
-                phase = 3;
+            case 2:  // row header
+                row_header();
                 break;
-            }
             case 3:  // row samples
                 if (k <= N1) {
                     p = rowv + mulv(mulv(normalize(v1 - v0), (double)k), d);
@@ -299,16 +419,9 @@ struct SampleGen {
                 m++;
                 phase = 2;
                 break;
-            case 4:  // v2, then the edge v1 -> v2 (|v1 - v2|^2 is the edge's squared length)
+            case 4:  // v2, then the edge v1 -> v2
                 p = v2;
-                N = __builtin_sqrt(len2(v1 - v2)) / d;
-                if (N > 1) {
-                    if (N == (int)N) N -= 1;
-                    k = 1;
-                    phase = 5;
-                } else {
-                    phase = 6;
-                }
+                start_edge12();
                 return true;
             case 5:
                 if (k <= N) {
@@ -319,14 +432,7 @@ struct SampleGen {
                 phase = 6;
                 break;
             case 6:  // the edge v2 -> v0
-                N = __builtin_sqrt(len2(v2 - v0)) / d;
-                if (N > 1) {
-                    if (N == (int)N) N -= 1;
-                    k = 1;
-                    phase = 7;
-                } else {
-                    return false;
-                }
+                if (!start_edge20()) return false;
                 break;
             case 7:
                 if (k <= N) {
