@@ -74,21 +74,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (STATS) flush_stats(st, stats);
 }
 
-// Small batches: a wave per triangle, its samples spread over the lanes
-// (face_out_group), so that few triangles with many samples each still fill
-// the machine.  Same outputs and pending queue as faces_kernel.
-template <bool F32>
-__global__ __launch_bounds__(kBlock) void faces_wave_kernel(BvhView B, const double* __restrict__ tri9,
-                                                            int64_t n, double eps2, double d,
-                                                            double T, double sqrtT,
-                                                            uint8_t* __restrict__ out,
-                                                            unsigned* n_pending, int64_t* pending)
+// A group of G lanes per triangle, its samples spread over the lanes
+// (face_out_group).  Small batches run G = 64 (a wave per triangle), so that
+// few triangles with many samples each still fill the machine.  Same outputs
+// and pending queue as faces_kernel.
+template <bool F32, int G>
+__global__ __launch_bounds__(kBlock) void faces_group_kernel(BvhView B, const double* __restrict__ tri9,
+                                                             int64_t n, double eps2, double d,
+                                                             double T, double sqrtT,
+                                                             uint8_t* __restrict__ out,
+                                                             unsigned* n_pending, int64_t* pending,
+                                                             const int32_t* __restrict__ order)
 {
     __shared__ __align__(16) unsigned char cache[kTreeCacheBytes];
     const BvhWalk W = cache_tree_top<F32>(B, cache);
-    const int r = threadIdx.x % 64;
-    const int64_t i = (xcd_block() * (int64_t)kBlock + threadIdx.x) / 64;
+    const int r = threadIdx.x % G;
+    const int gbase = (threadIdx.x % 64) - r;
+    int64_t i = (xcd_block() * (int64_t)kBlock + threadIdx.x) / G;
     if (i >= n) return;
+    if (order) i = order[i];
     const double* t = tri9 + 9 * i;
     if (!certainly_not_collinear(t, t + 3, t + 6)) {
         if (r == 0) {
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void faces_wave_kernel(BvhView B, const dou
         }
         return;
     }
-    const bool o = face_out_group<F32, 64>(W, ld3(t), ld3(t + 3), ld3(t + 6), eps2, d, T, sqrtT, r, 0);
+    const bool o = face_out_group<F32, G>(W, ld3(t), ld3(t + 3), ld3(t + 6), eps2, d, T, sqrtT, r, gbase);
     if (r == 0) out[i] = o ? 1 : 0;
 }
 
@@ -121,7 +125,7 @@ __global__ void zero_u32(unsigned* p) { *p = 0u; }
 
 // batches at least this large are walked in Morton order of their queries
 constexpr int64_t kSortMin = 1 << 16;
-// face batches below this size run a wave per triangle (faces_wave_kernel)
+// face batches below this size run a wave per triangle (faces_group_kernel)
 constexpr int64_t kWaveMax = 1 << 16;
 
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
@@ -441,9 +445,9 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
     if (n < kWaveMax && !stats) {
         dim3 gw(grid_for(n * 64));
         if (precision == TWG_FP64)
-            hipLaunchKernelGGL((faces_wave_kernel<false>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending);
+            hipLaunchKernelGGL((faces_group_kernel<false, 64>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
         else
-            hipLaunchKernelGGL((faces_wave_kernel<true>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending);
+            hipLaunchKernelGGL((faces_group_kernel<true, 64>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
     } else if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
         else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
