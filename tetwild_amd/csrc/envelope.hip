@@ -25,6 +25,10 @@ __device__ __forceinline__ void flush_stats(const Stats& st, unsigned long long*
     }
 }
 
+// LDS copy of the tree top for the point kernel (its registers allow more
+// resident blocks than 32 KB of LDS per block would)
+constexpr int kPointCacheBytes = 16384;
+
 template <bool F32, bool STATS>
 __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double* __restrict__ p3,
                                                         int64_t n, double eps2, double T,
@@ -32,8 +36,8 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
                                                         unsigned long long* stats,
                                                         const int32_t* __restrict__ order)
 {
-    __shared__ __align__(16) unsigned char cache[kTreeCacheBytes];
-    const BvhWalk W = cache_tree_top<F32>(B, cache);
+    __shared__ __align__(16) unsigned char cache[kPointCacheBytes];
+    const BvhWalk W = cache_tree_top<F32, kPointCacheBytes>(B, cache);
     int64_t i = xcd_block() * kBlock + threadIdx.x;
     Stats st;
     if (i < n) {

@@ -24,6 +24,8 @@ struct Partial {
     unsigned long long buckets[11];
 };
 
+constexpr int kUnroll = 4;
+
 struct ReduceArgs {
     const double* __restrict__ e;
     int64_t n;
@@ -74,37 +76,55 @@ __global__ __launch_bounds__(kBlock) void energy_partial_kernel(ReduceArgs A)
     p.bad = 0;
 #pragma unroll
     for (int k = 0; k < 11; ++k) p.buckets[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < A.n; i += stride) {
-        if (A.removed && A.removed[i]) continue;
-        const double e = A.e[i];
-        // getFilterEnergy: non-removed tets, locks ignored (:656-671); the
-        // bucket index first, then an unrolled update (no run-time indexing)
-        int bk = -1;
-        if (e > A.lim[10]) {
-            bk = 10;
-        } else {
+    // kUnroll coalesced elements per thread per step, their loads issued
+    // before any of them is used (the loop is latency-bound otherwise)
+    const int64_t stride = (int64_t)gridDim.x * kBlock * kUnroll;
+    const bool use_locks = A.tets && A.locked;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < A.n;
+         base += stride) {
+        uint8_t rm[kUnroll];
+        double ev[kUnroll];
+        int4 tv[kUnroll];
 #pragma unroll
-            for (int j = 9; j >= 0; --j)
-                if (e > A.lim[j] && e <= A.lim[j + 1]) bk = j;
+        for (int u = 0; u < kUnroll; ++u) {
+            const int64_t i = base + (int64_t)u * kBlock;
+            const bool in = i < A.n;
+            rm[u] = in ? (A.removed ? A.removed[i] : 0) : 1;
+            ev[u] = in ? A.e[i] : 0.0;
+            tv[u] = in && use_locks ? A.tets[i] : make_int4(0, 0, 0, 0);
         }
 #pragma unroll
-        for (int k = 0; k < 11; ++k) p.buckets[k] += (bk == k) ? 1ull : 0ull;
-        // isTetLocked_ui (:1377-1384)
-        if (A.tets && A.locked) {
-            const int4 t = A.tets[i];
-            if (t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 || t.z >= A.nv ||
-                t.w < 0 || t.w >= A.nv) {
-                p.bad++;
-                continue;
+        for (int u = 0; u < kUnroll; ++u) {
+            if (rm[u]) continue;
+            const double e = ev[u];
+            // getFilterEnergy: non-removed tets, locks ignored (:656-671); the
+            // bucket index first, then an unrolled update (no run-time indexing)
+            int bk = -1;
+            if (e > A.lim[10]) {
+                bk = 10;
+            } else {
+#pragma unroll
+                for (int j = 9; j >= 0; --j)
+                    if (e > A.lim[j] && e <= A.lim[j + 1]) bk = j;
             }
-            if (A.locked[t.x] || A.locked[t.y] || A.locked[t.z] || A.locked[t.w]) continue;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) p.buckets[k] += (bk == k) ? 1ull : 0ull;
+            // isTetLocked_ui (:1377-1384)
+            if (use_locks) {
+                const int4 t = tv[u];
+                if (t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 || t.z >= A.nv ||
+                    t.w < 0 || t.w >= A.nv) {
+                    p.bad++;
+                    continue;
+                }
+                if (A.locked[t.x] || A.locked[t.y] || A.locked[t.z] || A.locked[t.w]) continue;
+            }
+            // getAvgMaxEnergy / getMaxEnergy (:605-635), getSecondMaxEnergy (:637-650)
+            p.max = e > p.max ? e : p.max;
+            if (e != kMaxEnergy) p.second = e > p.second ? e : p.second;
+            p.sum += e;
+            p.count++;
         }
-        // getAvgMaxEnergy / getMaxEnergy (:605-635), getSecondMaxEnergy (:637-650)
-        p.max = e > p.max ? e : p.max;
-        if (e != kMaxEnergy) p.second = e > p.second ? e : p.second;
-        p.sum += e;
-        p.count++;
     }
     block_reduce(p, lds);
     if (threadIdx.x == 0) A.part[blockIdx.x] = p;

@@ -128,12 +128,13 @@ __device__ __forceinline__ BvhWalk walk(const BvhView& B)
 constexpr int kTreeCacheBytes = 32768;
 
 // Every thread of the block calls this once, before any traversal: copies
-// node slots [0, ncache) of the tree (FP32 or FP64 records) into cache and
-// returns a walker that reads them from there.
-template <bool F32>
+// node slots [0, ncache) of the tree (FP32 or FP64 records, BYTES bytes at
+// most) into cache and returns a walker that reads them from there.
+template <bool F32, int BYTES = kTreeCacheBytes>
 __device__ __forceinline__ BvhWalk cache_tree_top(const BvhView& B, void* cache)
 {
-    const int64_t per = kTreeCacheBytes / (F32 ? (int64_t)sizeof(BvhNode32) : (int64_t)sizeof(BvhNode64));
+    static_assert(BYTES % 128 == 0, "cache size: an even number of FP64 nodes");
+    const int64_t per = BYTES / (F32 ? (int64_t)sizeof(BvhNode32) : (int64_t)sizeof(BvhNode64));
     const int64_t nc = B.nnode < per ? B.nnode : per;
     const uint4* src = F32 ? reinterpret_cast<const uint4*>(B.n32) : reinterpret_cast<const uint4*>(B.n64);
     uint4* dst = static_cast<uint4*>(cache);
