@@ -1,0 +1,227 @@
+// Host build of the device headers (twg_geom.hpp, twg_bvh.hpp) with the GPU
+// qualifiers removed, checking the logic the kernels run:
+//   1. SampleGen against the oracle's sampleTriangle (bit-exact samples and
+//      counts), and SampleGen::skip against stepping one sample at a time;
+//   2. in_envelope -- global walker and LDS walker over the top levels, FP64
+//      and FP32 boxes -- against the brute-force minimum over all facets, on
+//      trees of 1..50k facets (the tree built by the library's own host code);
+//   3. face_out_group's block split (G lanes emulated one after another, the
+//      ballot as the OR of the lanes' step results) against face_out_sampled,
+//      and that every sample of an inside triangle is tested exactly once.
+// Exit status 0 when every check passes.
+#include "twg_bvh.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "build_part.hpp"
+
+extern "C" int64_t orc_sample_triangle(const double* v0, const double* v1, const double* v2,
+                                       double d, double* ps, int64_t cap);
+using namespace twg;
+
+static long g_bad = 0;
+#define CHECK(c, ...)                  \
+    do {                               \
+        if (!(c)) {                    \
+            if (g_bad < 10) printf(__VA_ARGS__); \
+            g_bad++;                   \
+        }                              \
+    } while (0)
+
+static void check_sampler(std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(-1, 1);
+    std::vector<double> ps(3 * 100000);
+    long samples = 0, positions = 0;
+    for (int it = 0; it < 6000; ++it) {
+        double t[9];
+        const double s = (it % 3 == 0) ? 0.01 : (it % 3 == 1 ? 0.003 : 0.02);
+        for (int k = 0; k < 9; ++k) t[k] = U(rng) * s;
+        if (it % 17 == 0)  // nearly collinear
+            for (int k = 0; k < 3; ++k) t[6 + k] = t[k] + 0.5 * (t[3 + k] - t[k]) + (k == 2 ? 1e-4 : 0);
+        if (it % 23 == 0)  // an edge of exactly 4 sampling distances
+            for (int k = 0; k < 3; ++k) t[3 + k] = t[k] + (k == 0 ? 0.004 : 0.0);
+        const double d = 0.001;
+        const int64_t n = orc_sample_triangle(t, t + 3, t + 6, d, ps.data(), 100000);
+        const V3<double> a = mk3(t[0], t[1], t[2]), b = mk3(t[3], t[4], t[5]), c = mk3(t[6], t[7], t[8]);
+        std::vector<V3<double>> seq;
+        for_each_sample(a, b, c, d, [&](V3<double> p) {
+            seq.push_back(p);
+            return (int64_t)seq.size() > n + 10;
+        });
+        CHECK((int64_t)seq.size() == n, "sampler: triangle %d gives %zu samples, oracle %lld\n", it,
+              seq.size(), (long long)n);
+        for (int64_t j = 0; j < n && j < (int64_t)seq.size(); ++j)
+            CHECK(seq[j].x == ps[3 * j] && seq[j].y == ps[3 * j + 1] && seq[j].z == ps[3 * j + 2],
+                  "sampler: triangle %d sample %lld differs\n", it, (long long)j);
+        samples += n;
+        SampleGen cnt;
+        cnt.init(a, b, c, d);
+        CHECK(cnt.skip(INT64_MAX) == (int64_t)seq.size(), "skip: count of triangle %d\n", it);
+        const size_t step = it % 5 == 0 ? 1 : 1 + seq.size() / 7;
+        for (size_t s0 = 0; s0 <= seq.size(); s0 += step) {
+            SampleGen g;
+            g.init(a, b, c, d);
+            const int64_t got = g.skip((int64_t)s0);
+            V3<double> p;
+            const bool more = g.next(p);
+            positions++;
+            CHECK(got == (int64_t)s0, "skip: triangle %d skipped %lld of %zu\n", it, (long long)got, s0);
+            if (s0 < seq.size())
+                CHECK(more && !memcmp(&p, &seq[s0], sizeof p), "skip: triangle %d sample %zu\n", it, s0);
+            else
+                CHECK(!more, "skip: triangle %d past the end\n", it);
+        }
+    }
+    printf("sampler: 6000 triangles, %ld samples, %ld skip positions\n", samples, positions);
+}
+
+struct Tree {
+    std::vector<double> tri;
+    std::vector<BvhNode64> n64;
+    std::vector<BvhNode32> n32;
+    BvhView view;
+};
+
+static void build(Tree& T, int nf, std::mt19937_64& rng, double size)
+{
+    std::uniform_real_distribution<double> U(0, 1);
+    T.tri.resize(9 * (size_t)nf);
+    for (int f = 0; f < nf; ++f) {
+        const double cx = U(rng), cy = U(rng), cz = 0.05 * sin(7 * cx) * cos(5 * cy);
+        for (int k = 0; k < 3; ++k) {
+            T.tri[9 * f + 3 * k] = cx + size * (U(rng) - 0.5);
+            T.tri[9 * f + 3 * k + 1] = cy + size * (U(rng) - 0.5);
+            T.tri[9 * f + 3 * k + 2] = cz + 0.25 * size * (U(rng) - 0.5);
+        }
+    }
+    const int64_t nnode = max_node(1, 0, nf) + 1;
+    T.n64.assign(nnode, BvhNode64{});
+    for (auto& N : T.n64) {
+        for (int c = 0; c < 3; ++c) {
+            N.lo[c] = INFINITY;
+            N.hi[c] = -INFINITY;
+        }
+        N.leaf = -2;
+    }
+    int depth = 0;
+    fill(T.n64, T.tri, 1, 0, nf, 0, depth);
+    T.n32.resize(nnode);
+    for (int64_t i = 0; i < nnode; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            T.n32[i].lo[c] = round_down(T.n64[i].lo[c]);
+            T.n32[i].hi[c] = round_up(T.n64[i].hi[c]);
+        }
+        T.n32[i].leaf = T.n64[i].leaf;
+        T.n32[i].pad = 0;
+    }
+    T.view = {T.n64.data(), T.n32.data(), T.tri.data(), nf, nnode};
+}
+
+static void check_traversal(std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(0, 1);
+    std::vector<unsigned char> cache(kTreeCacheBytes);
+    long queries = 0;
+    for (int nf : {1, 2, 3, 7, 100, 1000, 3001, 20001}) {
+        Tree T;
+        build(T, nf, rng, 0.02);
+        const double eps = 0.03, eps2 = eps * eps, Tt = eps2 * (1 + 1e-9), sqrtT = sqrt(Tt);
+        for (int mode = 0; mode < 4; ++mode) {
+            const bool f32 = mode & 1, lds = mode & 2;
+            const BvhWalk W = lds ? (f32 ? cache_tree_top<true>(T.view, cache.data())
+                                         : cache_tree_top<false>(T.view, cache.data()))
+                                  : walk(T.view);
+            for (int q = 0; q < 600; ++q) {
+                const V3<double> p = mk3(U(rng), U(rng), 0.05 * sin(7 * U(rng)) + 0.01 * (U(rng) - 0.5));
+                double best = INFINITY;
+                for (int f = 0; f < nf; ++f) best = std::min(best, facet_d2(W, f, p));
+                Stats st;
+                int hint = (q % 3 == 0) ? (int)(rng() % nf) : -1;
+                const bool got = f32 ? in_envelope<true, false>(W, p, eps2, Tt, sqrtT, hint, st)
+                                     : in_envelope<false, false>(W, p, eps2, Tt, sqrtT, hint, st);
+                CHECK(got == (best <= eps2), "traversal: %d facets, mode %d, query %d\n", nf, mode, q);
+                queries++;
+            }
+        }
+    }
+    printf("traversal: %ld queries against brute force\n", queries);
+}
+
+template <int G>
+static bool group_emulated(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c, double eps2,
+                           double d, double Tt, double sqrtT, long& tested)
+{
+    const double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
+    if (!(sqrt(l) / d <= kMaxEdgeSamples)) return true;
+    SampleGen cnt;
+    cnt.init(a, b, c, d);
+    const int64_t total = cnt.skip(INT64_MAX);
+    const int64_t K = (total + G - 1) / G;
+    std::vector<SampleGen> g(G);
+    std::vector<int> hint(G, -1);
+    Stats st;
+    for (int r = 0; r < G; ++r) {
+        g[r].init(a, b, c, d);
+        g[r].skip((int64_t)r * K);
+    }
+    for (int64_t j = 0; j < K; ++j) {
+        bool any = false;
+        for (int r = 0; r < G; ++r) {
+            V3<double> q = a;
+            const bool have = (int64_t)r * K + j < total && g[r].next(q);
+            tested += have;
+            any |= have && !in_envelope<false, false>(B, q, eps2, Tt, sqrtT, hint[r], st);
+        }
+        if (any) return true;
+    }
+    return false;
+}
+
+static void check_group(std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(0, 1);
+    Tree T;
+    build(T, 4000, rng, 0.03);
+    const BvhWalk W = walk(T.view);
+    const double eps = 0.01, eps2 = eps * eps, Tt = eps2 * (1 + 1e-9), sqrtT = sqrt(Tt);
+    long outs = 0;
+    for (int q = 0; q < 800; ++q) {
+        const double cx = U(rng), cy = U(rng), s = 0.02 + 0.08 * U(rng);
+        double t9[9];
+        for (int k = 0; k < 3; ++k) {
+            t9[3 * k] = cx + s * (U(rng) - 0.5);
+            t9[3 * k + 1] = cy + s * (U(rng) - 0.5);
+            t9[3 * k + 2] = 0.05 * sin(7 * cx) * cos(5 * cy) + 0.02 * (U(rng) - 0.5);
+        }
+        Stats st;
+        const double d = eps * 0.5;
+        const bool ref = face_out_sampled<false, false>(W, t9, eps2, d, Tt, sqrtT, st);
+        int64_t total = 0;
+        for_each_sample(ld3(t9), ld3(t9 + 3), ld3(t9 + 6), d, [&](V3<double>) {
+            total++;
+            return false;
+        });
+        long t64 = 0, t8 = 0, t2 = 0;
+        const bool g64 = group_emulated<64>(W, ld3(t9), ld3(t9 + 3), ld3(t9 + 6), eps2, d, Tt, sqrtT, t64);
+        const bool g8 = group_emulated<8>(W, ld3(t9), ld3(t9 + 3), ld3(t9 + 6), eps2, d, Tt, sqrtT, t8);
+        const bool g2 = group_emulated<2>(W, ld3(t9), ld3(t9 + 3), ld3(t9 + 6), eps2, d, Tt, sqrtT, t2);
+        CHECK(g64 == ref && g8 == ref && g2 == ref, "group: triangle %d decision\n", q);
+        if (!ref) CHECK(t64 == total && t8 == total && t2 == total, "group: triangle %d coverage\n", q);
+        outs += ref;
+    }
+    printf("group split: 800 triangles, %ld out\n", outs);
+}
+
+int main()
+{
+    std::mt19937_64 rng(20240601);
+    check_sampler(rng);
+    check_traversal(rng);
+    check_group(rng);
+    printf("%ld failures\n", g_bad);
+    return g_bad != 0;
+}
