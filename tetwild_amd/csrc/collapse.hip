@@ -14,13 +14,14 @@
 //                  marks the point checks and the epsilon check still open and
 //                  maps the candidate's triangles to it.  Orientation comes
 //                  from the floating-point filter; a candidate with no tet
-//                  certainly flipped but some tet undecided is marked, and
-//   1b.            the exact ring pass redoes the marked candidates with the
+//                  certainly flipped but some tet undecided is listed, and
+//   1b.            the exact ring pass redoes the listed candidates with the
 //                  exact predicate (the same code, so the same results).
 //   2. point pass  one thread per candidate: the open point checks.
-//   3. face pass   one thread per triangle of a candidate whose epsilon check
-//                  is still open: isFaceOutEnvelop; triangles whose
-//                  collinearity the filter cannot decide are marked, and
+//   3. face pass   a wave per triangle of a candidate whose epsilon check is
+//                  still open (its samples spread over the lanes):
+//                  isFaceOutEnvelop; triangles whose collinearity the filter
+//                  cannot decide are marked, and
 //   3b.            the exact face pass finishes them.
 //   4. final pass  one thread per candidate: the reference's return code.
 // The exact predicates stay out of passes 1 and 3, which keeps their register
@@ -71,9 +72,12 @@ struct CollapseArgs {
     double* __restrict__ new_q;
     unsigned* __restrict__ work;
     int32_t* __restrict__ tri_cand;
-    unsigned* counter;  // the ring pass's queue
-    unsigned chunk;     // candidates per queue pop
+    unsigned* counter;     // [0] the ring pass's queue, [kExactCount] exact-list length
+    int32_t* exact_list;   // candidates the ring pass left to the exact pass
+    unsigned chunk;        // candidates per queue pop
 };
+
+constexpr int kExactCount = 16;  // counter slot (64 bytes after the queue)
 
 __device__ __forceinline__ V3<double> vtx(const CollapseArgs& A, int id)
 {
@@ -125,6 +129,85 @@ __device__ __forceinline__ double tet_quality_positive(const Tet& T)
     return e;
 }
 
+// The common case of ring_candidate (at most TILE ring tets, so lane tl owns
+// tet tl): one sweep validates the lane's tet, then orients it and evaluates
+// its quality; the qualities are written only when no tet is flipped or
+// undecided, as in the sweep-per-check order.  Same outputs as the general
+// path below.
+template <int TILE, bool EXACT>
+__device__ void ring_candidate_fused(const CollapseArgs& A, int64_t c, int tl, int wbase, int v1,
+                                     int v2, unsigned fl, int m, const int4* rt, int64_t r0,
+                                     int64_t t0, int64_t t1)
+{
+    const bool v1_sf = fl & TWG_CV1_SURFACE, v1_bd = fl & TWG_CV1_BOUNDARY;
+    const bool v2_sf = fl & TWG_CV2_SURFACE, v2_bd = fl & TWG_CV2_BOUNDARY;
+    const bool mine = tl < m;
+    const int4 t = mine ? rt[tl] : make_int4(0, 0, 0, 0);
+    bool bid = mine && (t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 ||
+                        t.z >= A.nv || t.w < 0 || t.w >= A.nv || !has_vertex(t, v1));
+    for (int64_t k = t0 + tl; k < t1; k += TILE) {
+        int a = A.tri_ids[3 * k], b = A.tri_ids[3 * k + 1], e = A.tri_ids[3 * k + 2];
+        bid = bid || a < 0 || a >= A.nv || b < 0 || b >= A.nv || e < 0 || e >= A.nv;
+    }
+    unsigned work = 0;
+    uint8_t st = TWG_COLLAPSE_SUCCESS;
+    const bool bad = tile_any<TILE>(wbase, bid);
+    if (bad) {
+        st = TWG_COLLAPSE_BAD_INPUT;
+    } else {
+        const V3<double> p1 = vtx(A, v1), p2 = vtx(A, v2);
+        const double dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+        const bool degenerate = (dx * dx + dy * dy + dz * dz) == 0.0;
+        if (v1_bd && !v2_bd && !degenerate) work |= kNeedBoundaryPoint;
+        if (v1_sf && !v2_sf && !degenerate) work |= kNeedSurfacePoint;
+        const bool removed = mine && has_vertex(t, v2);
+        bool flip = false, undecided = false;
+        double q = __builtin_nan("");
+        if (mine && !removed) {
+            Tet T;
+            new_tet(A, t, v1, p2, T);
+            const int o = EXACT ? orient3d(T.p[0], T.p[1], T.p[2], T.p[3])
+                                : orient3d_filter(T.p[0], T.p[1], T.p[2], T.p[3]);
+            if (o == kUndecided) undecided = true;
+            else flip = o != 1;
+            q = tet_quality_positive(T);
+        }
+        if (tile_any<TILE>(wbase, flip)) {
+            st = TWG_COLLAPSE_FLIP;
+        } else if (!EXACT && tile_any<TILE>(wbase, undecided)) {
+            work = kNeedExact;
+        } else {
+            double om = mine ? A.old_q[r0 + tl] : 0.0;
+            om = om > 0.0 ? om : 0.0;
+            double nm = mine && !removed ? (q > 0.0 ? q : 0.0) : 0.0;
+            if (mine) A.new_q[r0 + tl] = q;
+            om = tile_max<TILE>(om);
+            nm = tile_max<TILE>(nm);
+            if (A.check_quality) {
+                if (A.is_soft) om = A.soft_energy;
+                if (fl & TWG_CV1_UNROUNDED) nm = 0.0;
+            }
+            if (A.check_quality && !degenerate && !(nm <= om)) {
+                st = TWG_COLLAPSE_QUALITY;
+            } else if (A.check_eps && v1_sf) {
+                st = TWG_COLLAPSE_ENVELOP_SUC;
+                if (!degenerate && t1 > t0) work |= kFacesOpen;
+            }
+        }
+    }
+    if (!bad)
+        for (int64_t k = t0 + tl; k < t1; k += TILE)
+            A.tri_cand[k] = (work & kFacesOpen) ? (int32_t)c : -1;
+    if (tl == 0) {
+        A.status[c] = st;
+        A.work[c] = work;
+        if (!EXACT && (work & kNeedExact)) {
+            const unsigned k = atomicAdd(A.counter + kExactCount, 1u);
+            A.exact_list[k] = (int32_t)c;
+        }
+    }
+}
+
 // EXACT: orientations from the exact predicate; otherwise from the filter,
 // with undecided rings marked kNeedExact
 template <int TILE, bool EXACT>
@@ -146,6 +229,10 @@ __device__ void ring_candidate(const CollapseArgs& A, int64_t c, int tl, int wba
                (v1_sf && !A.has_sf) || (v1_bd && !A.has_bt);
     const int m = bad ? 0 : (int)(r1 - r0);
     const int4* rt = A.ring + r0;
+    if (!bad && m <= TILE) {
+        ring_candidate_fused<TILE, EXACT>(A, c, tl, wbase, v1, v2, fl, m, rt, r0, t0, t1);
+        return;
+    }
     if (!bad) {
         bool bid = false;
         for (int j = tl; j < m; j += TILE) {
@@ -225,6 +312,10 @@ __device__ void ring_candidate(const CollapseArgs& A, int64_t c, int tl, int wba
     if (tl == 0) {
         A.status[c] = st;
         A.work[c] = work;
+        if (!EXACT && (work & kNeedExact)) {
+            const unsigned k = atomicAdd(A.counter + kExactCount, 1u);
+            A.exact_list[k] = (int32_t)c;
+        }
     }
 }
 
@@ -247,19 +338,24 @@ __global__ __launch_bounds__(kBlock) void collapse_ring_kernel(CollapseArgs A)
     }
 }
 
-// the candidates the filter pass marked, with the exact predicate; a tile per
-// candidate, grid-stride (the loop bound is uniform across a tile)
+// the candidates the filter pass listed, with the exact predicate; a tile per
+// candidate, grid-stride over the list (the loop bound is uniform across a tile)
 template <int TILE>
 __global__ __launch_bounds__(kBlock) void collapse_ring_exact_kernel(CollapseArgs A)
 {
     const int tl = threadIdx.x % TILE;
     const int wbase = (threadIdx.x % 64) - tl;
     const int64_t stride = (int64_t)gridDim.x * (kBlock / TILE);
-    for (int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / TILE; c < A.n; c += stride)
-        if (A.work[c] & kNeedExact) ring_candidate<TILE, true>(A, c, tl, wbase);
+    const int64_t listed = A.counter[kExactCount];
+    for (int64_t k = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / TILE; k < listed; k += stride)
+        ring_candidate<TILE, true>(A, A.exact_list[k], tl, wbase);
 }
 
-__global__ void reset_queue(unsigned* c) { *c = 0u; }
+__global__ void reset_queue(unsigned* c)
+{
+    c[0] = 0u;
+    c[kExactCount] = 0u;
+}
 
 template <bool F32>
 __global__ __launch_bounds__(kBlock) void collapse_point_kernel(CollapseArgs A)
@@ -412,21 +508,23 @@ int default_tile(const CollapseArgs& A)
     return mean <= 12.0 ? 16 : (mean <= 40.0 ? 32 : 64);
 }
 
-// bytes of per-batch scratch: work[n], tri_cand[n_tri], the queue counter
+// bytes of per-batch scratch: work[n], tri_cand[n_tri], the counters (256
+// bytes), exact_list[n]
 size_t scratch_bytes(const CollapseArgs& A)
 {
-    return (((size_t)A.n * 4 + 255) & ~(size_t)255) +
-           (((size_t)std::max<int64_t>(A.n_tri, 1) * 4 + 255) & ~(size_t)255) + 256;
+    const size_t w = ((size_t)std::max<int64_t>(A.n, 1) * 4 + 255) & ~(size_t)255;
+    return 2 * w + (((size_t)std::max<int64_t>(A.n_tri, 1) * 4 + 255) & ~(size_t)255) + 256;
 }
 
 void bind_scratch(CollapseArgs& A, void* base)
 {
     char* b = static_cast<char*>(base);
-    const size_t w = ((size_t)A.n * 4 + 255) & ~(size_t)255;
+    const size_t w = ((size_t)std::max<int64_t>(A.n, 1) * 4 + 255) & ~(size_t)255;
     const size_t t = ((size_t)std::max<int64_t>(A.n_tri, 1) * 4 + 255) & ~(size_t)255;
     A.work = reinterpret_cast<unsigned*>(b);
     A.tri_cand = reinterpret_cast<int32_t*>(b + w);
     A.counter = reinterpret_cast<unsigned*>(b + w + t);
+    A.exact_list = reinterpret_cast<int32_t*>(b + w + t + 256);
 }
 
 using RingFn = void (*)(CollapseArgs);
