@@ -21,7 +21,9 @@ Secondary measurements in the same JSON line ("secondary"):
                  autotuned tiles, the batch replayed from a hipGraph, FP64 and
                  mixed
   reduce         the energy reductions (twg_energy_stats) over 10.1M stored
-                 qualities and the configuration check (twg_tet_groups) over
+                 qualities, the bulk quality pass (twg_tet_qualities, FP64
+                 and mixed) and isFlip (twg_tet_flip) over the 10.1M lattice
+                 tets, and the configuration check (twg_tet_groups) over
                  2.5M groups of four tets
 Each carries its CPU baseline (the CPU restatement, one host core, bounded
 sample) and its parity counts against that restatement.
@@ -492,6 +494,32 @@ def bench_reduce(ctx, args, result):
         sec["max_bucket_count_mismatch_vs_cpu"] = int(
             (s[1] != so[1]) + (s[2] != so[2]) + (s[3] != so[3]) + int(list(s[4]) != list(so[4])))
         sec["sum_rel_diff_vs_cpu"] = abs(s[0] - so[0]) / abs(so[0])
+    # the bulk quality pass (calTetQualities over every tet) and isFlip over
+    # every tet, on the same lattice
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        wall, kern = events_time(lambda: ctx.tet_qualities(tt, tx, ty, tz, precision=prec), k, 2)
+        sec[f"qualities_per_s_{name}"] = n / wall
+        sec[f"qualities_gbs_{name}"] = (n * (16 + 8 + 1) + 24 * nv) / wall / 1e9
+    wall, kern = events_time(lambda: ctx.tet_flip(tt, tx, ty, tz), k, 2)
+    sec["flip_checks_per_s"] = n / wall
+    sec["flip_gbs"] = (n * (16 + 1) + 24 * nv) / wall / 1e9
+    q64 = ctx.tet_qualities(tt, tx, ty, tz, precision=0)[0]
+    qm = ctx.tet_qualities(tt, tx, ty, tz, precision=1)[0]
+    # the mixed pass returns FP64 values within tol (relative) and the same
+    # MAX_ENERGY sentinels
+    finite = q64 < 1e50
+    sec["qualities_mixed_max_rel_err"] = ((qm - q64).abs() / q64.abs())[finite].max().item()
+    sec["qualities_sentinel_mismatch_mixed_vs_fp64"] = int(((q64 == 1e50) != (qm == 1e50)).sum().item())
+    if not args.no_cpu:
+        log("[cpu] timing the CPU baseline sample")
+        from oracle import oracle as O
+        kq = 2_000_000
+        t0 = time.perf_counter()
+        qo = O.tet_qualities(tets[:kq], x, y, z)
+        el = time.perf_counter() - t0
+        sec["qualities_cpu_baseline"] = {"value": kq / el, "unit": "tets/s", "cores": 1, "kind": "port",
+                                         "sample": f"first {kq} lattice tets, CPU restatement"}
+        sec["qualities_bit_mismatch_vs_cpu"] = bits_mismatch(q64[:kq].cpu().numpy(), qo)
     g = n // 4
     off = torch.arange(0, 4 * g + 1, 4, dtype=torch.int32, device=ctx.device)
     groups = tt[:4 * g].contiguous()
