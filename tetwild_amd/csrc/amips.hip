@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void tet_quality_kernel(
     const double* __restrict__ y, const double* __restrict__ z, int64_t nv, float tol,
     double* __restrict__ out, uint8_t* __restrict__ fallback, Pending P)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     Tet T;
     uint8_t fb = 0;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void tet_flip_kernel(const int32_t* __restr
                                                           const double* __restrict__ z, int64_t nv,
                                                           uint8_t* __restrict__ flipped, Pending P)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     Tet T;
     bool pend = false;

@@ -4,8 +4,8 @@
 // getFilterEnergy (src/tetwild/LocalOperations.cpp:605-693), in one pass over
 // the per-tet slim energies resident in HBM.
 //
-// Each thread walks a grid-stride slice; blocks reduce through LDS in a fixed
-// tree and write partials; one block reduces the partials in the same fixed
+// Each block walks a contiguous range of the tets; blocks reduce through LDS
+// in a fixed tree and write partials; one block reduces the partials in the same fixed
 // order.  The result is deterministic for a given device (the grid depends
 // only on the CU count).  Max, second max, counts and the histogram are exact;
 // the sum differs from the reference's sequential loop by rounding only.
@@ -76,19 +76,22 @@ __global__ __launch_bounds__(kBlock) void energy_partial_kernel(ReduceArgs A)
     p.bad = 0;
 #pragma unroll
     for (int k = 0; k < 11; ++k) p.buckets[k] = 0;
-    // kUnroll coalesced elements per thread per step, their loads issued
-    // before any of them is used (the loop is latency-bound otherwise)
-    const int64_t stride = (int64_t)gridDim.x * kBlock * kUnroll;
+    // Each block walks its own contiguous range (blocks in XCD order, so the
+    // vertex flags its tets read stay in one L2), kUnroll coalesced elements
+    // per thread per step with their loads issued before any is used.
+    const int64_t step = (int64_t)kBlock * kUnroll;
+    const int64_t per = ((A.n + gridDim.x - 1) / gridDim.x + step - 1) / step * step;
+    const int64_t lo = xcd_block() * per;
+    const int64_t hi = lo + per < A.n ? lo + per : A.n;
     const bool use_locks = A.tets && A.locked;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < A.n;
-         base += stride) {
+    for (int64_t base = lo + threadIdx.x; base < hi; base += step) {
         uint8_t rm[kUnroll];
         double ev[kUnroll];
         int4 tv[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int64_t i = base + (int64_t)u * kBlock;
-            const bool in = i < A.n;
+            const bool in = i < hi;
             rm[u] = in ? (A.removed ? A.removed[i] : 0) : 1;
             ev[u] = in ? A.e[i] : 0.0;
             tv[u] = in && use_locks ? A.tets[i] : make_int4(0, 0, 0, 0);
