@@ -73,6 +73,23 @@ struct TileCtx {
     double* red;    // block LDS, kQ doubles per thread
 };
 
+// acc + v[0] + v[1] + ... + v[cnt - 1] in that order (v[r] = base[r * stride]
+// in LDS): the reads go out four at a time before their additions, which
+// stay sequential (the same sum as a plain loop, bit for bit)
+template <int TILE>
+__device__ __forceinline__ double ordered_sum(double acc, const double* base, int stride, int cnt)
+{
+    for (int r0 = 0; r0 < cnt; r0 += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = r0 + u < cnt ? base[(r0 + u) * stride] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (r0 + u < cnt) acc += v[u];
+    }
+    return acc;
+}
+
 template <int TILE>
 __device__ __forceinline__ bool tile_any(const TileCtx<TILE>& c, bool pred)
 {
@@ -145,8 +162,7 @@ __device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const i
         for (int q = 0; q < kQ; ++q) mine[q] = val[q];
         wave_sync();
         int cnt = m - s < TILE ? m - s : TILE;
-        if (c.tl < kQ)
-            for (int r = 0; r < cnt; ++r) acc += c.red[(c.tbase + r) * kQ + c.tl];
+        if (c.tl < kQ) acc = ordered_sum<TILE>(acc, c.red + c.tbase * kQ + c.tl, kQ, cnt);
         wave_sync();
     }
     E = __shfl(acc, 0, TILE);
@@ -184,8 +200,7 @@ __device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const
         c.red[(c.tbase + c.tl) * kQ] = e;
         wave_sync();
         int cnt = m - s < TILE ? m - s : TILE;
-        if (c.tl == 0)
-            for (int r = 0; r < cnt; ++r) acc += c.red[(c.tbase + r) * kQ];
+        if (c.tl == 0) acc = ordered_sum<TILE>(acc, c.red + c.tbase * kQ, kQ, cnt);
         wave_sync();
     }
     double s = __shfl(acc, 0, TILE);
@@ -421,6 +436,108 @@ __device__ __forceinline__ double quality_at(const Tet& T)
     return e;
 }
 
+// one-ring surface face k with v at pf: (pf, a, b) as the reference forms it
+__device__ __forceinline__ void ring_face(const SmoothArgs& A, int64_t k, V3<double> pf,
+                                          double t9[9])
+{
+    const V3<double> qa = vtx(A, A.tri_ab[2 * k]), qb = vtx(A, A.tri_ab[2 * k + 1]);
+    t9[0] = pf.x; t9[1] = pf.y; t9[2] = pf.z;
+    t9[3] = qa.x; t9[4] = qa.y; t9[5] = qa.z;
+    t9[6] = qb.x; t9[7] = qb.y; t9[8] = qb.z;
+}
+
+// smoothSurface's envelope test of the one-ring surface faces
+// (src/tetwild/VertexSmoother.cpp:413-429): true when some face that is not
+// collinear has a sample with no facet within eps (isFaceOutEnvelop for each
+// face).  The samples of all faces are spread over the tile, TILE faces at a
+// time: lane k counts the samples of face k (SampleGen::skip), a scan over
+// the tile gives the faces' offsets, and lane r walks samples [r K, r K + K)
+// of the concatenated sequence (K = ceil(total / TILE)), moving on to the
+// next face with samples when one ends and carrying its facet hint; a ballot
+// after each step stops the tile at the first sample outside.  The decision
+// does not depend on which lane tests which sample.
+template <int TILE>
+__device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<double> pf,
+                               int64_t t0, int64_t t1)
+{
+    const BvhWalk W = walk(A.sf);
+    int hint = -1;
+    Stats st;
+    for (int64_t base = t0; base < t1; base += TILE) {
+        const int nfc = t1 - base < TILE ? (int)(t1 - base) : TILE;
+        int64_t cnt = 0;
+        bool huge = false;
+        if (c.tl < nfc) {
+            double t9[9];
+            ring_face(A, base + c.tl, pf, t9);
+            if (!collinear3(t9, t9 + 3, t9 + 6)) {
+                const V3<double> a = ld3(t9), b = ld3(t9 + 3), cc = ld3(t9 + 6);
+                const double l = fmax(len2(a - b), fmax(len2(b - cc), len2(cc - a)));
+                if (!(__builtin_sqrt(l) / A.d <= kMaxEdgeSamples)) {
+                    huge = true;  // reported out without sampling
+                } else {
+                    SampleGen g;
+                    g.init(a, b, cc, A.d);
+                    cnt = g.skip(INT64_MAX);
+                }
+            }
+        }
+        if (tile_any(c, huge)) return true;
+        // faces of this chunk with samples (bit k: face base + k)
+        const unsigned long long tm = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
+        const unsigned long long has = (__ballot(cnt > 0) >> c.wbase) & tm;
+        if (has == 0ull) continue;
+        // inclusive scan of the counts over the tile
+        int64_t incl = cnt;
+        for (int off = 1; off < TILE; off <<= 1) {
+            const int64_t o = __shfl_up(incl, off, TILE);
+            if (c.tl >= off) incl += o;
+        }
+        const int64_t total = __shfl(incl, TILE - 1, TILE);
+        const int64_t K = (total + TILE - 1) / TILE;
+        const int64_t first = (int64_t)c.tl * K;
+        // the face holding sample `first` and that face's offset
+        int f = -1;
+        int64_t fstart = 0;
+        for (int k = 0; k < nfc; ++k) {
+            const int64_t ik = __shfl(incl, k, TILE), ck = __shfl(cnt, k, TILE);
+            if (f < 0 && first < ik) {
+                f = k;
+                fstart = ik - ck;
+            }
+        }
+        SampleGen g;
+        if (f >= 0) {
+            double t9[9];
+            ring_face(A, base + f, pf, t9);
+            g.init(ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.d);
+            g.skip(first - fstart);
+        }
+        for (int64_t j = 0; j < K; ++j) {
+            bool out = false;
+            if (f >= 0 && first + j < total) {
+                V3<double> smp;
+                bool got = g.next(smp);
+                while (!got) {
+                    // this face is done: the next face of the chunk with samples
+                    // (one exists, since first + j < total and the counts are
+                    // the generator's own)
+                    const unsigned long long later = f >= 63 ? 0ull : has & ~((2ull << f) - 1ull);
+                    if (later == 0ull) break;
+                    f = __ffsll((long long)later) - 1;
+                    double t9[9];
+                    ring_face(A, base + f, pf, t9);
+                    g.init(ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.d);
+                    got = g.next(smp);
+                }
+                if (got) out = !in_envelope<false, false>(W, smp, A.eps2, A.T, A.sqrtT, hint, st);
+            }
+            if (tile_any(c, out)) return true;
+        }
+    }
+    return false;
+}
+
 template <int TILE, bool MIXED>
 __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64_t p)
 {
@@ -492,17 +609,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
                 if (!(nm < om)) {
                     st = TWG_SMOOTH_REJECT_QUALITY;
                 } else {
-                    bool out = false;
-                    Stats sts;
-                    for (int64_t k = t0 + c.tl; k < t1 && !out; k += TILE) {
-                        double t9[9];
-                        t9[0] = pf.x; t9[1] = pf.y; t9[2] = pf.z;
-                        V3<double> qa = vtx(A, A.tri_ab[2 * k]), qb = vtx(A, A.tri_ab[2 * k + 1]);
-                        t9[3] = qa.x; t9[4] = qa.y; t9[5] = qa.z;
-                        t9[6] = qb.x; t9[7] = qb.y; t9[8] = qb.z;
-                        out = face_out<false, false>(walk(A.sf), t9, A.eps2, A.d, A.T, A.sqrtT, sts);
-                    }
-                    if (tile_any(c, out)) {
+                    if (ring_faces_out<TILE>(A, c, pf, t0, t1)) {
                         st = TWG_SMOOTH_REJECT_ENVELOPE;
                     } else {
                         pfin = pf;
@@ -525,7 +632,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
 }
 
 template <int TILE, bool MIXED>
-__global__ __launch_bounds__(kBlock) void smooth_kernel(SmoothArgs A)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void smooth_kernel(SmoothArgs A)
 {
     __shared__ double red[kBlock * kQ];
     TileCtx<TILE> c;
