@@ -137,11 +137,13 @@ __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double
                                                          double* __restrict__ d2out,
                                                          int32_t* __restrict__ fout)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    __shared__ __align__(16) unsigned char cache[kPointCacheBytes];
+    const BvhWalk W = cache_tree_top<false, kPointCacheBytes>(B, cache);
+    const int64_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     V3<double> np;
     double sq;
-    int best = nearest_facet(B, ld3(p3 + 3 * i), np, sq);
+    int best = nearest_facet(W, ld3(p3 + 3 * i), np, sq);
     cp3[3 * i + 0] = np.x;
     cp3[3 * i + 1] = np.y;
     cp3[3 * i + 2] = np.z;

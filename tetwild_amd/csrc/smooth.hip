@@ -32,6 +32,9 @@ namespace twg {
 namespace {
 
 constexpr int kQ = 10;  // E, J(3), H(6 unique) per tet
+// LDS copy of the surface tree's top levels (FP64 nodes: 8 levels), read by
+// the projection and by the envelope test of the one-ring faces
+constexpr int kSmoothTreeCacheBytes = 16384;
 
 struct SmoothArgs {
     const double* __restrict__ x;
@@ -71,6 +74,7 @@ struct TileCtx {
     int tbase;      // first thread of the tile in the block
     int wbase;      // first lane of the tile in the wave
     double* red;    // block LDS, kQ doubles per thread
+    BvhWalk sfw;    // the surface tree, its top levels in LDS
 };
 
 // acc + v[0] + v[1] + ... + v[cnt - 1] in that order (v[r] = base[r * stride]
@@ -460,7 +464,7 @@ template <int TILE>
 __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<double> pf,
                                int64_t t0, int64_t t1)
 {
-    const BvhWalk W = walk(A.sf);
+    const BvhWalk& W = c.sfw;
     int hint = -1;
     Stats st;
     for (int64_t base = t0; base < t1; base += TILE) {
@@ -591,7 +595,8 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
         } else {
             V3<double> pf;
             double sq;
-            nearest_facet(kind == 2 ? A.bt : A.sf, pn, pf, sq);
+            if (kind == 2) nearest_facet(walk(A.bt), pn, pf, sq);
+            else nearest_facet(c.sfw, pn, pf, sq);
             if (ring_flip<TILE>(A, c, rt, m, v, pf)) {
                 st = TWG_SMOOTH_REJECT_FLIP;
             } else {
@@ -635,7 +640,10 @@ template <int TILE, bool MIXED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void smooth_kernel(SmoothArgs A)
 {
     __shared__ double red[kBlock * kQ];
+    __shared__ __align__(16) unsigned char tcache[kSmoothTreeCacheBytes];
     TileCtx<TILE> c;
+    // every thread of the block, before the loop (the copy synchronises)
+    c.sfw = A.has_sf ? cache_tree_top<false, kSmoothTreeCacheBytes>(A.sf, tcache) : walk(A.sf);
     c.tl = threadIdx.x % TILE;
     c.tbase = threadIdx.x - c.tl;
     c.wbase = (threadIdx.x % 64) - c.tl;

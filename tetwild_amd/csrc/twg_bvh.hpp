@@ -299,16 +299,22 @@ __device__ bool in_envelope(const BvhWalk& B, V3<double> p, double eps2, double 
 // below the best so far.  Stackless: one bit per level marks a far child still
 // to visit; its box distance is recomputed on the way back (the same value the
 // recursion keeps on its stack), so the walk needs no per-lane memory.
-__device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<double>& np,
+// node k of the FP64 tree, from the walker's LDS copy when it holds it
+__device__ __forceinline__ BvhNode64 node64(const BvhWalk& B, int64_t k)
+{
+    if (k < B.ncache) return from_lds(B.c64 + k);
+    return B.n64[k];
+}
+
+__device__ __forceinline__ int nearest_facet(const BvhWalk& B, V3<double> p, V3<double>& np,
                                              double& sq)
 {
     int64_t nd = 1;
-    while (B.n64[nd].leaf < 0) {
-        const BvhNode64& L = B.n64[2 * nd];
-        const BvhNode64& R = B.n64[2 * nd + 1];
+    while (node64(B, nd).leaf < 0) {
+        const BvhNode64 L = node64(B, 2 * nd), R = node64(B, 2 * nd + 1);
         nd = box_center_d2(p, L.lo, L.hi) < box_center_d2(p, R.lo, R.hi) ? 2 * nd : 2 * nd + 1;
     }
-    int best = B.n64[nd].leaf;
+    int best = node64(B, nd).leaf;
     np = ld3(B.tri + 9 * (int64_t)best);
     sq = len2(np - p);
     auto visit = [&](int leaf) {
@@ -321,9 +327,12 @@ __device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<
             sq = dd;
         }
     };
-    if (B.n64[1].leaf >= 0) {  // a one-facet tree: the root is the leaf
-        visit(B.n64[1].leaf);
-        return best;
+    {
+        const int root_leaf = node64(B, 1).leaf;
+        if (root_leaf >= 0) {  // a one-facet tree: the root is the leaf
+            visit(root_leaf);
+            return best;
+        }
     }
     int64_t n = 1;
     int depth = 0;
@@ -331,15 +340,17 @@ __device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<
     for (;;) {
         // n is internal: the nearer child first
         const int64_t c = 2 * n;
-        const double dl = box_signed_d2(p, B.n64[c].lo, B.n64[c].hi);
-        const double dr = box_signed_d2(p, B.n64[c + 1].lo, B.n64[c + 1].hi);
+        BvhNode64 L, R;
+        load_pair(B, c, L, R);
+        const double dl = box_signed_d2(p, L.lo, L.hi);
+        const double dr = box_signed_d2(p, R.lo, R.hi);
         const bool left = dl < dr;
         // the far child is never nearer, so a pruned near child prunes both
         if ((left ? dl : dr) < sq) {
             trail |= 1u << (depth + 1);
             n = left ? c : c + 1;
             depth++;
-            const int leaf = B.n64[n].leaf;
+            const int leaf = left ? L.leaf : R.leaf;
             if (leaf < 0) continue;
             visit(leaf);
         }
@@ -351,8 +362,9 @@ __device__ __forceinline__ int nearest_facet(const BvhView& B, V3<double> p, V3<
             const int64_t far = (n >> (depth - d)) ^ 1;
             n = far;
             depth = d;
-            if (!(box_signed_d2(p, B.n64[far].lo, B.n64[far].hi) < sq)) continue;
-            const int leaf = B.n64[far].leaf;
+            const BvhNode64 F = node64(B, far);
+            if (!(box_signed_d2(p, F.lo, F.hi) < sq)) continue;
+            const int leaf = F.leaf;
             if (leaf < 0) break;
             visit(leaf);
         }
