@@ -46,6 +46,9 @@ __device__ __forceinline__ bool load(const GroupArgs& A, int4 t, Tet& T)
     return true;
 }
 
+// groups of at most this many tets are evaluated in one sweep
+constexpr int kSweepMax = 6;
+
 // EXACT: every orientation decided exactly; otherwise undecided groups are
 // marked kGroupPending
 template <bool EXACT>
@@ -55,6 +58,53 @@ __device__ void eval_group(const GroupArgs& A, int64_t g)
     if (b < 0 || e < b || e > A.n_tets) {
         A.flipped[g] = TWG_GROUP_BAD_INPUT;
         A.max_q[g] = __builtin_nan("");
+        return;
+    }
+    if (e - b <= kSweepMax) {
+        // one sweep: each tet's vertices are loaded once for its orientation
+        // and its quality; the qualities are stored only when the group
+        // turns out not flipped (same outputs as the two sweeps below)
+        const int cnt = (int)(e - b);
+        int4 tv[kSweepMax];
+#pragma unroll
+        for (int k = 0; k < kSweepMax; ++k) tv[k] = k < cnt ? A.tets[b + k] : make_int4(0, 0, 0, 0);
+        bool flip = false, undecided = false, bad = false;
+        double qv[kSweepMax];
+        double m = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSweepMax; ++k) {
+            qv[k] = 0.0;
+            if (k >= cnt || flip || bad) continue;
+            Tet T;
+            if (!load(A, tv[k], T)) {
+                bad = true;
+                continue;
+            }
+            const int o = EXACT ? orient3d(T.p[0], T.p[1], T.p[2], T.p[3])
+                                : orient3d_filter(T.p[0], T.p[1], T.p[2], T.p[3]);
+            if (o == kUndecided) undecided = true;
+            else flip = o != 1;
+            double q = amips_energy(T);
+            if (is_inf(q) || is_nan(q) || q <= 0) q = kMaxEnergy;
+            qv[k] = q;
+            m = q > m ? q : m;
+        }
+        if (bad) {
+            A.flipped[g] = TWG_GROUP_BAD_INPUT;
+            A.max_q[g] = __builtin_nan("");
+        } else if (flip) {
+            A.flipped[g] = 1;
+            A.max_q[g] = __builtin_nan("");
+        } else if (!EXACT && undecided) {
+            A.flipped[g] = kGroupPending;
+        } else {
+            if (A.q)
+#pragma unroll
+                for (int k = 0; k < kSweepMax; ++k)
+                    if (k < cnt) A.q[b + k] = qv[k];
+            A.flipped[g] = 0;
+            A.max_q[g] = m;
+        }
         return;
     }
     // isFlip (src/tetwild/LocalOperations.cpp:555-573): any tet not POSITIVE
