@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
+from tetwild_amd import _capi as C
 from tetwild_amd import workloads as W
 from tetwild_amd.ops import SmoothBatch
 
@@ -154,3 +155,7 @@ def test_autotune_and_empty(ctx):
     E0["ring_off"] = E["ring_off"][:1]
     E0["tri_off"] = E["tri_off"][:1]
     SmoothBatch(ctx, E0).run()
+    # a NULL descriptor is an argument error, not a crash
+    L = C.lib()
+    assert L.twg_smooth_batch(ctx.h, None, 0, 0, 0) == C.TWG_EINVAL
+    assert L.twg_smooth_autotune(ctx.h, None, 0, None, None, None) == C.TWG_EINVAL

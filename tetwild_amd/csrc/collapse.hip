@@ -579,16 +579,18 @@ int enqueue(twg_ctx* ctx, const CollapseArgs& A, int tile, bool f32, hipStream_t
 }
 
 // autotuner cache: (device, f32, size class) -> (tile, chunk)
-std::map<std::tuple<int, bool, int>, std::pair<int, unsigned>> g_tuned;
+TuneCache g_tuned;
 
 int size_class(int64_t n) { return n < 4096 ? 0 : (n < 65536 ? 1 : 2); }
 
 void pick(const twg_ctx* ctx, const CollapseArgs& A, bool f32, int tile, int chunk, int& t,
           unsigned& ch)
 {
-    auto it = g_tuned.find(std::make_tuple(ctx->device, f32, size_class(A.n)));
-    t = tile ? tile : (it != g_tuned.end() ? it->second.first : default_tile(A));
-    ch = chunk ? (unsigned)chunk : (it != g_tuned.end() ? it->second.second : 1u);
+    int tt = default_tile(A);
+    unsigned tc = 1u;
+    g_tuned.get(ctx->device, f32, size_class(A.n), tt, tc);
+    t = tile ? tile : tt;
+    ch = chunk ? (unsigned)chunk : tc;
 }
 
 int check_launch_args(int precision, int tile, int chunk, const char* fn)
@@ -677,7 +679,7 @@ int twg_collapse_autotune(twg_ctx* ctx, const twg_collapse_desc* d, int precisio
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        g_tuned[std::make_tuple(ctx->device, f32, size_class(A.n))] = {bt, bc};
+        g_tuned.put(ctx->device, f32, size_class(A.n), bt, bc);
     }
     if (tile_out) *tile_out = bt;
     if (chunk_out) *chunk_out = (int)bc;

@@ -739,7 +739,7 @@ int enqueue(twg_ctx* ctx, const SmoothArgs& A, int tile, bool mixed, hipStream_t
 }
 
 // autotuner cache: (device, mixed, size class) -> (tile, chunk)
-std::map<std::tuple<int, bool, int>, std::pair<int, unsigned>> g_tuned;
+TuneCache g_tuned;
 
 int size_class(int64_t n) { return n < 4096 ? 0 : (n < 65536 ? 1 : 2); }
 
@@ -799,7 +799,7 @@ int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* d, int precision, i
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_tuned[std::make_tuple(ctx->device, precision == TWG_MIXED, size_class(d->n))] = {bt, bc};
+    g_tuned.put(ctx->device, precision == TWG_MIXED, size_class(d->n), bt, bc);
     if (tile_out) *tile_out = bt;
     if (chunk_out) *chunk_out = (int)bc;
     if (ms_out) *ms_out = best;
@@ -812,12 +812,15 @@ int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int 
     TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED, "twg_smooth_batch: bad precision");
     TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64, "twg_smooth_batch: tile must be 0, 16, 32 or 64");
     TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_batch: chunk out of range");
-    if (d && d->n == 0) return TWG_OK;
+    TWG_REQUIRE(d != nullptr, "twg_smooth_batch: NULL descriptor");
+    if (d->n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
     if (int r = ensure_scratch(ctx, 256)) return r;
-    auto it = g_tuned.find(std::make_tuple(ctx->device, precision == TWG_MIXED, size_class(d ? d->n : 0)));
-    int t = tile ? tile : (it != g_tuned.end() ? it->second.first : 32);
-    unsigned ch = chunk ? (unsigned)chunk : (it != g_tuned.end() ? it->second.second : 1u);
+    int tt = 32;
+    unsigned tc = 1u;
+    g_tuned.get(ctx->device, precision == TWG_MIXED, size_class(d->n), tt, tc);
+    int t = tile ? tile : tt;
+    unsigned ch = chunk ? (unsigned)chunk : tc;
     SmoothArgs A;
     if (int r = make_args(ctx, d, (unsigned*)ctx->scratch, ch, A)) return r;
     return enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur);
@@ -834,9 +837,11 @@ int twg_smooth_plan_create(twg_ctx* ctx, const twg_smooth_desc* d, int precision
     twg_smooth_plan* P = new twg_smooth_plan();
     P->ctx = ctx;
     P->mixed = precision == TWG_MIXED;
-    auto it = g_tuned.find(std::make_tuple(ctx->device, P->mixed, size_class(d->n)));
-    P->tile = tile ? tile : (it != g_tuned.end() ? it->second.first : 32);
-    P->chunk = chunk ? (unsigned)chunk : (it != g_tuned.end() ? it->second.second : 1u);
+    int tt = 32;
+    unsigned tc = 1u;
+    g_tuned.get(ctx->device, P->mixed, size_class(d->n), tt, tc);
+    P->tile = tile ? tile : tt;
+    P->chunk = chunk ? (unsigned)chunk : tc;
     hipError_t e = hipMalloc(&P->counter, 256);
     if (e != hipSuccess) {
         delete P;

@@ -6,7 +6,10 @@
 #include <stdint.h>
 
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 
 #include "../../include/tetwild_gpu.h"
 
@@ -45,6 +48,31 @@ int morton_order(twg_ctx* ctx, const double* items, int64_t n, int stride, const
 int bind(twg_ctx* ctx);
 
 constexpr int kBlock = 256;
+
+// Autotuner results: (device, precision flag, size class) -> (tile, chunk),
+// shared by every context of the process.  Contexts may live on different
+// host threads, so the map is guarded.
+class TuneCache {
+public:
+    bool get(int device, bool flag, int size_class, int& tile, unsigned& chunk)
+    {
+        std::lock_guard<std::mutex> lock(m_);
+        auto it = map_.find(std::make_tuple(device, flag, size_class));
+        if (it == map_.end()) return false;
+        tile = it->second.first;
+        chunk = it->second.second;
+        return true;
+    }
+    void put(int device, bool flag, int size_class, int tile, unsigned chunk)
+    {
+        std::lock_guard<std::mutex> lock(m_);
+        map_[std::make_tuple(device, flag, size_class)] = {tile, chunk};
+    }
+
+private:
+    std::mutex m_;
+    std::map<std::tuple<int, bool, int>, std::pair<int, unsigned>> map_;
+};
 
 inline unsigned grid_for(int64_t n, int block = kBlock)
 {
