@@ -196,13 +196,42 @@ __device__ __forceinline__ float f32_threshold(V3<double> p, double sqrtT)
     return tf;
 }
 
-template <bool F32, bool STATS>
+// Levels above the hint facet's leaf whose subtree is searched first when
+// the hint itself misses: consecutive samples of a triangle usually move to a
+// neighbouring facet, which lies in that subtree.
+constexpr int kHintLevels = 5;
+
+// the implicit tree's node holding facet position b: the median splits of
+// the tree's build (b + (e - b) / 2), integer work only
+__device__ __forceinline__ int64_t leaf_node(int64_t nf, int64_t b)
+{
+    int64_t n = 1, lo = 0, hi = nf;
+    while (hi - lo > 1) {
+        const int64_t m = lo + (hi - lo) / 2;
+        if (b < m) {
+            n = 2 * n;
+            hi = m;
+        } else {
+            n = 2 * n + 1;
+            lo = m;
+        }
+    }
+    return n;
+}
+
+// SUBTREE: after a hint miss, search the hint's ancestor subtree first
+template <bool F32, bool STATS, bool SUBTREE = true>
 __device__ bool in_envelope(const BvhWalk& B, V3<double> p, double eps2, double T, double sqrtT,
                             int& hint, Stats& st)
 {
+    int64_t first = 1;  // the subtree searched first
     if (hint >= 0) {
         if (STATS) st.leaves++;
         if (facet_d2(B, hint, p) <= eps2) return true;
+        if (SUBTREE) {
+            const int64_t a = leaf_node(B.nf, hint) >> kHintLevels;
+            first = a > 1 ? a : 1;
+        }
     }
     if (STATS) st.traversals++;
     if (B.nf == 1) {
@@ -213,81 +242,107 @@ __device__ bool in_envelope(const BvhWalk& B, V3<double> p, double eps2, double 
     }
     V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
     float Tf = F32 ? f32_threshold(p, sqrtT) : 0.f;
-    uint32_t trail = 0;
-    int depth = 0;
-    int64_t n = 1;
-    // the root box
-    bool ok;
-    if (F32) {
-        BvhNode32 r;
-        if (B.ncache > 1) r = from_lds(B.c32 + 1);
-        else r = B.n32[1];
-        ok = box_d2(pf, r.lo, r.hi) <= Tf;
-    } else {
-        BvhNode64 r;
-        if (B.ncache > 1) r = from_lds(B.c64 + 1);
-        else r = B.n64[1];
-        ok = box_d2(p, r.lo, r.hi) <= T;
-    }
-    if (!ok) return false;
-    for (;;) {
-        // n is internal here: look at both children
-        int64_t c = 2 * n;
-        double dl, dr;
-        bool okl, okr;
-        int leafl, leafr;
+    // the subtree of `first`, then (if it held no facet within eps) the whole
+    // tree; the decision -- some facet within eps -- is the whole tree's
+    for (int64_t root = first;; root = 1) {
+        uint32_t trail = 0;
+        int depth = 0;
+        int64_t n = root;
+        bool ok;
+        int root_leaf;
         if (F32) {
-            BvhNode32 L, R;
-            load_pair(B, c, L, R);
-            float fl = box_d2(pf, L.lo, L.hi), fr = box_d2(pf, R.lo, R.hi);
-            okl = fl <= Tf;
-            okr = fr <= Tf;
-            dl = fl;
-            dr = fr;
-            leafl = L.leaf;
-            leafr = R.leaf;
+            BvhNode32 r;
+            if (root < B.ncache) r = from_lds(B.c32 + root);
+            else r = B.n32[root];
+            ok = box_d2(pf, r.lo, r.hi) <= Tf;
+            root_leaf = r.leaf;
         } else {
-            BvhNode64 L, R;
-            load_pair(B, c, L, R);
-            dl = box_d2(p, L.lo, L.hi);
-            dr = box_d2(p, R.lo, R.hi);
-            okl = dl <= T;
-            okr = dr <= T;
-            leafl = L.leaf;
-            leafr = R.leaf;
+            BvhNode64 r;
+            if (root < B.ncache) r = from_lds(B.c64 + root);
+            else r = B.n64[root];
+            ok = box_d2(p, r.lo, r.hi) <= T;
+            root_leaf = r.leaf;
         }
-        if (STATS) st.boxes += 2;
-        bool descend = okl || okr;
-        if (descend) {
-            bool left = okl && (!okr || dl <= dr);
-            if (okl && okr) trail |= 1u << (depth + 1);
-            n = left ? c : c + 1;
-            depth++;
-            int leaf = left ? leafl : leafr;
-            if (leaf < 0) continue;
+        if (STATS) st.boxes++;
+        bool found = false;
+        if (ok && root_leaf >= 0) {
             if (STATS) st.leaves++;
-            if (facet_d2(B, leaf, p) <= eps2) {
-                hint = leaf;
-                return true;
+            if (facet_d2(B, root_leaf, p) <= eps2) {
+                hint = root_leaf;
+                found = true;
+            }
+        } else if (ok) {
+            for (;;) {
+                // n is internal here: look at both children
+                int64_t c = 2 * n;
+                double dl, dr;
+                bool okl, okr;
+                int leafl, leafr;
+                if (F32) {
+                    BvhNode32 L, R;
+                    load_pair(B, c, L, R);
+                    float fl = box_d2(pf, L.lo, L.hi), fr = box_d2(pf, R.lo, R.hi);
+                    okl = fl <= Tf;
+                    okr = fr <= Tf;
+                    dl = fl;
+                    dr = fr;
+                    leafl = L.leaf;
+                    leafr = R.leaf;
+                } else {
+                    BvhNode64 L, R;
+                    load_pair(B, c, L, R);
+                    dl = box_d2(p, L.lo, L.hi);
+                    dr = box_d2(p, R.lo, R.hi);
+                    okl = dl <= T;
+                    okr = dr <= T;
+                    leafl = L.leaf;
+                    leafr = R.leaf;
+                }
+                if (STATS) st.boxes += 2;
+                bool descend = okl || okr;
+                if (descend) {
+                    bool left = okl && (!okr || dl <= dr);
+                    if (okl && okr) trail |= 1u << (depth + 1);
+                    n = left ? c : c + 1;
+                    depth++;
+                    int leaf = left ? leafl : leafr;
+                    if (leaf < 0) continue;
+                    if (STATS) st.leaves++;
+                    if (facet_d2(B, leaf, p) <= eps2) {
+                        hint = leaf;
+                        found = true;
+                        break;
+                    }
+                }
+                // backtrack to the deepest pending far child (depths are
+                // counted from the subtree's root, which has no sibling here)
+                bool done = false;
+                for (;;) {
+                    if (trail == 0) {
+                        done = true;
+                        break;
+                    }
+                    int d = 31 - __clz(trail);
+                    trail &= ~(1u << d);
+                    n = (n >> (depth - d)) ^ 1;
+                    depth = d;
+                    int leaf;
+                    if (F32) leaf = n < B.ncache ? B.c32[n].leaf : B.n32[n].leaf;
+                    else leaf = n < B.ncache ? B.c64[n].leaf : B.n64[n].leaf;
+                    if (leaf < 0) break;
+                    if (STATS) st.leaves++;
+                    if (facet_d2(B, leaf, p) <= eps2) {
+                        hint = leaf;
+                        found = true;
+                        done = true;
+                        break;
+                    }
+                }
+                if (done) break;
             }
         }
-        // backtrack to the deepest pending far child
-        for (;;) {
-            if (trail == 0) return false;
-            int d = 31 - __clz(trail);
-            trail &= ~(1u << d);
-            n = (n >> (depth - d)) ^ 1;
-            depth = d;
-            int leaf;
-            if (F32) leaf = n < B.ncache ? B.c32[n].leaf : B.n32[n].leaf;
-            else leaf = n < B.ncache ? B.c64[n].leaf : B.n64[n].leaf;
-            if (leaf < 0) break;
-            if (STATS) st.leaves++;
-            if (facet_d2(B, leaf, p) <= eps2) {
-                hint = leaf;
-                return true;
-            }
-        }
+        if (found) return true;
+        if (root == 1) return false;
     }
 }
 
@@ -405,7 +460,7 @@ __device__ bool face_out_sampled(const BvhWalk& B, const double* t9, double eps2
 // sample, so it equals face_out_sampled's.  For batches of few triangles
 // with many samples each, where one lane per triangle leaves most of the
 // machine idle.
-template <bool F32, int G>
+template <bool F32, int G, bool SUBTREE = true>
 __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c,
                                double eps2, double d, double T, double sqrtT, int r, int gbase)
 {
@@ -431,7 +486,7 @@ __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<
     for (int64_t j = 0; j < K; ++j) {
         V3<double> q = a;
         const bool have = first + j < total && g.next(q);
-        const bool out = have && !in_envelope<F32, false>(B, q, eps2, T, sqrtT, hint, st);
+        const bool out = have && !in_envelope<F32, false, SUBTREE>(B, q, eps2, T, sqrtT, hint, st);
         if ((__ballot(out) >> gbase) & gmask) return true;
     }
     return false;
