@@ -198,8 +198,10 @@ __device__ __forceinline__ float f32_threshold(V3<double> p, double sqrtT)
 
 // Levels above the hint facet's leaf whose subtree is searched first when
 // the hint itself misses: consecutive samples of a triangle usually move to a
-// neighbouring facet, which lies in that subtree.
-constexpr int kHintLevels = 5;
+// nearby facet, which lies in that subtree.  Measured on 20 M query triangles
+// against 1 M facets: 3 / 5 / 7 / 10 / 12 levels gave 500 / 513 / 525 / 542 /
+// 539 M checks/s.
+constexpr int kHintLevels = 10;
 
 // the implicit tree's node holding facet position b: the median splits of
 // the tree's build (b + (e - b) / 2), integer work only
