@@ -7,7 +7,9 @@
 //      trees of 1..50k facets (the tree built by the library's own host code);
 //   3. face_out_group's block split (G lanes emulated one after another, the
 //      ballot as the OR of the lanes' step results) against face_out_sampled,
-//      and that every sample of an inside triangle is tested exactly once.
+//      and that every sample of an inside triangle is tested exactly once;
+//   4. collect_facets (one-lane group) against the facets whose boxes meet
+//      the query box, and in_envelope_list on its lists against brute force.
 // Exit status 0 when every check passes.
 #include "twg_bvh.hpp"
 
@@ -216,12 +218,87 @@ static void check_group(std::mt19937_64& rng)
     printf("group split: 800 triangles, %ld out\n", outs);
 }
 
+static void check_lists(std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(0, 1);
+    std::vector<unsigned char> cache(kTreeCacheBytes);
+    long lists = 0, points = 0, overflow = 0;
+    for (int nf : {2, 3, 7, 100, 1000, 3001, 20001}) {
+        Tree T;
+        build(T, nf, rng, 0.02);
+        const double eps = 0.01, eps2 = eps * eps, Tt = eps2 * (1 + 1e-9), sqrtT = sqrt(Tt);
+        for (int mode = 0; mode < 3; ++mode) {
+            const BvhWalk W = mode == 0 ? walk(T.view)
+                                        : (mode == 1 ? cache_tree_top<false>(T.view, cache.data())
+                                                     : cache_tree_top<true>(T.view, cache.data()));
+            for (int q = 0; q < 100; ++q) {
+                const int cap = q % 5 == 0 ? 16 : 4096;
+                std::vector<BvhNode32> lst(cap);
+                std::vector<uint32_t> fr(2 * cap);
+                const double cx = U(rng), cy = U(rng), s = (q % 3 == 0) ? 0.3 : 0.04 * U(rng);
+                double lo[3] = {cx, cy, 0.05 * sin(7 * cx) * cos(5 * cy) - 0.01};
+                double hi[3] = {cx + s * U(rng), cy + s * U(rng), lo[2] + 0.02 + 0.03 * U(rng)};
+                const double pad = 1.0001 * sqrtT + 1e-9;
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] -= pad;
+                    hi[k] += pad;
+                }
+                const int n = collect_facets<1>(W, lo, hi, lst.data(), fr.data(), cap, 0, 0);
+                // exact: FP64 leaf boxes meeting the box; loose: FP32 leaf boxes
+                std::vector<char> in_list(nf, 0), exact(nf, 0), loose(nf, 0);
+                for (int64_t k = 1; k < T.view.nnode; ++k) {
+                    const int f = T.n64[k].leaf;
+                    if (f < 0) continue;
+                    exact[f] = T.n64[k].lo[0] <= hi[0] && T.n64[k].lo[1] <= hi[1] && T.n64[k].lo[2] <= hi[2] &&
+                               T.n64[k].hi[0] >= lo[0] && T.n64[k].hi[1] >= lo[1] && T.n64[k].hi[2] >= lo[2];
+                    loose[f] = T.n32[k].lo[0] <= hi[0] + 1e-6 && T.n32[k].lo[1] <= hi[1] + 1e-6 &&
+                               T.n32[k].lo[2] <= hi[2] + 1e-6 && T.n32[k].hi[0] >= lo[0] - 1e-6 &&
+                               T.n32[k].hi[1] >= lo[1] - 1e-6 && T.n32[k].hi[2] >= lo[2] - 1e-6;
+                }
+                if (n < 0) {
+                    // only a small capacity overflows (a work list can, with no facet in the box)
+                    CHECK(cap < 4096, "lists: %d facets, mode %d, query %d: -1\n", nf, mode, q);
+                    overflow++;
+                    continue;
+                }
+                bool ok = true;
+                for (int j = 0; j < n; ++j) {
+                    const int f = lst[j].leaf;
+                    ok = ok && f >= 0 && f < nf && !in_list[f] && loose[f];
+                    if (f >= 0 && f < nf) in_list[f] = 1;
+                }
+                for (int f = 0; f < nf; ++f) ok = ok && (!exact[f] || in_list[f]);
+                CHECK(ok, "lists: %d facets, mode %d, query %d: list differs from the box query\n", nf, mode, q);
+                lists++;
+                // points of the unpadded box against brute force
+                std::vector<double> tri(9 * (size_t)std::max(n, 1));
+                for (int j = 0; j < n; ++j)
+                    for (int k = 0; k < 9; ++k) tri[9 * j + k] = T.tri[9 * (size_t)lst[j].leaf + k];
+                int hl = -1;
+                for (int r = 0; r < 16; ++r) {
+                    V3<double> p;
+                    double* pc[3] = {&p.x, &p.y, &p.z};
+                    for (int k = 0; k < 3; ++k) *pc[k] = lo[k] + pad + (hi[k] - lo[k] - 2 * pad) * U(rng);
+                    double best = INFINITY;
+                    for (int f = 0; f < nf; ++f) best = std::min(best, facet_d2(W, f, p));
+                    const bool got = in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hl);
+                    CHECK(got == (best <= eps2), "lists: %d facets, mode %d, query %d, point %d\n", nf, mode, q, r);
+                    if (r % 7 == 0) hl = -1;
+                    points++;
+                }
+            }
+        }
+    }
+    printf("facet lists: %ld lists (%ld overflowed), %ld points against brute force\n", lists, overflow, points);
+}
+
 int main()
 {
     std::mt19937_64 rng(20240601);
     check_sampler(rng);
     check_traversal(rng);
     check_group(rng);
+    check_lists(rng);
     printf("%ld failures\n", g_bad);
     return g_bad != 0;
 }

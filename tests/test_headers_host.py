@@ -1,4 +1,5 @@
-"""The device headers' sampler, tree traversal and lane-group split, built for
+"""The device headers' sampler, tree traversal, lane-group split and facet
+lists, built for
 the host with g++ (GPU qualifiers stripped from copies of the headers, the
 tree built by the library's own host code) and checked against the oracle's
 sampleTriangle and against brute force: tests/host/headers_check.cpp."""
@@ -24,8 +25,12 @@ def host_copy(src, dst):
     # device-only helpers the checks do not use
     s = re.sub(r"inline int64_t xcd_block\(\)\s*\{.*?\n\}", "", s, flags=re.S)
     if os.path.basename(src) == "twg_bvh.hpp":
+        # one-lane groups: a ballot is the lane's own predicate
+        s = s.replace("__builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\");", "")
+        s = s.replace("__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");", "")
+        s = s.replace("__builtin_amdgcn_wave_barrier();", "").replace("__popcll(", "__builtin_popcountll(")
         s = ("struct uint4 { unsigned x, y, z, w; };\n"
-             "inline unsigned long long __ballot(bool) { return 0; }\n") + s
+             "inline unsigned long long __ballot(bool b) { return b ? 1ull : 0ull; }\n") + s
     open(dst, "w").write(s)
 
 

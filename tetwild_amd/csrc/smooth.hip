@@ -61,20 +61,32 @@ struct SmoothArgs {
     unsigned chunk;
 };
 
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// LDS per block, an equal slice for each tile: the tile's Newton reductions
+// (kQ doubles per lane) while it solves a problem, and while it tests the
+// one-ring faces its list of nearby surface facets -- per entry a 32-byte
+// record, two 4-byte work-list slots (collect_facets) and the facet's FP64
+// triangle (72 bytes).  The two uses never overlap in time.
+constexpr int kTilePoolBytes = 57344;
+constexpr int kListEntryBytes = 32 + 8 + 72;
+template <int TILE>
+constexpr int tile_bytes() { return kTilePoolBytes / (kBlock / TILE); }
+template <int TILE>
+constexpr int ring_cap() { return tile_bytes<TILE>() / kListEntryBytes; }
+static_assert(tile_bytes<16>() >= 16 * kQ * 8 && tile_bytes<64>() >= 64 * kQ * 8, "reduction rows");
+static_assert(ring_cap<64>() == 128 && ring_cap<16>() >= 16, "list capacity");
+
+typedef __attribute__((address_space(3))) double LdsF64;
 
 template <int TILE>
 struct TileCtx {
-    int tl;         // lane within the tile
-    int tbase;      // first thread of the tile in the block
-    int wbase;      // first lane of the tile in the wave
-    double* red;    // block LDS, kQ doubles per thread
-    BvhWalk sfw;    // the surface tree, its top levels in LDS
+    int tl;            // lane within the tile
+    int tbase;         // first thread of the tile in the block
+    int wbase;         // first lane of the tile in the wave
+    double* red;       // the tile's slice: kQ doubles per lane of the tile
+    BvhWalk sfw;       // the surface tree, its top levels in LDS
+    LdsNode32W* lst;   // the same slice: facet list (ring_cap entries),
+    LdsU32* fr;        // work lists (2 ring_cap entries)
+    LdsF64* tri;       // and the listed facets' triangles (9 doubles each)
 };
 
 // acc + v[0] + v[1] + ... + v[cnt - 1] in that order (v[r] = base[r * stride]
@@ -161,12 +173,12 @@ __device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const i
             val[4] = h[0]; val[5] = h[1]; val[6] = h[2];
             val[7] = h[4]; val[8] = h[5]; val[9] = h[8];
         }
-        double* mine = c.red + (c.tbase + c.tl) * kQ;
+        double* mine = c.red + c.tl * kQ;
 #pragma unroll
         for (int q = 0; q < kQ; ++q) mine[q] = val[q];
         wave_sync();
         int cnt = m - s < TILE ? m - s : TILE;
-        if (c.tl < kQ) acc = ordered_sum<TILE>(acc, c.red + c.tbase * kQ + c.tl, kQ, cnt);
+        if (c.tl < kQ) acc = ordered_sum<TILE>(acc, c.red + c.tl, kQ, cnt);
         wave_sync();
     }
     E = __shfl(acc, 0, TILE);
@@ -201,10 +213,10 @@ __device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const
             load_ring_tet(A, rt[j], v, X, false, T);
             e = amips_energy(T);
         }
-        c.red[(c.tbase + c.tl) * kQ] = e;
+        c.red[c.tl * kQ] = e;
         wave_sync();
         int cnt = m - s < TILE ? m - s : TILE;
-        if (c.tl == 0) acc = ordered_sum<TILE>(acc, c.red + c.tbase * kQ, kQ, cnt);
+        if (c.tl == 0) acc = ordered_sum<TILE>(acc, c.red, kQ, cnt);
         wave_sync();
     }
     double s = __shfl(acc, 0, TILE);
@@ -270,64 +282,90 @@ __device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const int
     return tile_any(c, f);
 }
 
-// 3x3 Householder QR with column pivoting (see the host restatement)
-__device__ void solve3_qrcp(const double* Ain, const double* b, double* x)
+// 3x3 Householder QR with column pivoting (see the host restatement).
+// Every array index is a compile-time constant after unrolling (pivots and
+// the rank select values instead of indexing), so the arrays stay in
+// registers; the arithmetic and its order are the restatement's.
+__device__ __forceinline__ void solve3_qrcp(const double* Ain, const double* b, double* x)
 {
     double A[3][3];
     int perm[3] = {0, 1, 2};
     double tau[3] = {0, 0, 0};
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j) A[i][j] = Ain[3 * i + j];
     double norm[3];
     double maxn = 0.0;
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
         norm[j] = __builtin_sqrt(A[0][j] * A[0][j] + A[1][j] * A[1][j] + A[2][j] * A[2][j]);
         if (norm[j] > maxn) maxn = norm[j];
     }
     double thr = (maxn * 0x1p-52) * (maxn * 0x1p-52) / 3.0;
     int rank = 3;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         int big = k;
+        double nb = norm[k];
+#pragma unroll
         for (int j = k + 1; j < 3; ++j)
-            if (norm[j] > norm[big]) big = j;
-        if (rank == 3 && norm[big] * norm[big] < thr * (double)(3 - k)) rank = k;
-        if (big != k) {
-            for (int i = 0; i < 3; ++i) {
-                double t = A[i][k];
-                A[i][k] = A[i][big];
-                A[i][big] = t;
+            if (norm[j] > nb) {
+                big = j;
+                nb = norm[j];
             }
-            double t = norm[k];
-            norm[k] = norm[big];
-            norm[big] = t;
-            int p = perm[k];
-            perm[k] = perm[big];
-            perm[big] = p;
+        if (rank == 3 && nb * nb < thr * (double)(3 - k)) rank = k;
+        // swap column k with column big (big >= k)
+#pragma unroll
+        for (int j = k + 1; j < 3; ++j) {
+            if (big == j) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    double t = A[i][k];
+                    A[i][k] = A[i][j];
+                    A[i][j] = t;
+                }
+                double t = norm[k];
+                norm[k] = norm[j];
+                norm[j] = t;
+                int p = perm[k];
+                perm[k] = perm[j];
+                perm[j] = p;
+            }
         }
         double tail = 0.0;
+#pragma unroll
         for (int i = k + 1; i < 3; ++i) tail += A[i][k] * A[i][k];
         double c0 = A[k][k];
         double beta;
         if (tail <= 0x1p-1022) {
             tau[k] = 0.0;
             beta = c0;
+#pragma unroll
             for (int i = k + 1; i < 3; ++i) A[i][k] = 0.0;
         } else {
             beta = __builtin_sqrt(c0 * c0 + tail);
             if (c0 >= 0.0) beta = -beta;
+#pragma unroll
             for (int i = k + 1; i < 3; ++i) A[i][k] = A[i][k] / (c0 - beta);
             tau[k] = (beta - c0) / beta;
         }
         A[k][k] = beta;
-        for (int j = k + 1; j < 3; ++j) {
-            if (tau[k] == 0.0) break;
-            double w = A[k][j];
-            for (int i = k + 1; i < 3; ++i) w += A[i][k] * A[i][j];
-            A[k][j] -= tau[k] * w;
-            for (int i = k + 1; i < 3; ++i) A[i][j] -= tau[k] * A[i][k] * w;
+        if (tau[k] != 0.0) {
+#pragma unroll
+            for (int j = k + 1; j < 3; ++j) {
+                double w = A[k][j];
+#pragma unroll
+                for (int i = k + 1; i < 3; ++i) w += A[i][k] * A[i][j];
+                A[k][j] -= tau[k] * w;
+#pragma unroll
+                for (int i = k + 1; i < 3; ++i) A[i][j] -= tau[k] * A[i][k] * w;
+            }
         }
+#pragma unroll
         for (int j = k + 1; j < 3; ++j) {
             double s = 0.0;
+#pragma unroll
             for (int i = k + 1; i < 3; ++i) s += A[i][j] * A[i][j];
             norm[j] = __builtin_sqrt(s);
         }
@@ -337,20 +375,31 @@ __device__ void solve3_qrcp(const double* Ain, const double* b, double* x)
         return;
     }
     double c[3] = {b[0], b[1], b[2]};
-    for (int k = 0; k < rank; ++k) {
-        if (tau[k] == 0.0) continue;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= rank || tau[k] == 0.0) continue;
         double w = c[k];
+#pragma unroll
         for (int i = k + 1; i < 3; ++i) w += A[i][k] * c[i];
         c[k] -= tau[k] * w;
+#pragma unroll
         for (int i = k + 1; i < 3; ++i) c[i] -= tau[k] * A[i][k] * w;
     }
-    for (int i = rank - 1; i >= 0; --i) {
+#pragma unroll
+    for (int i = 2; i >= 0; --i) {
+        if (i >= rank) continue;
         double s = c[i];
-        for (int j = i + 1; j < rank; ++j) s -= A[i][j] * c[j];
+#pragma unroll
+        for (int j = i + 1; j < 3; ++j)
+            if (j < rank) s -= A[i][j] * c[j];
         c[i] = s / A[i][i];
     }
     double out[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < rank; ++i) out[perm[i]] = c[i];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if (i < rank && perm[i] == t) out[t] = c[i];
     x[0] = out[0];
     x[1] = out[1];
     x[2] = out[2];
@@ -458,15 +507,68 @@ __device__ __forceinline__ void ring_face(const SmoothArgs& A, int64_t k, V3<dou
 // the tile gives the faces' offsets, and lane r walks samples [r K, r K + K)
 // of the concatenated sequence (K = ceil(total / TILE)), moving on to the
 // next face with samples when one ends and carrying its facet hint; a ballot
-// after each step stops the tile at the first sample outside.  The decision
-// does not depend on which lane tests which sample.
+// after each step stops the tile at the first sample outside.  A sample is
+// tested against the tile's LDS list of the surface facets whose boxes meet
+// the faces' padded box (collect_facets; the tree walk when the list does not
+// fit), starting from the facet that held the lane's previous sample or, at
+// first, from `hint` (a facet near pf, or -1).  The decision depends neither
+// on the list order, the hints, nor on which lane tests which sample.
 template <int TILE>
 __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<double> pf,
-                               int64_t t0, int64_t t1)
+                               int64_t t0, int64_t t1, int hint)
 {
     const BvhWalk& W = c.sfw;
-    int hint = -1;
     Stats st;
+    if (t1 <= t0) return false;
+    // (the list lives in the tile's LDS slice, over its reduction rows)
+    // the surface facets that can hold a sample: those whose boxes meet the
+    // faces' box padded by more than sqrt(T) (the samples lie in the faces'
+    // box up to their rounding)
+    int nl = -1, hl = -1;
+    {
+        double lo[3] = {pf.x, pf.y, pf.z}, hi[3] = {pf.x, pf.y, pf.z};
+        for (int64_t k = t0 + c.tl; k < t1; k += TILE) {
+            const V3<double> qa = vtx(A, A.tri_ab[2 * k]), qb = vtx(A, A.tri_ab[2 * k + 1]);
+            const double qs[6] = {qa.x, qa.y, qa.z, qb.x, qb.y, qb.z};
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                lo[j % 3] = qs[j] < lo[j % 3] ? qs[j] : lo[j % 3];
+                hi[j % 3] = qs[j] > hi[j % 3] ? qs[j] : hi[j % 3];
+            }
+        }
+        double amax = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            for (int off = TILE / 2; off > 0; off >>= 1) {
+                const double ol = __shfl_xor(lo[k], off, TILE), oh = __shfl_xor(hi[k], off, TILE);
+                lo[k] = ol < lo[k] ? ol : lo[k];
+                hi[k] = oh > hi[k] ? oh : hi[k];
+            }
+            amax = fmax(amax, fmax(fabs(lo[k]), fabs(hi[k])));
+        }
+        const double pad = 1.0001 * A.sqrtT + 1e-9 * amax;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] -= pad;
+            hi[k] += pad;
+        }
+        wave_sync();  // the slice's reduction rows are no longer read
+        nl = collect_facets<TILE>(W, lo, hi, c.lst, c.fr, ring_cap<TILE>(), c.tl, c.wbase);
+        for (int i = c.tl; i < nl; i += TILE) {
+            const double* t = W.tri + 9 * (int64_t)c.lst[i].leaf;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) c.tri[9 * i + k] = t[k];
+        }
+        wave_sync();
+        if (nl > 0 && hint >= 0) {  // the hint's position (a facet appears once)
+            int pos = -1;
+            for (int i = c.tl; i < nl; i += TILE)
+                if (c.lst[i].leaf == hint) pos = i;
+            const unsigned long long m = __ballot(pos >= 0) >> c.wbase;
+            const unsigned long long tm = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
+            if (m & tm) hl = __shfl(pos, __ffsll((long long)(m & tm)) - 1, TILE);
+        }
+    }
     for (int64_t base = t0; base < t1; base += TILE) {
         const int nfc = t1 - base < TILE ? (int)(t1 - base) : TILE;
         int64_t cnt = 0;
@@ -534,7 +636,9 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                     g.init(ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.d);
                     got = g.next(smp);
                 }
-                if (got) out = !in_envelope<false, false>(W, smp, A.eps2, A.T, A.sqrtT, hint, st);
+                if (got)
+                    out = nl >= 0 ? !in_envelope_list(c.lst, c.tri, nl, smp, A.eps2, A.sqrtT, hl)
+                                  : !in_envelope<false, false>(W, smp, A.eps2, A.T, A.sqrtT, hint, st);
             }
             if (tile_any(c, out)) return true;
         }
@@ -595,8 +699,15 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
         } else {
             V3<double> pf;
             double sq;
-            if (kind == 2) nearest_facet(walk(A.bt), pn, pf, sq);
-            else nearest_facet(c.sfw, pn, pf, sq);
+            // pf lies on the facet it was projected to: when that facet is one of
+            // the envelope's, it starts every lane's facet hint below
+            int hint = -1;
+            if (kind == 2) {
+                const int f = nearest_facet(walk(A.bt), pn, pf, sq);
+                if (A.bt.n64 == A.sf.n64) hint = f;
+            } else {
+                hint = nearest_facet(c.sfw, pn, pf, sq);
+            }
             if (ring_flip<TILE>(A, c, rt, m, v, pf)) {
                 st = TWG_SMOOTH_REJECT_FLIP;
             } else {
@@ -614,7 +725,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
                 if (!(nm < om)) {
                     st = TWG_SMOOTH_REJECT_QUALITY;
                 } else {
-                    if (ring_faces_out<TILE>(A, c, pf, t0, t1)) {
+                    if (ring_faces_out<TILE>(A, c, pf, t0, t1, hint)) {
                         st = TWG_SMOOTH_REJECT_ENVELOPE;
                     } else {
                         pfin = pf;
@@ -639,15 +750,22 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
 template <int TILE, bool MIXED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void smooth_kernel(SmoothArgs A)
 {
-    __shared__ double red[kBlock * kQ];
     __shared__ __align__(16) unsigned char tcache[kSmoothTreeCacheBytes];
+    __shared__ __align__(16) unsigned char pool[kTilePoolBytes];
     TileCtx<TILE> c;
     // every thread of the block, before the loop (the copy synchronises)
     c.sfw = A.has_sf ? cache_tree_top<false, kSmoothTreeCacheBytes>(A.sf, tcache) : walk(A.sf);
     c.tl = threadIdx.x % TILE;
     c.tbase = threadIdx.x - c.tl;
     c.wbase = (threadIdx.x % 64) - c.tl;
-    c.red = red;
+    {
+        constexpr int cap = ring_cap<TILE>();
+        unsigned char* slice = pool + (threadIdx.x / TILE) * tile_bytes<TILE>();
+        c.red = (double*)slice;
+        c.lst = (LdsNode32W*)slice;
+        c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
+        c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
+    }
     for (;;) {
         unsigned start = 0;
         if (c.tl == 0) start = atomicAdd(A.counter, A.chunk);

@@ -349,6 +349,206 @@ __device__ bool in_envelope(const BvhWalk& B, V3<double> p, double eps2, double 
 }
 
 
+// ---- the facets near a small region, in LDS --------------------------------
+// writable LDS records
+typedef __attribute__((address_space(3))) BvhNode32 LdsNode32W;
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+
+// the wave's LDS writes so far, visible to its other lanes
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// x rounded to FP32 downward / upward
+__device__ __forceinline__ float f32_down(double x)
+{
+    float f = (float)x;
+    if ((double)f > x) f = f > 0.f ? u2f32(f2u(f) - 1u) : (f < 0.f ? u2f32(f2u(f) + 1u) : -0x1p-149f);
+    return f;
+}
+__device__ __forceinline__ float f32_up(double x) { return -f32_down(-x); }
+
+template <typename R, typename S>
+__device__ __forceinline__ bool boxes_meet(const R* alo, const R* ahi, const S* blo, const S* bhi)
+{
+    return alo[0] <= bhi[0] && alo[1] <= bhi[1] && alo[2] <= bhi[2] && ahi[0] >= blo[0] &&
+           ahi[1] >= blo[1] && ahi[2] >= blo[2];
+}
+
+__device__ __forceinline__ void put_lds(LdsNode32W* d, const BvhNode32& s)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d->lo[k] = s.lo[k];
+        d->hi[k] = s.hi[k];
+    }
+    d->leaf = s.leaf;
+    d->pad = 0;
+}
+
+// The facets whose boxes meet the box [lo, hi], found by the G lanes of a
+// group (G a power of two up to 64, lane gl, the group's first lane gbase in
+// the wave) expanding a work list of internal nodes that meet the box (fr:
+// two buffers of cap entries).  A lane expands one node: its children from
+// the walker's FP64 LDS copy when they are there, else its children and
+// grandchildren from the FP32 records in global memory, read together (one
+// memory latency for two levels; a grandchild's box lies inside its
+// parent's, and FP32 boxes are rounded outward, so no facet is lost).
+// Ballots place what meets the box -- leaves into lst as FP32 records,
+// internal nodes into the next work list.  Returns the number of facets in
+// lst, or -1 (nothing usable in lst) when more than cap facets, or more than
+// cap nodes in one work list, meet the box, when the box is not finite, or
+// for a one-facet tree.  Every lane of the group calls it.
+template <int G>
+__device__ int collect_facets(const BvhWalk& W, const double lo[3], const double hi[3],
+                              LdsNode32W* lst, LdsU32* fr, int cap, int gl, int gbase)
+{
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) finite = finite && is_finite(lo[k]) && is_finite(hi[k]);
+    if (!finite || W.nf < 2) return -1;
+    float lof[3], hif[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lof[k] = f32_down(lo[k]);
+        hif[k] = f32_up(hi[k]);
+    }
+    const unsigned long long gm = (G == 64 ? ~0ull : ((1ull << (G & 63)) - 1ull)) << gbase;
+    const unsigned long long below = ((1ull << (gbase + gl)) - 1ull) & gm;
+    LdsU32* cur = fr;
+    LdsU32* nxt = fr + cap;
+    if (gl == 0) cur[0] = 1u;
+    wave_sync();
+    int ncur = 1, nl = 0;
+    while (ncur > 0) {
+        int nn = 0;
+        for (int i0 = 0; i0 < ncur; i0 += G) {
+            const int i = i0 + gl;
+            // up to four candidates: slot k holds node id[k] (0: none)
+            BvhNode32 cand[4];
+            int64_t id[4] = {0, 0, 0, 0};
+            bool ok[4] = {false, false, false, false};
+            if (i < ncur) {
+                const int64_t ch = 2 * (int64_t)cur[i];
+                if (ch < W.ncache && W.c64) {
+                    const BvhNode64 L64 = from_lds(W.c64 + ch), R64 = from_lds(W.c64 + ch + 1);
+                    ok[0] = boxes_meet(L64.lo, L64.hi, lo, hi);
+                    ok[1] = boxes_meet(R64.lo, R64.hi, lo, hi);
+                    id[0] = ch;
+                    id[1] = ch + 1;
+                    cand[0].leaf = L64.leaf;
+                    cand[1].leaf = R64.leaf;
+                    if (ok[0] && L64.leaf >= 0) cand[0] = W.n32[ch];
+                    if (ok[1] && R64.leaf >= 0) cand[1] = W.n32[ch + 1];
+                } else {
+                    BvhNode32 C[2], GC[4];
+                    if (ch < W.ncache) {
+                        C[0] = from_lds(W.c32 + ch);
+                        C[1] = from_lds(W.c32 + ch + 1);
+                    } else {
+                        C[0] = W.n32[ch];
+                        C[1] = W.n32[ch + 1];
+                    }
+                    // grandchildren slots exist for an internal child (read
+                    // before the child is known to be internal; used only then)
+                    const int64_t g0 = 2 * ch;
+                    if (g0 + 1 < W.nnode) {
+                        GC[0] = W.n32[g0];
+                        GC[1] = W.n32[g0 + 1];
+                    }
+                    if (g0 + 3 < W.nnode) {
+                        GC[2] = W.n32[g0 + 2];
+                        GC[3] = W.n32[g0 + 3];
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (!boxes_meet(C[h].lo, C[h].hi, lof, hif)) continue;
+                        if (C[h].leaf >= 0) {
+                            cand[2 * h] = C[h];
+                            id[2 * h] = ch + h;
+                            ok[2 * h] = true;
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) {
+                                cand[2 * h + e] = GC[2 * h + e];
+                                id[2 * h + e] = 2 * (ch + h) + e;
+                                ok[2 * h + e] = boxes_meet(GC[2 * h + e].lo, GC[2 * h + e].hi, lof, hif);
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool leaf = ok[k] && cand[k].leaf >= 0, inner = ok[k] && cand[k].leaf < 0;
+                const unsigned long long ml = __ballot(leaf) & gm, mi = __ballot(inner) & gm;
+                const int pl = nl + __popcll(ml & below), qi = nn + __popcll(mi & below);
+                if (leaf && pl < cap) put_lds(lst + pl, cand[k]);
+                if (inner && qi < cap) nxt[qi] = (uint32_t)id[k];
+                nl += __popcll(ml);
+                nn += __popcll(mi);
+            }
+        }
+        if (nl > cap || nn > cap) return -1;
+        wave_sync();
+        LdsU32* t = cur;
+        cur = nxt;
+        nxt = t;
+        ncur = nn;
+    }
+    return nl;
+}
+
+typedef __attribute__((address_space(3))) const double LdsF64c;
+
+// facet_d2 of a triangle held in LDS (9 doubles): the same computation on
+// the same values
+__device__ __forceinline__ double tri_d2(LdsF64c* t, V3<double> p)
+{
+    V3<double> cp;
+    return point_tri_d2<double, false>(p, mk3(t[0], t[1], t[2]), mk3(t[3], t[4], t[5]),
+                                       mk3(t[6], t[7], t[8]), cp);
+}
+
+// in_envelope's decision for a point p of the box that a collect_facets list
+// was made for, with that box padded by at least sqrt(T) beyond every point
+// tested (then the list holds every facet the tree walk could accept): the
+// facet at list position hl first (the one that held the previous point, -1:
+// none), then the others, 64 positions at a time: the FP32 box tests of the
+// window first, into a bit mask, then the triangles of the boxes that pass.
+// (Box tests and triangle distances interleaved would make a wave pay a
+// triangle distance at every position where some lane's box passes.)  tri
+// holds the listed facets' triangles, 9 doubles per position.  On a hit hl
+// becomes the facet's position.
+__device__ __forceinline__ bool in_envelope_list(LdsNode32W* lst, LdsF64c* tri, int nl,
+                                                 V3<double> p, double eps2, double sqrtT, int& hl)
+{
+    if (hl >= 0 && tri_d2(tri + 9 * hl, p) <= eps2) return true;
+    const V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
+    const float Tf = f32_threshold(p, sqrtT);
+    for (int base = 0; base < nl; base += 64) {
+        const int n = nl - base < 64 ? nl - base : 64;
+        uint64_t m = 0;
+        for (int k = 0; k < n; ++k) {
+            const BvhNode32 e = from_lds(lst + base + k);
+            m |= (uint64_t)(box_d2(pf, e.lo, e.hi) <= Tf) << k;
+        }
+        if (hl >= base && hl < base + n) m &= ~(1ull << (hl - base));
+        while (m) {
+            const int i = base + __builtin_ctzll(m);
+            m &= m - 1;
+            if (tri_d2(tri + 9 * i, p) <= eps2) {
+                hl = i;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
 // nearest_facet with the reference's traversal order
 // (src/tetwild/geogram/mesh_AABB.h:130-141, mesh_AABB.cpp:381-480):
 // get_nearest_facet_hint, then depth-first search visiting the nearer child

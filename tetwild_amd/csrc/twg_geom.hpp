@@ -194,14 +194,21 @@ __device__ __forceinline__ V3<double> sel3(int i, V3<double> x0, V3<double> x1, 
 // then the two other edges), so a caller has a single consumption site.
 struct SampleGen {
     // State kept between samples: the rotated triangle, the spacing, the
-    // current edge's sample count, the current row's start and the counters.
-    // The unit directions and the angles of the rows are recomputed where they
-    // are used, with the same expressions (so every sample is bit-identical):
-    // kept, they would hold ~40 more registers across the envelope traversal
-    // that runs between two samples.
-    V3<double> v0, v1, v2, rowv;
+    // current edge's sample count, the current row's start, the unit
+    // direction that the current edge or row steps along, and the counters.
+    // The angles of the rows are recomputed for each row header, with the same
+    // expressions (so every sample is bit-identical): kept, they would hold
+    // ~30 more registers across the envelope traversal that runs between two
+    // samples.  skip(), which runs no traversal, computes them once per call.
+    V3<double> v0, v1, v2, rowv, dir;
     double d, N;
     int phase, k, m, M, N1;
+
+    // what every row header of the triangle uses
+    struct RowGeom {
+        V3<double> n02, n12;
+        double sin_v0, tan_v0, sin_v1;
+    };
 
     __device__ __forceinline__ static double sqrt3_2() { return __builtin_sqrt(3.0) / 2; }
 
@@ -229,39 +236,51 @@ struct SampleGen {
         v0 = sel3(max_i, a, b, c);
         v1 = sel3(max_i, b, c, a);
         v2 = sel3(max_i, c, a, b);
+        dir = normalize(v1 - v0);  // the longest edge and every row
         V3<double> e01 = v1 - v0;
         double h = dist(divv(mulv(e01, dot(v2 - v0, e01)), lsm) + v0, v2);
         M = (int)(h / (sqrt3_2() * d));
         phase = 0;
     }
 
+    __device__ __forceinline__ RowGeom row_geom() const
+    {
+        RowGeom g;
+        g.n02 = normalize(v2 - v0);
+        g.n12 = normalize(v2 - v1);
+        g.sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
+        g.tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
+        g.sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
+        return g;
+    }
+
     // phase 2: the header of row m -- its start and sample count (phase 3),
     // or the end of the rows (phase 4)
+    __device__ __forceinline__ void row_header(const RowGeom& g)
+    {
+        const double s32 = sqrt3_2();
+        int n = (int)(s32 / g.tan_v0 * m + 0.5);
+        int n1 = (int)(s32 / g.tan_v0 * m);
+        if (m % 2 == 0 && n == n1) n += 1;
+        V3<double> v0_m = v0 + mulv(g.n02, m * s32 * d / g.sin_v0);
+        V3<double> v1_m = v1 + mulv(g.n12, m * s32 * d / g.sin_v1);
+        if (dist(v0_m, v1_m) <= d) {
+            phase = 4;
+            return;
+        }
+        double delta_d = ((n + (m % 2) / 2.0) - m * s32 / g.tan_v0) * d;
+        rowv = v0_m + mulv(dir, delta_d);
+        N1 = (int)(dist(rowv, v1_m) / d);
+        k = 0;
+        phase = 3;
+    }
     __device__ __forceinline__ void row_header()
     {
         if (m > M) {
             phase = 4;
             return;
         }
-        const double s32 = sqrt3_2();
-        const V3<double> n02 = normalize(v2 - v0), n12 = normalize(v2 - v1);
-        const double sin_v0 = len(cross(v2 - v0, v1 - v0)) / (dist(v0, v2) * dist(v0, v1));
-        const double tan_v0 = len(cross(v2 - v0, v1 - v0)) / dot(v2 - v0, v1 - v0);
-        const double sin_v1 = len(cross(v2 - v1, v0 - v1)) / (dist(v1, v2) * dist(v0, v1));
-        int n = (int)(s32 / tan_v0 * m + 0.5);
-        int n1 = (int)(s32 / tan_v0 * m);
-        if (m % 2 == 0 && n == n1) n += 1;
-        V3<double> v0_m = v0 + mulv(n02, m * s32 * d / sin_v0);
-        V3<double> v1_m = v1 + mulv(n12, m * s32 * d / sin_v1);
-        if (dist(v0_m, v1_m) <= d) {
-            phase = 4;
-            return;
-        }
-        double delta_d = ((n + (m % 2) / 2.0) - m * s32 / tan_v0) * d;
-        rowv = v0_m + mulv(normalize(v1 - v0), delta_d);
-        N1 = (int)(dist(rowv, v1_m) / d);
-        k = 0;
-        phase = 3;
+        row_header(row_geom());
     }
 
     // after v2: the edge v1 -> v2 (|v1 - v2|^2 is the edge's squared length)
@@ -270,6 +289,7 @@ struct SampleGen {
         N = __builtin_sqrt(len2(v1 - v2)) / d;
         if (N > 1) {
             if (N == (int)N) N -= 1;
+            dir = normalize(v2 - v1);
             k = 1;
             phase = 5;
         } else {
@@ -283,6 +303,7 @@ struct SampleGen {
         N = __builtin_sqrt(len2(v2 - v0)) / d;
         if (!(N > 1)) return false;
         if (N == (int)N) N -= 1;
+        dir = normalize(v0 - v2);
         k = 1;
         phase = 7;
         return true;
@@ -295,6 +316,8 @@ struct SampleGen {
     __device__ __forceinline__ int64_t skip(int64_t s)
     {
         int64_t done = 0;
+        RowGeom g;
+        bool have_g = false;
         for (;;) {
             if (done >= s) return done;
             const int64_t want = s - done;
@@ -328,7 +351,15 @@ struct SampleGen {
                 }
                 break;
             case 2:
-                row_header();
+                if (m > M) {
+                    phase = 4;
+                } else {
+                    if (!have_g) {
+                        g = row_geom();
+                        have_g = true;
+                    }
+                    row_header(g);
+                }
                 break;
             case 3:
                 if (k <= N1) {
@@ -391,7 +422,7 @@ struct SampleGen {
                 return false;
             case 0:  // longest edge
                 if (k <= N) {
-                    p = v0 + mulv(mulv(normalize(v1 - v0), d), (double)k);
+                    p = v0 + mulv(mulv(dir, d), (double)k);
                     k++;
                     return true;
                 }
@@ -412,7 +443,7 @@ struct SampleGen {
                 break;
             case 3:  // row samples
                 if (k <= N1) {
-                    p = rowv + mulv(mulv(normalize(v1 - v0), (double)k), d);
+                    p = rowv + mulv(mulv(dir, (double)k), d);
                     k++;
                     return true;
                 }
@@ -425,7 +456,7 @@ struct SampleGen {
                 return true;
             case 5:
                 if (k <= N) {
-                    p = v1 + mulv(mulv(normalize(v2 - v1), d), (double)k);
+                    p = v1 + mulv(mulv(dir, d), (double)k);
                     k++;
                     return true;
                 }
@@ -436,7 +467,7 @@ struct SampleGen {
                 break;
             case 7:
                 if (k <= N) {
-                    p = v2 + mulv(mulv(normalize(v0 - v2), d), (double)k);
+                    p = v2 + mulv(mulv(dir, d), (double)k);
                     k++;
                     return true;
                 }
