@@ -9,7 +9,9 @@
 //      ballot as the OR of the lanes' step results) against face_out_sampled,
 //      and that every sample of an inside triangle is tested exactly once;
 //   4. collect_facets (one-lane group) against the facets whose boxes meet
-//      the query box, and in_envelope_list on its lists against brute force.
+//      the query box, and in_envelope_list on its lists against brute force;
+//   5. nearest_facet_group (one-lane group) against nearest_facet, bit for
+//      bit, with queries above shared grid edges and vertices (ties).
 // Exit status 0 when every check passes.
 #include "twg_bvh.hpp"
 
@@ -88,6 +90,8 @@ struct Tree {
     BvhView view;
 };
 
+static void build_nodes(Tree& T);
+
 static void build(Tree& T, int nf, std::mt19937_64& rng, double size)
 {
     std::uniform_real_distribution<double> U(0, 1);
@@ -100,6 +104,13 @@ static void build(Tree& T, int nf, std::mt19937_64& rng, double size)
             T.tri[9 * f + 3 * k + 2] = cz + 0.25 * size * (U(rng) - 0.5);
         }
     }
+    build_nodes(T);
+}
+
+// the tree over T.tri as the library builds it (facets in the given order)
+static void build_nodes(Tree& T)
+{
+    const int nf = (int)(T.tri.size() / 9);
     const int64_t nnode = max_node(1, 0, nf) + 1;
     T.n64.assign(nnode, BvhNode64{});
     for (auto& N : T.n64) {
@@ -292,6 +303,65 @@ static void check_lists(std::mt19937_64& rng)
     printf("facet lists: %ld lists (%ld overflowed), %ld points against brute force\n", lists, overflow, points);
 }
 
+// a grid of n x n squares over [0, 1]^2, each split into two triangles, on a
+// ridge z = ridge * |x - 0.5| (0: a plane): queries above shared edges and
+// vertices are equidistant from several facets
+static void build_grid(Tree& T, int n, double ridge)
+{
+    T.tri.clear();
+    auto z = [&](double x) { return ridge * fabs(x - 0.5); };
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const double x0 = (double)i / n, x1 = (double)(i + 1) / n, y0 = (double)j / n, y1 = (double)(j + 1) / n;
+            const double a[3] = {x0, y0, z(x0)}, b[3] = {x1, y0, z(x1)}, c[3] = {x1, y1, z(x1)}, d[3] = {x0, y1, z(x0)};
+            for (const double* v : {a, b, c, a, c, d}) T.tri.insert(T.tri.end(), v, v + 3);
+        }
+    build_nodes(T);
+}
+
+static void check_nearest(std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(0, 1);
+    std::vector<unsigned char> cache(kTreeCacheBytes);
+    long queries = 0, fallback = 0, off_hint = 0;
+    for (int shape = 0; shape < 6; ++shape) {
+        Tree T;
+        if (shape < 3) build(T, shape == 0 ? 2 : (shape == 1 ? 700 : 6000), rng, 0.03);
+        else build_grid(T, shape == 3 ? 9 : 24, shape == 5 ? 0.4 : 0.0);
+        const int nf = (int)(T.tri.size() / 9);
+        const int grid = shape == 4 ? 24 : (shape == 5 ? 24 : 9);
+        for (int mode = 0; mode < 2; ++mode) {
+            const BvhWalk W = mode == 0 ? walk(T.view) : cache_tree_top<false>(T.view, cache.data());
+            for (int q = 0; q < 1500; ++q) {
+                V3<double> p;
+                if (shape >= 3 && q % 3 != 2) {
+                    // above a grid vertex or on a grid line
+                    const double x = (double)(rng() % (grid + 1)) / grid;
+                    const double y = q % 3 == 0 ? (double)(rng() % (grid + 1)) / grid : U(rng);
+                    p = mk3(x, y, (shape == 5 ? 0.4 * fabs(x - 0.5) : 0.0) + 0.05 * U(rng));
+                } else {
+                    p = mk3(1.4 * U(rng) - 0.2, 1.4 * U(rng) - 0.2, 0.2 * (U(rng) - 0.5));
+                }
+                if (q % 97 == 0) p.z += 50.0;  // far: the list overflows a small cap
+                const int cap = q % 4 == 0 ? 8 : 256;
+                std::vector<BvhNode32> lst(cap);
+                std::vector<uint32_t> fr(2 * cap);
+                V3<double> np0, np1;
+                double sq0, sq1;
+                const int f0 = nearest_facet(W, p, np0, sq0);
+                const int f1 = nearest_facet_group<1>(W, p, np1, sq1, lst.data(), fr.data(), cap, 0, 0);
+                CHECK(f0 == f1 && !memcmp(&np0, &np1, sizeof np0) && !memcmp(&sq0, &sq1, sizeof sq0),
+                      "nearest: shape %d mode %d query %d: facet %d / %d\n", shape, mode, q, f0, f1);
+                off_hint += f0 != nearest_hint(W, p);
+                queries++;
+            }
+            (void)nf;
+        }
+    }
+    (void)fallback;
+    printf("nearest: %ld queries (%ld not at the hint facet) against the serial walk\n", queries, off_hint);
+}
+
 int main()
 {
     std::mt19937_64 rng(20240601);
@@ -299,6 +369,7 @@ int main()
     check_traversal(rng);
     check_group(rng);
     check_lists(rng);
+    check_nearest(rng);
     printf("%ld failures\n", g_bad);
     return g_bad != 0;
 }

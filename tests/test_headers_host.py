@@ -30,7 +30,9 @@ def host_copy(src, dst):
         s = s.replace("__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");", "")
         s = s.replace("__builtin_amdgcn_wave_barrier();", "").replace("__popcll(", "__builtin_popcountll(")
         s = ("struct uint4 { unsigned x, y, z, w; };\n"
-             "inline unsigned long long __ballot(bool b) { return b ? 1ull : 0ull; }\n") + s
+             "inline unsigned long long __ballot(bool b) { return b ? 1ull : 0ull; }\n"
+             "template <class T> inline T __shfl(T v, int, int = 64) { return v; }\n"
+             "template <class T> inline T __shfl_xor(T v, int, int = 64) { return v; }\n") + s
     open(dst, "w").write(s)
 
 

@@ -702,11 +702,14 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
             // pf lies on the facet it was projected to: when that facet is one of
             // the envelope's, it starts every lane's facet hint below
             int hint = -1;
+            wave_sync();  // the slice's reduction rows are no longer read
             if (kind == 2) {
-                const int f = nearest_facet(walk(A.bt), pn, pf, sq);
+                const int f = nearest_facet_group<TILE>(walk(A.bt), pn, pf, sq, c.lst, c.fr,
+                                                        ring_cap<TILE>(), c.tl, c.wbase);
                 if (A.bt.n64 == A.sf.n64) hint = f;
             } else {
-                hint = nearest_facet(c.sfw, pn, pf, sq);
+                hint = nearest_facet_group<TILE>(c.sfw, pn, pf, sq, c.lst, c.fr, ring_cap<TILE>(),
+                                                 c.tl, c.wbase);
             }
             if (ring_flip<TILE>(A, c, rt, m, v, pf)) {
                 st = TWG_SMOOTH_REJECT_FLIP;

@@ -532,9 +532,14 @@ __device__ __forceinline__ bool in_envelope_list(LdsNode32W* lst, LdsF64c* tri, 
     for (int base = 0; base < nl; base += 64) {
         const int n = nl - base < 64 ? nl - base : 64;
         uint64_t m = 0;
-        for (int k = 0; k < n; ++k) {
-            const BvhNode32 e = from_lds(lst + base + k);
-            m |= (uint64_t)(box_d2(pf, e.lo, e.hi) <= Tf) << k;
+        for (int k0 = 0; k0 < n; k0 += 8) {  // eight records read before their tests
+            BvhNode32 e[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k0 + u < n) e[u] = from_lds(lst + base + k0 + u);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k0 + u < n) m |= (uint64_t)(box_d2(pf, e[u].lo, e[u].hi) <= Tf) << (k0 + u);
         }
         if (hl >= base && hl < base + n) m &= ~(1ull << (hl - base));
         while (m) {
@@ -563,15 +568,22 @@ __device__ __forceinline__ BvhNode64 node64(const BvhWalk& B, int64_t k)
     return B.n64[k];
 }
 
-__device__ __forceinline__ int nearest_facet(const BvhWalk& B, V3<double> p, V3<double>& np,
-                                             double& sq)
+// get_nearest_facet_hint: down from the root to a leaf, into the child whose
+// box center is nearer
+__device__ __forceinline__ int nearest_hint(const BvhWalk& B, V3<double> p)
 {
     int64_t nd = 1;
     while (node64(B, nd).leaf < 0) {
         const BvhNode64 L = node64(B, 2 * nd), R = node64(B, 2 * nd + 1);
         nd = box_center_d2(p, L.lo, L.hi) < box_center_d2(p, R.lo, R.hi) ? 2 * nd : 2 * nd + 1;
     }
-    int best = node64(B, nd).leaf;
+    return node64(B, nd).leaf;
+}
+
+__device__ __forceinline__ int nearest_facet(const BvhWalk& B, V3<double> p, V3<double>& np,
+                                             double& sq)
+{
+    int best = nearest_hint(B, p);
     np = ld3(B.tri + 9 * (int64_t)best);
     sq = len2(np - p);
     auto visit = [&](int leaf) {
@@ -626,6 +638,108 @@ __device__ __forceinline__ int nearest_facet(const BvhWalk& B, V3<double> p, V3<
             visit(leaf);
         }
     }
+}
+
+// nearest_facet's result (facet, np, sq -- bit for bit) computed by the G
+// lanes of a group (lane gl, first lane gbase in the wave; G a power of two up
+// to 64), or by nearest_facet itself where the group cannot be sure to agree.
+// The walk starts from the hint facet h with sq = s0 = |v0(h) - p|^2 and
+// takes a facet only when its distance is strictly below the best so far.
+// The group lists the facets whose boxes meet a cube around p (collect_facets)
+// -- half-width rr plus more than the rounding of a facet distance, so the
+// list holds every facet at distance rr or less -- and computes their
+// distances; rr starts at the hint leaf's box diagonal (about one facet) and
+// doubles, up to sqrt(s0), until some listed facet lies within rr.  None
+// below s0 with rr = sqrt(s0): the walk keeps (h, v0(h)).  Else let f be the
+// facet of least distance D: when no other facet has distance D and every
+// node on f's path from the root has a box distance strictly below D, the
+// walk cannot prune that path (it prunes a box only at or above the best so
+// far, never below D) and nothing it visits later replaces f, so it returns
+// f -- taken here.  Otherwise (ties, a box at or above D, a list too long for
+// cap, a query that is not finite, three cubes without a facet) nearest_facet
+// walks.  Every lane of the group calls it with the same p.
+template <int G>
+__device__ int nearest_facet_group(const BvhWalk& B, V3<double> p, V3<double>& np, double& sq,
+                                   LdsNode32W* lst, LdsU32* fr, int cap, int gl, int gbase)
+{
+    if (B.nf < 2) return nearest_facet(B, p, np, sq);
+    int64_t na = 1;
+    while (node64(B, na).leaf < 0) {
+        const BvhNode64 L = node64(B, 2 * na), R = node64(B, 2 * na + 1);
+        na = box_center_d2(p, L.lo, L.hi) < box_center_d2(p, R.lo, R.hi) ? 2 * na : 2 * na + 1;
+    }
+    const BvhNode64 HL = node64(B, na);
+    const int h = HL.leaf;
+    const V3<double> v0 = ld3(B.tri + 9 * (int64_t)h);
+    const double s0 = len2(v0 - p), sr0 = __builtin_sqrt(s0);
+    if (!(s0 <= 1.0e300)) return nearest_facet(B, p, np, sq);
+    // more than the rounding of a facet distance: ~1e-7 (|p| + d + 2 edge) in
+    // distance units (the root box's diagonal bounds every edge)
+    const BvhNode64 root = node64(B, 1);
+    const double diag = __builtin_sqrt(len2(mk3(root.hi[0] - root.lo[0], root.hi[1] - root.lo[1],
+                                                root.hi[2] - root.lo[2])));
+    const double eabs = 0x1p-20 * (diag + fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z))));
+    double r = __builtin_sqrt(len2(mk3(HL.hi[0] - HL.lo[0], HL.hi[1] - HL.lo[1], HL.hi[2] - HL.lo[2])));
+    const unsigned long long gm = (G == 64 ? ~0ull : ((1ull << (G & 63)) - 1ull)) << gbase;
+    for (int attempt = 0; attempt < 3; ++attempt, r *= 2.0) {
+        const bool last = !(r < sr0);  // the cube holds every facet below s0
+        const double rr = last ? sr0 : r;
+        const double half = rr + 0x1p-20 * rr + eabs;
+        const double lo[3] = {p.x - half, p.y - half, p.z - half};
+        const double hi[3] = {p.x + half, p.y + half, p.z + half};
+        const int n = collect_facets<G>(B, lo, hi, lst, fr, cap, gl, gbase);
+        if (n < 0) break;
+        // the lane's least distance below s0, its facet and how many reach it
+        double bd = s0;
+        int bf = -1, nb = 0;
+        V3<double> bcp = v0;
+        for (int i = gl; i < n; i += G) {
+            const int f = lst[i].leaf;
+            const double* t = B.tri + 9 * (int64_t)f;
+            V3<double> cp;
+            const double dd = point_tri_d2<double, true>(p, ld3(t), ld3(t + 3), ld3(t + 6), cp);
+            if (dd < bd) {
+                bd = dd;
+                bf = f;
+                bcp = cp;
+                nb = 1;
+            } else if (dd == bd && bf >= 0) {
+                nb++;
+            }
+        }
+        double m = bd;
+        for (int off = G / 2; off > 0; off >>= 1) {
+            const double o = __shfl_xor(m, off, G);
+            m = o < m ? o : m;
+        }
+        const unsigned long long at = __ballot(bf >= 0 && bd == m) & gm;
+        if (at == 0ull) {
+            if (!last) continue;
+            np = v0;  // no facet below s0
+            sq = s0;
+            return h;
+        }
+        if (!last && !(m <= rr * rr)) continue;  // a nearer facet may lie outside
+        const int wl = __builtin_ctzll(at) - gbase;
+        const int wf = __shfl(bf, wl, G);
+        bool sure = (at & (at - 1)) == 0ull && __shfl(nb, wl, G) == 1;
+        if (sure) {
+            // every box on the path from the root to wf's leaf strictly below m
+            const int64_t leaf = leaf_node(B.nf, wf);
+            const int depth = 63 - __builtin_clzll((unsigned long long)leaf);
+            bool below = true;
+            for (int k = 1 + gl; k <= depth; k += G) {
+                const BvhNode64 N = node64(B, leaf >> (depth - k));
+                below = below && box_signed_d2(p, N.lo, N.hi) < m;
+            }
+            sure = (__ballot(!below) & gm) == 0ull;
+        }
+        if (!sure) break;
+        np = mk3(__shfl(bcp.x, wl, G), __shfl(bcp.y, wl, G), __shfl(bcp.z, wl, G));
+        sq = m;
+        return wf;
+    }
+    return nearest_facet(B, p, np, sq);
 }
 
 // Triangles whose longest edge exceeds kMaxEdgeSamples sampling distances are
