@@ -149,6 +149,25 @@ def test_bad_surface_rejected(ctx):
         ctx.surface(V, F)
 
 
+def test_dense_faces_wave_lists(ctx, dev):
+    """A Morton-ordered batch with >= 8 queries per facet runs faces_list_kernel
+    (a wave's facets listed in LDS): decisions vs the oracle and vs the
+    per-sample tree walk (the statistics path)"""
+    V, F = W.bumpy_sphere(freq=16, seed=0)
+    eps = 1e-3
+    n = 70000
+    assert n >= 65536 and n >= 8 * len(F)
+    surf = ctx.surface(V, F)
+    Q = W.query_triangles(V, F, n, eps=eps, edge=0.002, seed=21)
+    ref, _ = O.Surface(V, F).face_out(Q, eps * eps, eps)
+    assert 0.2 < ref.mean() < 0.8
+    for precision in (0, 1):
+        out = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision).cpu().numpy()
+        walk, _ = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision, stats=True)
+        assert np.array_equal(out, ref), precision
+        assert np.array_equal(walk.cpu().numpy(), ref), precision
+
+
 def test_large_surface_faces(ctx, dev):
     """1M-facet bumpy sphere, 100k queries: decisions vs the oracle's tree"""
     V, F = W.bumpy_sphere(freq=224, seed=0)

@@ -78,6 +78,74 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (STATS) flush_stats(st, stats);
 }
 
+// One triangle per lane, as faces_kernel, for batches walked in Morton
+// order: the wave's 64 consecutive triangles lie close together, so the wave
+// lists once (collect_facets, LDS) the facets whose boxes meet the box of its
+// triangles padded by more than sqrt(T), with their triangles, and tests its
+// samples against the list (in_envelope_list) instead of walking the tree
+// per sample.  A wave whose list exceeds CAP entries walks the tree.
+template <bool F32, int CAP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void faces_list_kernel(
+    BvhView B, const double* __restrict__ tri9, int64_t n, double eps2, double d, double T,
+    double sqrtT, uint8_t* __restrict__ out, unsigned* n_pending, int64_t* pending,
+    const int32_t* __restrict__ order)
+{
+    constexpr int kWaveBytes = CAP * (32 + 8 + 72);
+    __shared__ __align__(16) unsigned char cache[kPointCacheBytes];
+    __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
+    const BvhWalk W = cache_tree_top<F32, kPointCacheBytes>(B, cache);
+    const int lane = threadIdx.x % 64;
+    unsigned char* mine = lists + (threadIdx.x / 64) * kWaveBytes;
+    LdsNode32W* lst = (LdsNode32W*)mine;
+    LdsU32* fr = (LdsU32*)(mine + CAP * 32);
+    LdsF64* tri = (LdsF64*)(mine + CAP * 40);
+    int64_t i = xcd_block() * kBlock + threadIdx.x;
+    bool act = false;
+    const double* t = tri9;
+    if (i < n) {
+        if (order) i = order[i];
+        t = tri9 + 9 * i;
+        if (certainly_not_collinear(t, t + 3, t + 6)) {
+            act = true;
+        } else {
+            unsigned k = atomicAdd(n_pending, 1u);
+            pending[k] = i;
+        }
+    }
+    // the box of the wave's triangles, padded
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (act)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            lo[k % 3] = t[k] < lo[k % 3] ? t[k] : lo[k % 3];
+            hi[k % 3] = t[k] > hi[k % 3] ? t[k] : hi[k % 3];
+        }
+    double amax = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ol = __shfl_xor(lo[k], off, 64), oh = __shfl_xor(hi[k], off, 64);
+            lo[k] = ol < lo[k] ? ol : lo[k];
+            hi[k] = oh > hi[k] ? oh : hi[k];
+        }
+        amax = fmax(amax, fmax(fabs(lo[k]), fabs(hi[k])));
+    }
+    const double pad = 1.0001 * sqrtT + 1e-9 * amax;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] -= pad;
+        hi[k] += pad;
+    }
+    const int nl = collect_facets<64>(W, lo, hi, lst, fr, CAP, lane, 0);
+    for (int e = lane; e < nl; e += 64) {
+        const double* f = B.tri + 9 * (int64_t)lst[e].leaf;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) tri[9 * e + k] = f[k];
+    }
+    wave_sync();
+    if (act) out[i] = face_out_listed<F32>(W, t, eps2, d, T, sqrtT, lst, tri, nl) ? 1 : 0;
+}
+
 // A group of G lanes per triangle, its samples spread over the lanes
 // (face_out_group).  Small batches run G = 64 (a wave per triangle), so that
 // few triangles with many samples each still fill the machine.  Same outputs
@@ -131,6 +199,13 @@ __global__ void zero_u32(unsigned* p) { *p = 0u; }
 constexpr int64_t kSortMin = 1 << 16;
 // face batches below this size run a wave per triangle (faces_group_kernel)
 constexpr int64_t kWaveMax = 1 << 16;
+// facets per wave list in faces_list_kernel (48: four blocks per CU), and
+// the queries per facet from which a wave's 64 triangles (about 64 / density
+// facets apart) usually fit it: measured on 1 M facets, 20 M queries ran
+// 540 -> 720 M checks/s with the lists, 2 M queries 422 -> 395 (overflowing
+// lists), so sparser batches keep faces_kernel
+constexpr int kFaceListCap = 48;
+constexpr int64_t kFaceListDensity = 8;
 
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
                                                          int64_t n, double* __restrict__ cp3,
@@ -454,6 +529,11 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
             hipLaunchKernelGGL((faces_group_kernel<false, 64>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
         else
             hipLaunchKernelGGL((faces_group_kernel<true, 64>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
+    } else if (order && !stats && n >= kFaceListDensity * B->nf) {
+        if (precision == TWG_FP64)
+            hipLaunchKernelGGL((faces_list_kernel<false, kFaceListCap>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
+        else
+            hipLaunchKernelGGL((faces_list_kernel<true, kFaceListCap>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
     } else if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
         else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);

@@ -75,8 +75,6 @@ constexpr int ring_cap() { return tile_bytes<TILE>() / kListEntryBytes; }
 static_assert(tile_bytes<16>() >= 16 * kQ * 8 && tile_bytes<64>() >= 64 * kQ * 8, "reduction rows");
 static_assert(ring_cap<64>() == 128 && ring_cap<16>() >= 16, "list capacity");
 
-typedef __attribute__((address_space(3))) double LdsF64;
-
 template <int TILE>
 struct TileCtx {
     int tl;            // lane within the tile

@@ -502,6 +502,7 @@ __device__ int collect_facets(const BvhWalk& W, const double lo[3], const double
     return nl;
 }
 
+typedef __attribute__((address_space(3))) double LdsF64;
 typedef __attribute__((address_space(3))) const double LdsF64c;
 
 // facet_d2 of a triangle held in LDS (9 doubles): the same computation on
@@ -762,6 +763,29 @@ __device__ bool face_out_sampled(const BvhWalk& B, const double* t9, double eps2
     return for_each_sample(a, b, c, d, [&](V3<double> s) {
         if (STATS) st.samples++;
         return !in_envelope<F32, STATS>(B, s, eps2, T, sqrtT, hint, st);
+    });
+}
+
+// face_out_sampled's decision with the facets from a collect_facets list
+// (lst, tri; see in_envelope_list) whose box holds the triangle padded by
+// more than sqrt(T): every sample lies in that box, so the list holds every
+// facet the tree walk could accept.  nl < 0: the tree walk.
+template <bool F32>
+__device__ bool face_out_listed(const BvhWalk& B, const double* t9, double eps2, double d,
+                                double T, double sqrtT, LdsNode32W* lst, LdsF64c* tri, int nl)
+{
+    if (nl < 0) {
+        Stats st;
+        return face_out_sampled<F32, false>(B, t9, eps2, d, T, sqrtT, st);
+    }
+    const V3<double> a = ld3(t9), b = ld3(t9 + 3), c = ld3(t9 + 6);
+    {
+        double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
+        if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
+    }
+    int hl = -1;
+    return for_each_sample(a, b, c, d, [&](V3<double> s) {
+        return !in_envelope_list(lst, tri, nl, s, eps2, sqrtT, hl);
     });
 }
 
