@@ -136,7 +136,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         lo[k] -= pad;
         hi[k] += pad;
     }
-    const int nl = collect_facets<64>(W, lo, hi, lst, fr, CAP, lane, 0);
+    // a coordinate that is not finite can leave the lanes with different
+    // boxes (min / max ignore NaN): then the whole wave walks the tree
+    bool fin = true;
+    if (act)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) fin = fin && is_finite(t[k]);
+    const int nl = __ballot(!fin) ? -1 : collect_facets<64>(W, lo, hi, lst, fr, CAP, lane, 0);
     for (int e = lane; e < nl; e += 64) {
         const double* f = B.tri + 9 * (int64_t)lst[e].leaf;
 #pragma unroll

@@ -525,15 +525,20 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
     int nl = -1, hl = -1;
     {
         double lo[3] = {pf.x, pf.y, pf.z}, hi[3] = {pf.x, pf.y, pf.z};
+        bool fin = is_finite(pf.x) && is_finite(pf.y) && is_finite(pf.z);
         for (int64_t k = t0 + c.tl; k < t1; k += TILE) {
             const V3<double> qa = vtx(A, A.tri_ab[2 * k]), qb = vtx(A, A.tri_ab[2 * k + 1]);
             const double qs[6] = {qa.x, qa.y, qa.z, qb.x, qb.y, qb.z};
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
+                fin = fin && is_finite(qs[j]);
                 lo[j % 3] = qs[j] < lo[j % 3] ? qs[j] : lo[j % 3];
                 hi[j % 3] = qs[j] > hi[j % 3] ? qs[j] : hi[j % 3];
             }
         }
+        // a coordinate that is not finite can leave the lanes with different
+        // boxes (min / max ignore NaN): then the tile walks the tree
+        const bool all_fin = !tile_any(c, !fin);
         double amax = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -551,7 +556,7 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
             hi[k] += pad;
         }
         wave_sync();  // the slice's reduction rows are no longer read
-        nl = collect_facets<TILE>(W, lo, hi, c.lst, c.fr, ring_cap<TILE>(), c.tl, c.wbase);
+        nl = all_fin ? collect_facets<TILE>(W, lo, hi, c.lst, c.fr, ring_cap<TILE>(), c.tl, c.wbase) : -1;
         for (int i = c.tl; i < nl; i += TILE) {
             const double* t = W.tri + 9 * (int64_t)c.lst[i].leaf;
 #pragma unroll
