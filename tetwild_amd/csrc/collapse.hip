@@ -383,10 +383,14 @@ __global__ __launch_bounds__(kBlock) void collapse_point_kernel(CollapseArgs A)
 
 // lanes per triangle in the face pass (samples spread over the group)
 constexpr int kFaceGroup = 64;
+// facets listed per wave in the face pass (a triangle's neighbourhood)
+constexpr int kFaceListCap = 64;
+constexpr int kFaceListBytes = kFaceListCap * (32 + 8 + 72);
 
 template <bool F32>
 __global__ __launch_bounds__(kBlock) void collapse_face_kernel(CollapseArgs A)
 {
+    __shared__ __align__(16) unsigned char face_lists[(kBlock / 64) * kFaceListBytes];
     const int r = threadIdx.x % kFaceGroup;
     const int gbase = (threadIdx.x % 64) - r;
     const int64_t k = (xcd_block() * (int64_t)kBlock + threadIdx.x) / kFaceGroup;
@@ -409,10 +413,38 @@ __global__ __launch_bounds__(kBlock) void collapse_face_kernel(CollapseArgs A)
         if (r == 0) A.tri_cand[k] = -2 - (int32_t)c;  // for the exact face pass
         return;
     }
-    // without the subtree step after a hint miss (on the 1M-candidate sheet
-    // workload this pass ran 6% slower with it)
-    if (face_out_group<F32, kFaceGroup, false>(walk(A.sf), ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.eps2,
-                                        A.d, A.T_sf, A.sqrtT_sf, r, gbase) &&
+    // the facets near the triangle, listed in the wave's LDS (collect_facets);
+    // the tree walk when they do not fit (without the subtree step after a
+    // hint miss: on the 1M-candidate sheet workload it ran 6% slower)
+    const BvhWalk W = walk(A.sf);
+    unsigned char* mine = face_lists + (threadIdx.x / 64) * kFaceListBytes;
+    LdsNode32W* lst = (LdsNode32W*)mine;
+    LdsU32* fr = (LdsU32*)(mine + kFaceListCap * 32);
+    LdsF64* tri = (LdsF64*)(mine + kFaceListCap * 40);
+    double lo[3], hi[3], amax = 0.0;
+    bool fin = true;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        lo[j] = fmin(t9[j], fmin(t9[3 + j], t9[6 + j]));
+        hi[j] = fmax(t9[j], fmax(t9[3 + j], t9[6 + j]));
+        fin = fin && is_finite(t9[j]) && is_finite(t9[3 + j]) && is_finite(t9[6 + j]);
+        amax = fmax(amax, fmax(fabs(lo[j]), fabs(hi[j])));
+    }
+    const double pad = 1.0001 * A.sqrtT_sf + 1e-9 * amax;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        lo[j] -= pad;
+        hi[j] += pad;
+    }
+    const int nl = fin ? collect_facets<64>(W, lo, hi, lst, fr, kFaceListCap, r, 0) : -1;
+    for (int e = r; e < nl; e += 64) {
+        const double* f = W.tri + 9 * (int64_t)lst[e].leaf;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) tri[9 * e + q] = f[q];
+    }
+    wave_sync();
+    if (face_out_group<F32, kFaceGroup, false>(W, ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.eps2, A.d,
+                                               A.T_sf, A.sqrtT_sf, r, gbase, lst, tri, nl) &&
         r == 0)
         atomicOr(A.work + c, kFaceOut);
 }

@@ -791,7 +791,8 @@ __device__ bool face_out_listed(const BvhWalk& B, const double* t9, double eps2,
 
 // face_out_sampled's decision computed by a group of G lanes (G a power of
 // two up to 64, groups aligned in the wave; every lane of a group passes the
-// same triangle).  The sampler's sequence of `total` samples is cut into G
+// same triangle), against a collect_facets list (lst, tri, nl >= 0; its box
+// holding the triangle padded by more than sqrt(T)) or the tree (nl < 0).  The sampler's sequence of `total` samples is cut into G
 // consecutive blocks of K = ceil(total / G); lane r moves to sample r K
 // (SampleGen::skip passes whole rows by their headers) and tests its block
 // in order, one sample per step, and after each step a ballot over the group
@@ -802,7 +803,8 @@ __device__ bool face_out_listed(const BvhWalk& B, const double* t9, double eps2,
 // machine idle.
 template <bool F32, int G, bool SUBTREE = true>
 __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c,
-                               double eps2, double d, double T, double sqrtT, int r, int gbase)
+                               double eps2, double d, double T, double sqrtT, int r, int gbase,
+                               LdsNode32W* lst = nullptr, LdsF64c* tri = nullptr, int nl = -1)
 {
     static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
     {
@@ -820,13 +822,14 @@ __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<
     SampleGen g;
     g.init(a, b, c, d);
     g.skip(first);
-    int hint = -1;
+    int hint = -1, hl = -1;
     Stats st;
     const unsigned long long gmask = G == 64 ? ~0ull : (1ull << (G & 63)) - 1ull;
     for (int64_t j = 0; j < K; ++j) {
         V3<double> q = a;
         const bool have = first + j < total && g.next(q);
-        const bool out = have && !in_envelope<F32, false, SUBTREE>(B, q, eps2, T, sqrtT, hint, st);
+        const bool out = have && !(nl >= 0 ? in_envelope_list(lst, tri, nl, q, eps2, sqrtT, hl)
+                                           : in_envelope<F32, false, SUBTREE>(B, q, eps2, T, sqrtT, hint, st));
         if ((__ballot(out) >> gbase) & gmask) return true;
     }
     return false;
