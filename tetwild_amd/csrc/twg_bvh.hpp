@@ -106,6 +106,9 @@ struct BvhWalk {
     LdsNode64* c64;
     LdsNode32* c32;
     int64_t ncache;
+    // which copy holds the cached slots (an LDS pointer cannot say: offset 0,
+    // where the copy may sit, is also its null value)
+    bool top64;
 };
 
 // a walker that reads every node from global memory
@@ -120,6 +123,7 @@ __device__ __forceinline__ BvhWalk walk(const BvhView& B)
     w.c64 = nullptr;
     w.c32 = nullptr;
     w.ncache = 0;
+    w.top64 = false;
     return w;
 }
 
@@ -145,6 +149,7 @@ __device__ __forceinline__ BvhWalk cache_tree_top(const BvhView& B, void* cache)
     if (F32) w.c32 = (LdsNode32*)cache;
     else w.c64 = (LdsNode64*)cache;
     w.ncache = nc;
+    w.top64 = !F32;
     return w;
 }
 
@@ -433,7 +438,7 @@ __device__ int collect_facets(const BvhWalk& W, const double lo[3], const double
             bool ok[4] = {false, false, false, false};
             if (i < ncur) {
                 const int64_t ch = 2 * (int64_t)cur[i];
-                if (ch < W.ncache && W.c64) {
+                if (ch < W.ncache && W.top64) {
                     const BvhNode64 L64 = from_lds(W.c64 + ch), R64 = from_lds(W.c64 + ch + 1);
                     ok[0] = boxes_meet(L64.lo, L64.hi, lo, hi);
                     ok[1] = boxes_meet(R64.lo, R64.hi, lo, hi);
