@@ -168,6 +168,30 @@ def test_dense_faces_wave_lists(ctx, dev):
         assert np.array_equal(walk.cpu().numpy(), ref), precision
 
 
+def test_dense_points_wave_lists(ctx, dev):
+    """A Morton-ordered batch with >= 8 points per facet runs points_list_kernel
+    (a wave's facets listed in LDS): decisions vs the oracle and vs the
+    per-point tree walk (the statistics path); a few points with NaN
+    coordinates send their waves to the tree walk"""
+    V, F = W.bumpy_sphere(freq=16, seed=0)
+    eps = 1e-3
+    n = 70000
+    assert n >= 65536 and n >= 8 * len(F)
+    surf = ctx.surface(V, F)
+    P = W.envelope_samples(V, F, n, eps=eps, band=0.01, seed=22)
+    ref = O.Surface(V, F).point_out(P, eps * eps)
+    assert 0.2 < ref.mean() < 0.8
+    Pn = P.copy()
+    Pn[::9973, 1] = np.nan
+    for precision in (0, 1):
+        for Q in (P, Pn):
+            out = ctx.envelope_points(surf, dev_t(dev, Q), eps * eps, precision=precision).cpu().numpy()
+            walk, _ = ctx.envelope_points(surf, dev_t(dev, Q), eps * eps, precision=precision, stats=True)
+            assert np.array_equal(out, walk.cpu().numpy()), precision
+        assert np.array_equal(ctx.envelope_points(surf, dev_t(dev, P), eps * eps,
+                                                  precision=precision).cpu().numpy(), ref), precision
+
+
 def test_large_surface_faces(ctx, dev):
     """1M-facet bumpy sphere, 100k queries: decisions vs the oracle's tree"""
     V, F = W.bumpy_sphere(freq=224, seed=0)

@@ -50,6 +50,77 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
     if (STATS) flush_stats(st, stats);
 }
 
+// points_kernel for batches walked in Morton order with many points per
+// facet: the wave's 64 consecutive points lie close together, so the wave
+// lists once (collect_facets, LDS) the facets whose boxes meet the box of its
+// points padded by more than sqrt(T), with their triangles, and each lane
+// tests its point against the list (in_envelope_list).  A wave whose list
+// exceeds CAP entries, or that holds a coordinate that is not finite, walks
+// the tree.
+template <bool F32, int CAP>
+__global__ __launch_bounds__(kBlock) void points_list_kernel(BvhView B, const double* __restrict__ p3,
+                                                             int64_t n, double eps2, double T,
+                                                             double sqrtT, uint8_t* __restrict__ out,
+                                                             const int32_t* __restrict__ order)
+{
+    constexpr int kWaveBytes = CAP * (32 + 8 + 72);
+    __shared__ __align__(16) unsigned char cache[kPointCacheBytes];
+    __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
+    const BvhWalk W = cache_tree_top<F32, kPointCacheBytes>(B, cache);
+    const int lane = threadIdx.x % 64;
+    unsigned char* mine = lists + (threadIdx.x / 64) * kWaveBytes;
+    LdsNode32W* lst = (LdsNode32W*)mine;
+    LdsU32* fr = (LdsU32*)(mine + CAP * 32);
+    LdsF64* tri = (LdsF64*)(mine + CAP * 40);
+    int64_t i = xcd_block() * kBlock + threadIdx.x;
+    const bool act = i < n;
+    V3<double> p = mk3(0.0, 0.0, 0.0);
+    if (act) {
+        if (order) i = order[i];
+        p = ld3(p3 + 3 * i);
+    }
+    double lo[3] = {act ? p.x : INFINITY, act ? p.y : INFINITY, act ? p.z : INFINITY};
+    double hi[3] = {act ? p.x : -INFINITY, act ? p.y : -INFINITY, act ? p.z : -INFINITY};
+    double amax = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ol = __shfl_xor(lo[k], off, 64), oh = __shfl_xor(hi[k], off, 64);
+            lo[k] = ol < lo[k] ? ol : lo[k];
+            hi[k] = oh > hi[k] ? oh : hi[k];
+        }
+        amax = fmax(amax, fmax(fabs(lo[k]), fabs(hi[k])));
+    }
+    const double pad = 1.0001 * sqrtT + 1e-9 * amax;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] -= pad;
+        hi[k] += pad;
+    }
+    // a coordinate that is not finite can leave the lanes with different
+    // boxes (min / max ignore NaN): then the whole wave walks the tree
+    const bool fin = !act || (is_finite(p.x) && is_finite(p.y) && is_finite(p.z));
+    const int nl = __ballot(!fin) ? -1 : collect_facets<64>(W, lo, hi, lst, fr, CAP, lane, 0);
+    for (int e = lane; e < nl; e += 64) {
+        const double* f = B.tri + 9 * (int64_t)lst[e].leaf;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) tri[9 * e + k] = f[k];
+    }
+    wave_sync();
+    if (act) {
+        bool in;
+        if (nl >= 0) {
+            int hl = -1;
+            in = in_envelope_list(lst, tri, nl, p, eps2, sqrtT, hl);
+        } else {
+            int hint = -1;
+            Stats st;
+            in = in_envelope<F32, false>(W, p, eps2, T, sqrtT, hint, st);
+        }
+        out[i] = in ? 0 : 1;
+    }
+}
+
 // Triangles the degeneracy filter cannot settle are queued (pending[],
 // *n_pending) for faces_exact_kernel, which keeps the exact predicate -- and its
 // register footprint -- out of this kernel.  Capped at 128 VGPRs (4 waves
@@ -253,6 +324,9 @@ constexpr int64_t kWaveMax = 1 << 16;
 // lists), so sparser batches keep faces_kernel
 constexpr int kFaceListCap = 48;
 constexpr int64_t kFaceListDensity = 8;
+// the same for points_list_kernel
+constexpr int kPointListCap = 32;
+constexpr int64_t kPointListDensity = 8;
 
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
                                                          int64_t n, double* __restrict__ cp3,
@@ -539,7 +613,12 @@ int twg_envelope_points(twg_ctx* ctx, const twg_bvh* B, const double* p3, int64_
     const int32_t* order = nullptr;
     if (n >= kSortMin)
         if (int r = morton_order(ctx, p3, n, 3, B->lo, B->hi, &order)) return r;
-    if (precision == TWG_FP64) {
+    if (order && !stats && n >= kPointListDensity * B->nf) {
+        if (precision == TWG_FP64)
+            hipLaunchKernelGGL((points_list_kernel<false, kPointListCap>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, order);
+        else
+            hipLaunchKernelGGL((points_list_kernel<true, kPointListCap>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, order);
+    } else if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((points_kernel<false, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
         else hipLaunchKernelGGL((points_kernel<false, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
     } else {
