@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void points_kernel(BvhView B, const double*
 // points padded by more than sqrt(T), with their triangles, and each lane
 // tests its point against the list (in_envelope_list).  A wave whose list
 // exceeds CAP entries, or that holds a coordinate that is not finite, walks
-// the tree.
+// the tree (from global memory: the LDS holds the lists).
 template <bool F32, int CAP>
 __global__ __launch_bounds__(kBlock) void points_list_kernel(BvhView B, const double* __restrict__ p3,
                                                              int64_t n, double eps2, double T,
@@ -64,9 +64,8 @@ __global__ __launch_bounds__(kBlock) void points_list_kernel(BvhView B, const do
                                                              const int32_t* __restrict__ order)
 {
     constexpr int kWaveBytes = CAP * (32 + 8 + 72);
-    __shared__ __align__(16) unsigned char cache[kPointCacheBytes];
     __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
-    const BvhWalk W = cache_tree_top<F32, kPointCacheBytes>(B, cache);
+    const BvhWalk W = walk(B);
     const int lane = threadIdx.x % 64;
     unsigned char* mine = lists + (threadIdx.x / 64) * kWaveBytes;
     LdsNode32W* lst = (LdsNode32W*)mine;
@@ -154,7 +153,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 // lists once (collect_facets, LDS) the facets whose boxes meet the box of its
 // triangles padded by more than sqrt(T), with their triangles, and tests its
 // samples against the list (in_envelope_list) instead of walking the tree
-// per sample.  A wave whose list exceeds CAP entries walks the tree.
+// per sample.  A wave whose list exceeds CAP entries walks the tree (from
+// global memory: the LDS holds the lists).
 template <bool F32, int CAP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void faces_list_kernel(
     BvhView B, const double* __restrict__ tri9, int64_t n, double eps2, double d, double T,
@@ -162,9 +162,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     const int32_t* __restrict__ order)
 {
     constexpr int kWaveBytes = CAP * (32 + 8 + 72);
-    __shared__ __align__(16) unsigned char cache[kPointCacheBytes];
-    __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
-    const BvhWalk W = cache_tree_top<F32, kPointCacheBytes>(B, cache);
+        __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
+    const BvhWalk W = walk(B);
     const int lane = threadIdx.x % 64;
     unsigned char* mine = lists + (threadIdx.x / 64) * kWaveBytes;
     LdsNode32W* lst = (LdsNode32W*)mine;
@@ -317,15 +316,17 @@ __global__ void zero_u32(unsigned* p) { *p = 0u; }
 constexpr int64_t kSortMin = 1 << 16;
 // face batches below this size run a wave per triangle (faces_group_kernel)
 constexpr int64_t kWaveMax = 1 << 16;
-// facets per wave list in faces_list_kernel (48: four blocks per CU), and
-// the queries per facet from which a wave's 64 triangles (about 64 / density
-// facets apart) usually fit it: measured on 1 M facets, 20 M queries ran
-// 540 -> 720 M checks/s with the lists, 2 M queries 422 -> 395 (overflowing
-// lists), so sparser batches keep faces_kernel
-constexpr int kFaceListCap = 48;
+// facets per wave list in faces_list_kernel (80: four blocks per CU; the
+// kernel keeps no LDS copy of the tree top -- 48 entries with a 16 KB copy
+// ran 8 % slower), and the queries per facet from which a wave's 64
+// triangles (about 64 / density facets apart) usually fit it: measured on
+// 1 M facets, 20 M queries ran 540 -> 720 M checks/s with the lists, 2 M
+// queries 422 -> 395 (overflowing lists), so sparser batches keep faces_kernel
+constexpr int kFaceListCap = 80;
 constexpr int64_t kFaceListDensity = 8;
-// the same for points_list_kernel
-constexpr int kPointListCap = 32;
+// the same for points_list_kernel (64 entries, no LDS copy of the tree top:
+// 32 entries with a 16 KB copy ran 2.00 G checks/s on 20 M points, this 2.92 G)
+constexpr int kPointListCap = 64;
 constexpr int64_t kPointListDensity = 8;
 
 __global__ __launch_bounds__(kBlock) void nearest_kernel(BvhView B, const double* __restrict__ p3,
