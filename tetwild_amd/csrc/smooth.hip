@@ -556,7 +556,9 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
             hi[k] += pad;
         }
         wave_sync();  // the slice's reduction rows are no longer read
-        nl = all_fin ? collect_facets<TILE>(W, lo, hi, c.lst, c.fr, ring_cap<TILE>(), c.tl, c.wbase) : -1;
+        // (listed from the global tree: ~1 % faster than through the LDS copy)
+        nl = all_fin ? collect_facets<TILE>(walk(A.sf), lo, hi, c.lst, c.fr, ring_cap<TILE>(), c.tl, c.wbase)
+                     : -1;
         for (int i = c.tl; i < nl; i += TILE) {
             const double* t = W.tri + 9 * (int64_t)c.lst[i].leaf;
 #pragma unroll
