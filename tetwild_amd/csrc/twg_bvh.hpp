@@ -693,7 +693,12 @@ __device__ int nearest_facet_group(const BvhWalk& B, V3<double> p, V3<double>& n
         const double half = rr + 0x1p-20 * rr + eabs;
         const double lo[3] = {p.x - half, p.y - half, p.z - half};
         const double hi[3] = {p.x + half, p.y + half, p.z + half};
-        const int n = collect_facets<G>(B, lo, hi, lst, fr, cap, gl, gbase);
+        // listed from the global tree (~1 % faster than through an LDS copy
+        // of its top; the descents above read the copy)
+        BvhWalk GW = B;
+        GW.ncache = 0;
+        GW.top64 = false;
+        const int n = collect_facets<G>(GW, lo, hi, lst, fr, cap, gl, gbase);
         if (n < 0) break;
         // the lane's least distance below s0, its facet and how many reach it
         double bd = s0;
