@@ -4,6 +4,7 @@ build (_ref/libref_amips.so).
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the product package.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -30,50 +31,69 @@ def lib():
         path = os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(path)
-        sig = {
-            "orc_amips_energy": (c_dbl, [P]),
-            "orc_amips_jacobian": (None, [P, P]),
-            "orc_amips_hessian": (None, [P, P]),
-            "orc_amips_ejh": (c_dbl, [P, P, P]),
-            "orc_rcbrt": (c_dbl, [c_dbl]),
-            "orc_amips_batch": (None, [P, c_i64, P, P, P, c_int, P, P, P]),
-            "orc_orient3d": (c_int, [P, P, P, P]),
-            "orc_collinear3": (c_int, [P, P, P]),
-            "orc_tet_qualities": (None, [P, c_i64, P, P, P, P]),
-            "orc_tet_flip": (None, [P, c_i64, P, P, P, P]),
-            "orc_sample_triangle": (c_i64, [P, P, P, c_dbl, P, c_i64]),
-            "orc_point_tri_d2": (c_dbl, [P, P, P, P, P]),
-            "orc_surface_create": (P, [P, c_i64, P, c_i64]),
-            "orc_surface_destroy": (None, [P]),
-            "orc_point_out": (c_int, [P, P, c_dbl]),
-            "orc_face_out": (c_int, [P, P, c_dbl, c_dbl, P]),
-            "orc_face_out_batch": (None, [P, P, c_i64, c_dbl, c_dbl, P, P]),
-            "orc_point_out_batch": (None, [P, P, c_i64, c_dbl, P]),
-            "orc_nearest": (c_i64, [P, P, P, P]),
-            "orc_point_out_brute": (c_int, [P, P, c_dbl]),
-            "orc_min_d2_brute": (c_dbl, [P, P]),
-            "orc_newtons_method": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P]),
-            "orc_newtons_update": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P, P, P, P]),
-            "orc_new_energy": (c_dbl, [P, c_int, P, P, P]),
-            "orc_solve3_qrcp": (c_int, [P, P, P]),
-            "orc_smooth_problem": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P, c_int,
-                                           P, P, c_dbl, c_dbl, P, P]),
-            "orc_energy_stats": (None, [P, c_i64, P, P, P, c_dbl, P, P, P, P, P]),
-            "orc_filter_energy": (c_dbl, [P, c_dbl, P]),
-            "orc_tet_groups": (None, [P, P, c_i64, P, P, P, P, P, P]),
-            "orc_remove_edge": (c_int, [ctypes.c_int32, ctypes.c_int32, P, c_int, P, P, P, P, P,
-                                        P, P, P]),
-            "orc_collapse_check": (c_int, [ctypes.c_int32, ctypes.c_int32, c_int, P, c_int, P, P,
-                                           P, P, P, c_int, P, P, c_dbl, c_dbl, c_int, c_int,
-                                           c_int, c_dbl, P]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _LIB = L
+        _LIB = load(path)
     return _LIB
+
+
+@contextlib.contextmanager
+def swapped(path):
+    """Every binding of this module calls into the library at `path` (a build
+    of the oracle's sources, e.g. one with a planted mistake) inside the
+    block."""
+    global _LIB
+    old = lib()
+    _LIB = load(path)
+    try:
+        yield _LIB
+    finally:
+        _LIB = old
+
+
+def load(path):
+    """ctypes handle on an oracle build, with every signature declared."""
+    L = ctypes.CDLL(path)
+    sig = {
+        "orc_amips_energy": (c_dbl, [P]),
+        "orc_amips_jacobian": (None, [P, P]),
+        "orc_amips_hessian": (None, [P, P]),
+        "orc_amips_ejh": (c_dbl, [P, P, P]),
+        "orc_rcbrt": (c_dbl, [c_dbl]),
+        "orc_amips_batch": (None, [P, c_i64, P, P, P, c_int, P, P, P]),
+        "orc_orient3d": (c_int, [P, P, P, P]),
+        "orc_collinear3": (c_int, [P, P, P]),
+        "orc_tet_qualities": (None, [P, c_i64, P, P, P, P]),
+        "orc_tet_flip": (None, [P, c_i64, P, P, P, P]),
+        "orc_sample_triangle": (c_i64, [P, P, P, c_dbl, P, c_i64]),
+        "orc_point_tri_d2": (c_dbl, [P, P, P, P, P]),
+        "orc_surface_create": (P, [P, c_i64, P, c_i64]),
+        "orc_surface_destroy": (None, [P]),
+        "orc_point_out": (c_int, [P, P, c_dbl]),
+        "orc_face_out": (c_int, [P, P, c_dbl, c_dbl, P]),
+        "orc_face_out_batch": (None, [P, P, c_i64, c_dbl, c_dbl, P, P]),
+        "orc_point_out_batch": (None, [P, P, c_i64, c_dbl, P]),
+        "orc_nearest": (c_i64, [P, P, P, P]),
+        "orc_point_out_brute": (c_int, [P, P, c_dbl]),
+        "orc_min_d2_brute": (c_dbl, [P, P]),
+        "orc_newtons_method": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P]),
+        "orc_newtons_update": (c_int, [P, c_int, ctypes.c_int32, P, P, P, P, P, P, P]),
+        "orc_new_energy": (c_dbl, [P, c_int, P, P, P]),
+        "orc_solve3_qrcp": (c_int, [P, P, P]),
+        "orc_smooth_problem": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P, c_int,
+                                       P, P, c_dbl, c_dbl, P, P]),
+        "orc_energy_stats": (None, [P, c_i64, P, P, P, c_dbl, P, P, P, P, P]),
+        "orc_filter_energy": (c_dbl, [P, c_dbl, P]),
+        "orc_tet_groups": (None, [P, P, c_i64, P, P, P, P, P, P]),
+        "orc_remove_edge": (c_int, [ctypes.c_int32, ctypes.c_int32, P, c_int, P, P, P, P, P,
+                                    P, P, P]),
+        "orc_collapse_check": (c_int, [ctypes.c_int32, ctypes.c_int32, c_int, P, c_int, P, P,
+                                       P, P, P, c_int, P, P, c_dbl, c_dbl, c_int, c_int,
+                                       c_int, c_dbl, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def ref():

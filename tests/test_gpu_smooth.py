@@ -159,3 +159,29 @@ def test_autotune_and_empty(ctx):
     L = C.lib()
     assert L.twg_smooth_batch(ctx.h, None, 0, 0, 0) == C.TWG_EINVAL
     assert L.twg_smooth_autotune(ctx.h, None, 0, None, None, None) == C.TWG_EINVAL
+
+
+def test_envelope_rejection_batch(ctx):
+    """smoothSurface's one-ring face test (src/tetwild/VertexSmoother.cpp:
+    412-434) with an envelope tight enough (eps = 6e-5 on one-rings of 0.01)
+    that it rejects some moves: statuses (REJECT_ENVELOPE among them),
+    positions and qualities equal the oracle's for every tile, FP64 and mixed"""
+    V, F = W.bumpy_sphere(freq=24, seed=0)
+    sf = ctx.surface(V, F)
+    S = O.Surface(V, F[sf.order()])
+    B = W.smoothing_batch(120, seed=7, surface_frac=0.4, V=V, F=F, boundary_frac=0.3)
+    q = O.tet_qualities(B["ring_tets"], B["x"], B["y"], B["z"])
+    eps2, d = 6e-5 ** 2, 1e-3
+    ref = O.smooth_batch(B, q, S, S, eps2, d)
+    counts = np.bincount(ref[1], minlength=8)
+    print("statuses", counts)
+    assert counts[6] > 0 and ((ref[1] == 0) & (B["kind"] > 0)).any()
+    batch = SmoothBatch(ctx, B, sf=sf, bt=sf, eps2=eps2, sampling_dist=d, old_q=q)
+    for tile in (16, 32, 64):
+        for prec in (0, 1):
+            print(f"tile={tile} precision={prec}", flush=True)
+            batch.status.fill_(255)
+            batch.new_q.zero_()
+            batch.run(precision=prec, tile=tile, chunk=1)
+            torch.cuda.synchronize()
+            check(batch, ref, B)

@@ -5,12 +5,17 @@ is pinned by mathematics (numpy's LU solve, rank-deficient cases, the
 identity X = X0 - a H^-1 J); NewtonsUpdate is pinned against the reference's
 own compiled per-tet functions summed with the reference's rotation rule;
 NewtonsMethod by its invariants (accepted positions lower the ring energy,
-never flip a tet, and the is_moved semantics)."""
+never flip a tet, and the is_moved semantics); its control constants are
+pinned in test_oracle_newton.py.  The surface body's envelope test of the
+one-ring faces (src/tetwild/VertexSmoother.cpp:412-434) is pinned by a brute
+force over every facet at an envelope tight enough to reject."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from tetwild_amd import workloads as W
+
+from oracle_mutants import fails
 
 
 def test_solve3_qrcp():
@@ -122,3 +127,53 @@ def test_smooth_problem_statuses():
         assert d2[0] < 1e-16  # on the surface up to rounding (distance < 1e-8)
         r0, r1 = B["ring_off"][p], B["ring_off"][p + 1]
         assert new_q[r0:r1].max() < q[r0:r1].max()
+
+
+def check_envelope_rejection():
+    """Surface one-rings of 0.01 on a sphere of radius ~1 with eps = 6e-5:
+    faces through the projected vertex leave the envelope for some rings.
+    For every surface problem that reaches the face test, status
+    REJECT_ENVELOPE must be exactly 'some sample of some one-ring face
+    (projected vertex, a, b) lies farther than eps from every facet', decided
+    by brute force over all facets; a moved vertex sits at the projection."""
+    V, F = W.bumpy_sphere(freq=24, seed=0)
+    S = O.Surface(V, F)
+    B = W.smoothing_batch(120, seed=7, surface_frac=0.4, V=V, F=F, boundary_frac=0.3)
+    x, y, z = B["x"], B["y"], B["z"]
+    P = np.stack([x, y, z], 1)
+    q = O.tet_qualities(B["ring_tets"], x, y, z)
+    eps, d = 6e-5, 1e-3
+    p_out, status, new_q = O.smooth_batch(B, q, S, S, eps * eps, d)
+    surf = B["kind"] > 0
+    n_rej = int((status == 6).sum())
+    n_mov = int(((status == 0) & surf).sum())
+    assert n_rej >= 5 and n_mov >= 3, (n_rej, n_mov)
+    for p in np.nonzero(surf & np.isin(status, [0, 6]))[0]:
+        r0, r1 = B["ring_off"][p], B["ring_off"][p + 1]
+        rt, v = B["ring_tets"][r0:r1], int(B["center"][p])
+        moved, pn = O.newtons_method(rt, v, x, y, z)
+        assert moved
+        cp, _, _ = S.nearest(pn[None])
+        pf = cp[0]
+        out = False
+        for a, b in B["tri_ab"][B["tri_off"][p]:B["tri_off"][p + 1]]:
+            if O.collinear3(pf, P[a], P[b]):
+                continue
+            ps = O.sample_triangle(pf, P[a], P[b], d)
+            out = out or bool(S.point_out_brute(ps, eps * eps).any())
+        assert out == (status[p] == 6), (p, status[p])
+        if status[p] == 0:
+            assert np.array_equal(p_out[p], pf)
+    return n_rej
+
+
+def test_envelope_rejection_branch():
+    assert check_envelope_rejection() > 0
+
+
+@pytest.mark.parametrize("name,old,new", [
+    ("no face test", "if (orc_face_out(sf, tri, eps2, d, NULL)) return ORC_SMOOTH_REJECT_ENVELOPE;",
+     "if (0 && orc_face_out(sf, tri, eps2, d, NULL)) return ORC_SMOOTH_REJECT_ENVELOPE;"),
+], ids=["no face test"])
+def test_envelope_rejection_catches_mutation(tmp_path, name, old, new):
+    assert fails(check_envelope_rejection, tmp_path, "smooth.c", old, new), name
