@@ -57,7 +57,10 @@ int twg_ctx_create(int device, twg_ctx** out);
 int twg_ctx_destroy(twg_ctx* ctx);
 /* Enqueue later calls on an external HIP stream (hipStream_t; NULL is the HIP
  * null stream).  twg_ctx_stream() returns the stream in use (initially the
- * context's own). */
+ * context's own).  The context's scratch is shared by all its calls, so the
+ * new stream is made to wait (an event, no host wait) for the work already
+ * enqueued on the old one -- unless either stream is capturing a graph, in
+ * which case ordering the two is the caller's. */
 int twg_ctx_set_stream(twg_ctx* ctx, void* hip_stream);
 void* twg_ctx_stream(twg_ctx* ctx);
 int twg_ctx_sync(twg_ctx* ctx);

@@ -77,7 +77,9 @@ int twg_ctx_create(int device, twg_ctx** out)
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->order, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->own) (void)hipStreamDestroy(c->own);
         delete c;
         return twg::hip_fail(e, "twg_ctx_create", __FILE__, __LINE__);
     }
@@ -96,6 +98,7 @@ int twg_ctx_destroy(twg_ctx* ctx)
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (int i = 0; i < 4; ++i)
         if (ctx->slot[i]) (void)hipFree(ctx->slot[i]);
+    if (ctx->order) (void)hipEventDestroy(ctx->order);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return TWG_OK;
@@ -104,7 +107,21 @@ int twg_ctx_destroy(twg_ctx* ctx)
 int twg_ctx_set_stream(twg_ctx* ctx, void* s)
 {
     TWG_REQUIRE(ctx != nullptr, "twg_ctx_set_stream: ctx is NULL");
-    ctx->cur = (hipStream_t)s;
+    hipStream_t ns = (hipStream_t)s;
+    if (ns == ctx->cur) return TWG_OK;
+    if (int r = twg::bind(ctx)) return r;
+    // Every call on the context shares its scratch: the new stream waits for
+    // what the old one has enqueued.  A stream that is capturing a graph
+    // takes no part in that (an event wait across a capture boundary is not
+    // a legal capture); the capturing caller orders its own streams.
+    hipStreamCaptureStatus a = hipStreamCaptureStatusNone, b = hipStreamCaptureStatusNone;
+    TWG_HIP(hipStreamIsCapturing(ctx->cur, &a));
+    TWG_HIP(hipStreamIsCapturing(ns, &b));
+    if (a == hipStreamCaptureStatusNone && b == hipStreamCaptureStatusNone) {
+        TWG_HIP(hipEventRecord(ctx->order, ctx->cur));
+        TWG_HIP(hipStreamWaitEvent(ns, ctx->order, 0));
+    }
+    ctx->cur = ns;
     return TWG_OK;
 }
 

@@ -690,20 +690,19 @@ int twg_collapse_autotune(twg_ctx* ctx, const twg_collapse_desc* d, int precisio
     int bt = default_tile(A);
     unsigned bc = 1;
     if (A.n > 0) {
-        hipEvent_t e0, e1;
-        TWG_HIP(hipEventCreate(&e0));
-        TWG_HIP(hipEventCreate(&e1));
+        EventPair ev;
+        TWG_HIP(ev.create());
         for (int t : tiles)
             for (unsigned ch : chunks) {
                 A.chunk = ch;
                 if ((r = enqueue(ctx, A, t, f32, ctx->cur))) return r;  // warm
-                TWG_HIP(hipEventRecord(e0, ctx->cur));
+                TWG_HIP(hipEventRecord(ev.e0, ctx->cur));
                 for (int rep = 0; rep < 2; ++rep)
                     if ((r = enqueue(ctx, A, t, f32, ctx->cur))) return r;
-                TWG_HIP(hipEventRecord(e1, ctx->cur));
-                TWG_HIP(hipEventSynchronize(e1));
+                TWG_HIP(hipEventRecord(ev.e1, ctx->cur));
+                TWG_HIP(hipEventSynchronize(ev.e1));
                 float ms = 0;
-                TWG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                TWG_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
                 ms /= 2;
                 if (ms < best) {
                     best = ms;
@@ -711,8 +710,6 @@ int twg_collapse_autotune(twg_ctx* ctx, const twg_collapse_desc* d, int precisio
                     bc = ch;
                 }
             }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         g_tuned.put(ctx->device, f32, size_class(A.n), bt, bc);
     }
     if (tile_out) *tile_out = bt;

@@ -899,9 +899,8 @@ int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* d, int precision, i
     if (int r = make_args(ctx, d, (unsigned*)ctx->scratch, 1, A)) return r;
     const int tiles[3] = {16, 32, 64};
     const unsigned chunks[3] = {1, 4, 16};
-    hipEvent_t e0, e1;
-    TWG_HIP(hipEventCreate(&e0));
-    TWG_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    TWG_HIP(ev.create());
     float best = 1e30f;
     int bt = 32;
     unsigned bc = 1;
@@ -909,13 +908,13 @@ int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* d, int precision, i
         for (unsigned ch : chunks) {
             A.chunk = ch;
             if (int r = enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur)) return r;  // warm
-            TWG_HIP(hipEventRecord(e0, ctx->cur));
+            TWG_HIP(hipEventRecord(ev.e0, ctx->cur));
             for (int rep = 0; rep < 2; ++rep)
                 if (int r = enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur)) return r;
-            TWG_HIP(hipEventRecord(e1, ctx->cur));
-            TWG_HIP(hipEventSynchronize(e1));
+            TWG_HIP(hipEventRecord(ev.e1, ctx->cur));
+            TWG_HIP(hipEventSynchronize(ev.e1));
             float ms = 0;
-            TWG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            TWG_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
             ms /= 2;
             if (ms < best) {
                 best = ms;
@@ -923,8 +922,6 @@ int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* d, int precision, i
                 bc = ch;
             }
         }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     g_tuned.put(ctx->device, precision == TWG_MIXED, size_class(d->n), bt, bc);
     if (tile_out) *tile_out = bt;
     if (chunk_out) *chunk_out = (int)bc;

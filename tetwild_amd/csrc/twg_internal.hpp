@@ -17,6 +17,9 @@ struct twg_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t cur = nullptr;
+    // recorded on the old stream when the caller switches streams, so work on
+    // the new one starts after the old one's use of the shared scratch
+    hipEvent_t order = nullptr;
     int num_cus = 256;
     // growable device scratch owned by the context
     void* scratch = nullptr;
@@ -72,6 +75,23 @@ public:
 private:
     std::mutex m_;
     std::map<std::tuple<int, bool, int>, std::pair<int, unsigned>> map_;
+};
+
+// The two timing events of an autotuner, destroyed on every path out of the
+// scope that owns them (early error returns included).
+struct EventPair {
+    hipEvent_t e0 = nullptr;
+    hipEvent_t e1 = nullptr;
+    hipError_t create()
+    {
+        hipError_t e = hipEventCreate(&e0);
+        return e == hipSuccess ? hipEventCreate(&e1) : e;
+    }
+    ~EventPair()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
 };
 
 inline unsigned grid_for(int64_t n, int block = kBlock)

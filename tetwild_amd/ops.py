@@ -59,6 +59,28 @@ class Context:
                 raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shape}")
         return t
 
+    def _amips_out(self, out, n, mode, gdtype, mixed):
+        """caller-supplied outputs of amips_eval / amips_eval_mixed: E (n)
+        float64, G (n, 3) and H (n, 9) of gdtype where the mode writes them,
+        fallback (n) uint8 for the mixed form; checked like the inputs, since
+        the library writes exactly n rows into each"""
+        want = 4 if mixed else 3
+        if not isinstance(out, (tuple, list)) or len(out) != want:
+            raise ValueError(f"out must be a tuple of {want} tensors")
+        E, G, H = out[0], out[1], out[2]
+        self._dev(E, torch.float64, "out E", (n,))
+        if mode >= C.TWG_AMIPS_EG:
+            if G is None:
+                raise ValueError("out G is required for this mode")
+            self._dev(G, gdtype, "out G", (n, 3))
+        if mode >= C.TWG_AMIPS_EGH:
+            if H is None:
+                raise ValueError("out H is required for this mode")
+            self._dev(H, gdtype, "out H", (n, 9))
+        if mixed and out[3] is not None:
+            self._dev(out[3], torch.uint8, "out fallback", (n,))
+        return out
+
     def _verts(self, x, y, z):
         self._dev(x, torch.float64, "x", (None,))
         self._dev(y, torch.float64, "y", (x.shape[0],))
@@ -85,7 +107,7 @@ class Context:
             G = torch.empty((n, 3), dtype=torch.float64, device=dev) if mode >= 1 else None
             H = torch.empty((n, 9), dtype=torch.float64, device=dev) if mode >= 2 else None
         else:
-            E, G, H = out
+            E, G, H = self._amips_out(out, n, mode, torch.float64, False)
         self._bind()
         C.check(self.L.twg_amips_eval(self.h, _ptr(tets), n, _ptr(x), _ptr(y), _ptr(z), nv, mode,
                                       _ptr(E), _ptr(G), _ptr(H)), "twg_amips_eval")
@@ -96,7 +118,7 @@ class Context:
         checked once (for timing loops: no per-call Python validation)."""
         n = self._tets(tets)
         nv = self._verts(x, y, z)
-        E, G, H = out
+        E, G, H = self._amips_out(out, n, mode, torch.float64, False)
         self._bind()
         f = self.L.twg_amips_eval
         args = (self.h, _ptr(tets), n, _ptr(x), _ptr(y), _ptr(z), nv, mode, _ptr(E), _ptr(G),
@@ -119,7 +141,7 @@ class Context:
             H = torch.empty((n, 9), dtype=torch.float32, device=dev) if mode >= 2 else None
             fb = torch.empty(n, dtype=torch.uint8, device=dev)
         else:
-            E, G, H, fb = out
+            E, G, H, fb = self._amips_out(out, n, mode, torch.float32, True)
         self._bind()
         C.check(self.L.twg_amips_eval_mixed(self.h, _ptr(tets), n, _ptr(x), _ptr(y), _ptr(z), nv,
                                             mode, float(tol), _ptr(E), _ptr(G), _ptr(H), _ptr(fb)),
@@ -206,6 +228,8 @@ class Context:
         n = tri9.shape[0]
         if out is None:
             out = torch.empty(n, dtype=torch.uint8, device=self.device)
+        else:
+            self._dev(out, torch.uint8, "out", (n,))
         st = torch.zeros(4, dtype=torch.int64, device=self.device) if stats else None
         self._bind()
         C.check(self.L.twg_envelope_faces(self.h, surf.h, _ptr(tri9), n, float(eps2),
