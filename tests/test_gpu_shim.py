@@ -117,7 +117,7 @@ def replay(P, tets, fs, flags, q, S, eps, d, batched):
         p, s, nq = O.smooth_batch(B, q[ring], S, S, eps * eps, d)
         return ring, p[0], s[0], nq
 
-    interior = [v for v in range(nv) if not flags[v] & 1]
+    interior = [v for v in range(nv) if not flags[v] & 3]
     groups = colour_classes(interior, C, tets) if batched else [[v] for v in interior]
     for grp in groups:
         res = [one(0, v) for v in grp]  # independent within a group: frozen positions
@@ -173,7 +173,7 @@ def test_shim_smooth_matches_replay(ctx, tmp_path, mode):
     # NewtonsUpdate / getNewEnergy records and the static energy function
     rec = raw[3 * nv + nt + 2:-1].reshape(-1, 15)
     C = conn(tets, nv)
-    interior = [v for v in range(nv) if not flags[v] & 1][:len(rec)]
+    interior = [v for v in range(nv) if not flags[v] & 3][:len(rec)]
     x, y, z = (np.ascontiguousarray(Pg[:, k]) for k in range(3))
     for v, row in zip(interior, rec):
         ok, E, J, H, X0 = O.newtons_update(tets[C[v]], v, x, y, z)

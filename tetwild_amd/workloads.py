@@ -541,6 +541,71 @@ def sheet_mesh(n, jitter=0.15, surf_jitter=0.02, seed=0, level=None):
             "b_V": V, "b_F": b_F}
 
 
+def icosphere_tetmesh(levels=4, sigma=0.002, seed=0, eps_rel=1e-3, edge_rel=1.0 / 20):
+    """The end-to-end CPU workload's input surface -- a subdivided icosphere
+    (levels=4: 2562 vertices, 5120 triangles, radius 1) jittered radially by
+    N(0, sigma) -- embedded in a tet mesh of its bounding region, standing in
+    for the stage-2 mesh MeshRefinement::refine works on.
+
+    Tolerances follow State (src/tetwild/State.cpp:19-46) with the default
+    Args (include/tetwild/Args.h:21,43,54: eps_rel 1e-3, edge 1/20, stage 1):
+    eps_input = eps_rel * bbox diagonal, sampling_dist = eps_input, eps =
+    eps_input * (1 - 1/sqrt(3)), ideal edge length L = edge_rel * diagonal.
+
+    The tet mesh is the Delaunay tetrahedralization (scipy) of the surface
+    vertices and a jittered cubic lattice of spacing L over [-1.2, 1.2]^3
+    without the points within L/2 of the unit sphere; every input triangle is
+    a face of it (asserted).  Tets are positively oriented (2 <-> 3 swapped
+    where needed); faces on the input surface have is_surface_fs 0; the
+    hull's vertices are the bbox vertices.  The surface is closed, so the
+    boundary tree is getSimpleMesh's single degenerate triangle at the origin
+    (src/tetwild/MeshRefinement.cpp:907-919).
+
+    Returns a dict in sheet_mesh's layout plus on_bbox, eps_input, eps, eps_2,
+    sampling_dist, edge_len and diag."""
+    from scipy.spatial import Delaunay
+    V, F = icosphere_jittered(levels, sigma, seed)
+    diag = float(np.linalg.norm(V.max(0) - V.min(0)))
+    L = edge_rel * diag
+    st = Stream(seed, 60)
+    g = np.arange(-1.2, 1.2 + 1e-9, 0.999 * L)
+    G = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    G = G + st.normal(G.size).reshape(-1, 3) * (0.1 * L)
+    r = np.linalg.norm(G, axis=1)
+    G = G[(r < 1 - 0.5 * L) | (r > 1 + 0.5 * L)]
+    P = np.ascontiguousarray(np.concatenate([V, G]))
+    D = Delaunay(P)
+    tets = D.simplices.astype(np.int32)
+    x, y, z = (np.ascontiguousarray(P[:, k]) for k in range(3))
+    bad = orientation(tets, x, y, z) <= 0
+    tets[bad] = tets[bad][:, [0, 1, 3, 2]]
+    assert (orientation(tets, x, y, z) > 0).all()
+    nv = len(P)
+    fkey = lambda f: (f[:, 0] * nv + f[:, 1]) * nv + f[:, 2]  # noqa: E731
+    skey = np.sort(fkey(np.sort(F.astype(np.int64), 1)))
+    fs = np.full(tets.shape, NOT_SURFACE, dtype=np.int32)
+    hit = np.zeros(len(skey), bool)
+    for j in range(4):
+        f = np.sort(np.stack([tets[:, (j + k) % 4] for k in (1, 2, 3)], 1).astype(np.int64), 1)
+        k = fkey(f)
+        pos = np.clip(np.searchsorted(skey, k), 0, len(skey) - 1)
+        m = skey[pos] == k
+        fs[m, j] = 0
+        hit[pos[m]] = True
+    assert hit.all(), "an input triangle is not a face of the tet mesh"
+    on_surface = np.zeros(nv, bool)
+    on_surface[:len(V)] = True
+    on_bbox = np.zeros(nv, bool)
+    on_bbox[np.unique(D.convex_hull)] = True
+    eps_input = eps_rel * diag
+    eps = eps_input * (1 - 1 / math.sqrt(3))
+    return {"x": x, "y": y, "z": z, "tets": np.ascontiguousarray(tets), "is_surface_fs": fs,
+            "on_surface": on_surface, "on_boundary": np.zeros(nv, bool), "on_bbox": on_bbox,
+            "sf_V": V, "sf_F": F, "b_V": np.zeros((1, 3)), "b_F": np.zeros((1, 3), np.int32),
+            "eps_input": eps_input, "eps": eps, "eps_2": eps * eps, "sampling_dist": eps_input,
+            "edge_len": L, "diag": diag}
+
+
 def vertex_tets(tets, nv):
     """conn_tets as CSR (off [nv + 1], tet ids ascending per vertex)"""
     inc_v = tets.reshape(-1).astype(np.int64)
