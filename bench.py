@@ -25,8 +25,11 @@ Secondary measurements in the same JSON line ("secondary"):
                  and mixed) and isFlip (twg_tet_flip) over the 10.1M lattice
                  tets, and the configuration check (twg_tet_groups) over
                  2.5M groups of four tets
-Each carries its CPU baseline (the CPU restatement, one host core, bounded
-sample) and its parity counts against that restatement.
+Each carries its CPU baseline (the reference's compiled AMIPS functions, or
+the CPU restatement, over every host thread OpenMP is given, with the
+one-core rate beside it; bounded samples), its parity counts against that
+restatement and, for the envelope and the smoothing line search, the counts
+of decisions resting on a near-tie.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
                   [--only amips,envelope,slivers,smoothing,collapse,reduce]
@@ -55,6 +58,31 @@ BYTES_PER_TET = 120
 BYTES_PER_VERTEX = 24
 # envelope: 72 B query + 1 B result; 48 B per box and 72 B per facet tested
 ENV_QUERY_BYTES, ENV_BOX_BYTES, ENV_LEAF_BYTES = 73, 48, 72
+
+
+def host_cpu():
+    """the host threads the CPU baseline uses (OMP_NUM_THREADS when set, else
+    every CPU this process may run on) and the CPU model"""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": n, "model": model}
+
+
+def cpu_entry(value, value_1core, unit, kind, sample, host):
+    """a cpu_baseline object: all host threads, with the one-core rate beside it"""
+    return {"value": value, "unit": unit, "cores": host["threads"], "kind": kind,
+            "sample": sample, "value_1core": value_1core, "nproc": host["nproc"],
+            "cpu_model": host["model"]}
 
 
 def log(msg):
@@ -179,17 +207,31 @@ def cpu_baseline_amips(x, y, z, tets, E_gpu, H_gpu, budget_s=12.0):
         el = time.perf_counter() - t0
         if el > budget_s or reps >= 20:
             break
+    one_core = n * reps / el
+    # all host threads over the same sample
+    host = host_cpu()
+    reps_mt = 0
+    t0 = time.perf_counter()
+    while True:
+        if kind == "reference":
+            O.ref().ref_amips_batch_ejh_mt(O._p(T), n, O._p(E), O._p(G), O._p(H), host["threads"])
+        else:
+            O.lib().orc_amips_batch_mt(O._p(np.ascontiguousarray(t)), n, O._p(x), O._p(y),
+                                       O._p(z), 2, O._p(E), O._p(G), O._p(H), host["threads"])
+        reps_mt += 1
+        el_mt = time.perf_counter() - t0
+        if el_mt > budget_s / 2 or reps_mt >= 40:
+            break
     Eo, Go, Ho = O.amips_batch(t, x, y, z, 2)
     ref_rel = None
     if kind == "reference":
         ok = np.isfinite(E)
         ref_rel = float(np.max(np.abs(E_gpu[:n][ok] - E[ok]) / np.abs(E[ok])))
-    return {
-        "value": n * reps / el, "unit": "tet-evals/s", "cores": 1, "kind": kind,
-        "sample": f"first {n} lattice tets x {reps} passes, E+grad+Hessian, "
-                  + ("reference comformalAMIPS*_new (g++ -O3)" if kind == "reference"
-                     else "CPU restatement"),
-    }, {"bit_mismatch_E": bits_mismatch(E_gpu[:n], Eo),
+    return cpu_entry(
+        n * reps_mt / el_mt, one_core, "tet-evals/s", kind,
+        f"first {n} lattice tets x {reps_mt} passes ({reps} on one core), E+grad+Hessian, "
+        + ("reference comformalAMIPS*_new (g++ -O3)" if kind == "reference" else "CPU restatement"),
+        host), {"bit_mismatch_E": bits_mismatch(E_gpu[:n], Eo),
         "bit_mismatch_H": bits_mismatch(H_gpu[:n], Ho), "sample_tets": n,
         "max_rel_err_vs_reference_E": ref_rel}
 
@@ -240,13 +282,25 @@ def bench_envelope(ctx, args, result, surf_cache):
         log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         S = O.Surface(V, F)
+        host = host_cpu()
         k = 20000 if args.quick else 200000
         t0 = time.perf_counter()
         ref, _ = S.face_out(Q[:k], eps * eps, eps)
         el = time.perf_counter() - t0
-        sec["cpu_baseline"] = {"value": k / el, "unit": "checks/s", "cores": 1, "kind": "port",
-                               "sample": f"first {k} queries, CPU restatement (tree)"}
-        sec["decision_mismatch_vs_cpu"] = int(np.count_nonzero(ref != out["fp64"][:k]))
+        km = 10 * k
+        t0 = time.perf_counter()
+        ref_mt = S.face_out_mt(Q[:km], eps * eps, eps, host["threads"])
+        el_mt = time.perf_counter() - t0
+        sec["cpu_baseline"] = cpu_entry(
+            km / el_mt, k / el, "checks/s", "port",
+            f"first {km} queries on all threads ({k} on one core), CPU restatement (tree)", host)
+        sec["decision_mismatch_vs_cpu"] = int(np.count_nonzero(ref_mt != out["fp64"][:km]))
+        # decisions resting on a sample within 4 ulp(eps^2) of eps^2 (the
+        # reference could decide those the other way)
+        band = 4 * float(np.spacing(eps * eps))
+        r = S.face_risk(Q[:k], eps * eps, eps, band, host["threads"])
+        sec["near_tie"] = {"queries": k, "band": band, "samples": r[0], "samples_in_band": r[1],
+                           "queries_with_band_sample": r[2], "queries_decided_in_band": r[3]}
     result["secondary"]["envelope"] = sec
 
 
@@ -309,9 +363,14 @@ def bench_slivers(ctx, args, result, surf_cache):
         log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         S = O.Surface(V, F)
-        k = 200000
-        ref = S.point_out(P[:k], eps * eps)
+        host = host_cpu()
+        k = 2_000_000
+        ref = S.point_out_mt(P[:k], eps * eps, host["threads"])
         sec["point_decision_mismatch_vs_cpu"] = int(np.count_nonzero(ref != o64[:k]))
+        sec["point_cpu_checked"] = k
+        band = 4 * float(np.spacing(eps * eps))
+        r = S.point_risk(P[:k], eps * eps, band, host["threads"])
+        sec["point_near_tie"] = {"points": r[0], "band": band, "points_in_band": r[1]}
     result["secondary"]["mixed_slivers"] = sec
 
 
@@ -372,13 +431,27 @@ def bench_smoothing(ctx, args, result, surf_cache):
         log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
         S = O.Surface(V, F[surf.order()])
+        host = host_cpu()
         t0 = time.perf_counter()
-        rp, rs, rq = O.smooth_batch(sub, old_q, S, S, eps * eps, eps)
+        O.smooth_batch_mt(sub, old_q, S, S, eps * eps, eps, nthreads=1)
         el = time.perf_counter() - t0
-        sec["cpu_baseline"] = {"value": k / el, "unit": "problems/s", "cores": 1, "kind": "port",
-                               "sample": f"first {k} problems, CPU restatement"}
-        sec["status_mismatch_vs_cpu"] = int(np.count_nonzero(rs != st[:k]))
-        sec["position_bit_mismatch_vs_cpu"] = bits_mismatch(p_gpu[:k], rp)
+        km = 20000 if args.quick else 100_000
+        sub = {kk: (v[:km] if kk in ("center", "kind") else v) for kk, v in B.items()}
+        sub["ring_off"] = B["ring_off"][:km + 1]
+        sub["tri_off"] = B["tri_off"][:km + 1]
+        t0 = time.perf_counter()
+        rp, rs, rq, risk = O.smooth_batch_mt(sub, old_q, S, S, eps * eps, eps, host["threads"])
+        el_mt = time.perf_counter() - t0
+        sec["cpu_baseline"] = cpu_entry(
+            km / el_mt, k / el, "problems/s", "port",
+            f"first {km} problems on all threads ({k} on one core), CPU restatement", host)
+        sec["status_mismatch_vs_cpu"] = int(np.count_nonzero(rs != st[:km]))
+        sec["position_bit_mismatch_vs_cpu"] = bits_mismatch(p_gpu[:km], rp)
+        # line-search comparisons within the summation band of a tie (the
+        # reference sums rings in unordered_set order)
+        sec["near_tie"] = {"problems": km, "energy_compares": risk[0],
+                           "energy_compares_in_band": risk[1], "threshold_tests": risk[2],
+                           "threshold_tests_in_band": risk[3]}
     result["secondary"]["smoothing"] = sec
 
 
@@ -434,18 +507,27 @@ def bench_collapse(ctx, args, result):
     if not args.no_cpu:
         log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
-        k = 20000 if args.quick else 200_000
-        sub = dict(B)
-        for key in ("v1", "v2", "flags"):
-            sub[key] = B[key][:k]
-        sub["ring_off"] = B["ring_off"][:k + 1]
-        sub["tri_off"] = B["tri_off"][:k + 1]
+        host = host_cpu()
         S, Sb = O.Surface(M["sf_V"], M["sf_F"]), O.Surface(M["b_V"], M["b_F"])
+
+        def first(k):
+            sub = dict(B)
+            for key in ("v1", "v2", "flags"):
+                sub[key] = B[key][:k]
+            sub["ring_off"] = B["ring_off"][:k + 1]
+            sub["tri_off"] = B["tri_off"][:k + 1]
+            return sub
+        k1 = 20000 if args.quick else 200_000
         t0 = time.perf_counter()
-        so, nqo = O.collapse_batch(sub, S, Sb, eps * eps, eps)
+        O.collapse_batch_mt(first(k1), S, Sb, eps * eps, eps, nthreads=1)
         el = time.perf_counter() - t0
-        sec["cpu_baseline"] = {"value": k / el, "unit": "candidates/s", "cores": 1,
-                               "kind": "port", "sample": f"first {k} candidates, CPU restatement"}
+        k = n
+        t0 = time.perf_counter()
+        so, nqo = O.collapse_batch_mt(first(k), S, Sb, eps * eps, eps, nthreads=host["threads"])
+        el_mt = time.perf_counter() - t0
+        sec["cpu_baseline"] = cpu_entry(
+            k / el_mt, k1 / el, "candidates/s", "port",
+            f"all {k} candidates on all threads (first {k1} on one core), CPU restatement", host)
         sec["status_mismatch_vs_cpu"] = int(np.count_nonzero(so != st[:k]))
         ok = np.isin(so, [0, 4])
         rows = np.repeat(ok, np.diff(B["ring_off"][:k + 1]))
@@ -490,7 +572,8 @@ def bench_reduce(ctx, args, result):
         so = O.energy_stats(q.cpu().numpy(), removed.cpu().numpy(), tets, locked.cpu().numpy())
         el = time.perf_counter() - t0
         sec["cpu_baseline"] = {"value": n / el, "unit": "tets/s", "cores": 1, "kind": "port",
-                               "sample": f"all {n} tets, CPU restatement"}
+                               "sample": f"all {n} tets, CPU restatement (one core: the sum "
+                                         "runs in index order, as the reference's loop)"}
         sec["max_bucket_count_mismatch_vs_cpu"] = int(
             (s[1] != so[1]) + (s[2] != so[2]) + (s[3] != so[3]) + int(list(s[4]) != list(so[4])))
         sec["sum_rel_diff_vs_cpu"] = abs(s[0] - so[0]) / abs(so[0])
@@ -513,13 +596,20 @@ def bench_reduce(ctx, args, result):
     if not args.no_cpu:
         log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
+        host = host_cpu()
         kq = 2_000_000
         t0 = time.perf_counter()
-        qo = O.tet_qualities(tets[:kq], x, y, z)
+        O.tet_qualities(tets[:kq], x, y, z)
         el = time.perf_counter() - t0
-        sec["qualities_cpu_baseline"] = {"value": kq / el, "unit": "tets/s", "cores": 1, "kind": "port",
-                                         "sample": f"first {kq} lattice tets, CPU restatement"}
-        sec["qualities_bit_mismatch_vs_cpu"] = bits_mismatch(q64[:kq].cpu().numpy(), qo)
+        qo = np.empty(n)
+        t0 = time.perf_counter()
+        O.lib().orc_tet_qualities_mt(O._p(tets), n, O._p(x), O._p(y), O._p(z), O._p(qo),
+                                     host["threads"])
+        el_mt = time.perf_counter() - t0
+        sec["qualities_cpu_baseline"] = cpu_entry(
+            n / el_mt, kq / el, "tets/s", "port",
+            f"all {n} lattice tets on all threads (first {kq} on one core), CPU restatement", host)
+        sec["qualities_bit_mismatch_vs_cpu"] = bits_mismatch(q64.cpu().numpy(), qo)
     g = n // 4
     off = torch.arange(0, 4 * g + 1, 4, dtype=torch.int32, device=ctx.device)
     groups = tt[:4 * g].contiguous()

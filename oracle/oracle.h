@@ -148,6 +148,16 @@ int orc_smooth_problem(int kind, int32_t v, const int32_t* ring_tets, int m, con
                        const orc_surface* bt, double eps2, double d, double* p_out,
                        double* new_q);
 
+/* orc_smooth_problem with near-tie accounting of its line search: risk[0..3]
+ * are incremented (energy comparisons, those within the summation band of a
+ * tie, |dE| < 1e-5 tests, those within the band of 1e-5; smooth.c states the
+ * band).  risk may be NULL. */
+int orc_smooth_problem_risk(int kind, int32_t v, const int32_t* ring_tets, int m,
+                            const double* x, const double* y, const double* z,
+                            const double* old_q, const int32_t* tri_ab, int ntri,
+                            const orc_surface* sf, const orc_surface* bt, double eps2, double d,
+                            double* p_out, double* new_q, int64_t* risk);
+
 /* ---- edge-collapse candidates, src/tetwild/EdgeCollapser.cpp:284-446 ---- */
 /* return codes: the reference's (src/tetwild/EdgeCollapser.h:52-56) */
 #define ORC_COLLAPSE_SUCCESS 0
@@ -197,6 +207,49 @@ void orc_tet_groups(const int32_t* tets, const int32_t* off, int64_t n_groups, c
 int orc_remove_edge(int32_t v1, int32_t v2, const int32_t* ring, int m, const double* old_q,
                     const double* x, const double* y, const double* z, int* equal_buget,
                     int32_t* new_tets, double* new_q, int* n_new);
+
+/* ---- CPU baseline: the batch loops above over host threads (baseline.c) --
+ * OpenMP, schedule(dynamic); nthreads <= 0 uses every thread OpenMP offers.
+ * Results are identical to the single-threaded loops (every item is
+ * independent and computed by the same code). */
+int orc_max_threads(void);
+void orc_amips_batch_mt(const int32_t* tets, int64_t n, const double* x, const double* y,
+                        const double* z, int mode, double* E, double* G, double* H,
+                        int nthreads);
+void orc_tet_qualities_mt(const int32_t* tets, int64_t n, const double* x, const double* y,
+                          const double* z, double* slim_energy, int nthreads);
+void orc_face_out_batch_mt(const orc_surface* s, const double* tri9, int64_t n, double eps2,
+                           double d, uint8_t* out, int nthreads);
+void orc_point_out_batch_mt(const orc_surface* s, const double* p3, int64_t n, double eps2,
+                            uint8_t* out, int nthreads);
+/* n one-ring problems (workloads.smoothing_batch layout); new_q is written
+ * for the ring of every surface problem that moved; risk (4 counters, may be
+ * NULL) sums orc_smooth_problem_risk's */
+void orc_smooth_batch_mt(int64_t n, const uint8_t* kind, const int32_t* center,
+                         const int32_t* ring_off, const int32_t* ring_tets, const double* x,
+                         const double* y, const double* z, const double* old_q,
+                         const int32_t* tri_off, const int32_t* tri_ab, const orc_surface* sf,
+                         const orc_surface* bt, double eps2, double d, double* p_out,
+                         uint8_t* status, double* new_q, int64_t* risk, int nthreads);
+/* n collapse candidates (workloads.collapse_batch layout) */
+void orc_collapse_batch_mt(int64_t n, const int32_t* v1, const int32_t* v2, const uint8_t* flags,
+                           const int32_t* ring_off, const int32_t* ring_tets, const double* x,
+                           const double* y, const double* z, const double* old_q,
+                           const int32_t* tri_off, const int32_t* tri_ids,
+                           const orc_surface* sf, const orc_surface* bt, double eps2, double d,
+                           int check_quality, int check_eps, int is_soft, double soft_energy,
+                           uint8_t* status, double* new_q, int nthreads);
+/* Near-tie accounting for isFaceOutEnvelop over n triangles: every sample's
+ * nearest squared distance (a full nearest search) against eps2 +- band.
+ * counts[0] samples, [1] samples with |d2 - eps2| <= band, [2] triangles with
+ * such a sample, [3] triangles whose decision rests on the band (no sample
+ * with d2 > eps2 + band, and one inside the band). */
+void orc_face_risk_batch_mt(const orc_surface* s, const double* tri9, int64_t n, double eps2,
+                            double d, double band, int64_t* counts, int nthreads);
+/* the same for isPointOutEnvelop: counts[0] points, [1] points with
+ * |d2 - eps2| <= band (each such decision rests on the band) */
+void orc_point_risk_batch_mt(const orc_surface* s, const double* p3, int64_t n, double eps2,
+                             double band, int64_t* counts, int nthreads);
 
 #ifdef __cplusplus
 }

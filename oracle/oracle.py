@@ -88,6 +88,19 @@ def load(path):
         "orc_collapse_check": (c_int, [ctypes.c_int32, ctypes.c_int32, c_int, P, c_int, P, P,
                                        P, P, P, c_int, P, P, c_dbl, c_dbl, c_int, c_int,
                                        c_int, c_dbl, P]),
+        "orc_smooth_problem_risk": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P,
+                                            c_int, P, P, c_dbl, c_dbl, P, P, P]),
+        "orc_max_threads": (c_int, []),
+        "orc_amips_batch_mt": (None, [P, c_i64, P, P, P, c_int, P, P, P, c_int]),
+        "orc_tet_qualities_mt": (None, [P, c_i64, P, P, P, P, c_int]),
+        "orc_face_out_batch_mt": (None, [P, P, c_i64, c_dbl, c_dbl, P, c_int]),
+        "orc_point_out_batch_mt": (None, [P, P, c_i64, c_dbl, P, c_int]),
+        "orc_smooth_batch_mt": (None, [c_i64, P, P, P, P, P, P, P, P, P, P, P, P, c_dbl, c_dbl,
+                                       P, P, P, P, c_int]),
+        "orc_collapse_batch_mt": (None, [c_i64, P, P, P, P, P, P, P, P, P, P, P, P, P, c_dbl,
+                                         c_dbl, c_int, c_int, c_int, c_dbl, P, P, c_int]),
+        "orc_face_risk_batch_mt": (None, [P, P, c_i64, c_dbl, c_dbl, c_dbl, P, c_int]),
+        "orc_point_risk_batch_mt": (None, [P, P, c_i64, c_dbl, c_dbl, P, c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -112,6 +125,7 @@ def ref():
         L.ref_amips_hessian.argtypes = [P, P]
         L.ref_amips_batch_ejh.argtypes = [P, ctypes.c_long, P, P, P]
         L.ref_amips_batch_e.argtypes = [P, ctypes.c_long, P]
+        L.ref_amips_batch_ejh_mt.argtypes = [P, ctypes.c_long, P, P, P, c_int]
         _REF = L
     return _REF
 
@@ -240,6 +254,39 @@ class Surface:
         lib().orc_point_out_batch(self.h, _p(pts), pts.shape[0], float(eps2), _p(out))
         return out
 
+    def face_out_mt(self, tris, eps2, d, nthreads=0):
+        """face_out over host threads (0: all OpenMP offers)"""
+        tris = _f64(tris).reshape(-1, 9)
+        out = np.empty(tris.shape[0], dtype=np.uint8)
+        lib().orc_face_out_batch_mt(self.h, _p(tris), tris.shape[0], float(eps2), float(d),
+                                    _p(out), int(nthreads))
+        return out
+
+    def point_out_mt(self, pts, eps2, nthreads=0):
+        pts = _f64(pts).reshape(-1, 3)
+        out = np.empty(pts.shape[0], dtype=np.uint8)
+        lib().orc_point_out_batch_mt(self.h, _p(pts), pts.shape[0], float(eps2), _p(out),
+                                     int(nthreads))
+        return out
+
+    def face_risk(self, tris, eps2, d, band, nthreads=0):
+        """near-tie counts of isFaceOutEnvelop: (samples, samples with
+        |min d2 - eps2| <= band, triangles with such a sample, triangles whose
+        decision rests on the band)"""
+        tris = _f64(tris).reshape(-1, 9)
+        c = np.zeros(4, dtype=np.int64)
+        lib().orc_face_risk_batch_mt(self.h, _p(tris), tris.shape[0], float(eps2), float(d),
+                                     float(band), _p(c), int(nthreads))
+        return tuple(int(v) for v in c)
+
+    def point_risk(self, pts, eps2, band, nthreads=0):
+        """(points, points with |min d2 - eps2| <= band)"""
+        pts = _f64(pts).reshape(-1, 3)
+        c = np.zeros(2, dtype=np.int64)
+        lib().orc_point_risk_batch_mt(self.h, _p(pts), pts.shape[0], float(eps2), float(band),
+                                      _p(c), int(nthreads))
+        return int(c[0]), int(c[1])
+
     def point_out_brute(self, pts, eps2):
         pts = _f64(pts).reshape(-1, 3)
         L = lib()
@@ -332,6 +379,71 @@ def smooth_batch(B, old_q=None, sf=None, bt=None, eps2=0.0, d=1.0):
         if s == 0 and int(B["kind"][p]) in (1, 2):
             new_q[r0:r1] = nq[:r1 - r0]
     return p_out, status, new_q
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
+
+
+def amips_batch_mt(tets, x, y, z, mode=2, nthreads=0):
+    """amips_batch over host threads"""
+    t = np.ascontiguousarray(tets, dtype=np.int32)
+    n = len(t)
+    E = np.empty(n)
+    G = np.empty((n, 3)) if mode >= 1 else None
+    H = np.empty((n, 9)) if mode >= 2 else None
+    lib().orc_amips_batch_mt(_p(t), n, _p(_f64(x)), _p(_f64(y)), _p(_f64(z)), mode, _p(E),
+                             _p(G) if G is not None else None, _p(H) if H is not None else None,
+                             int(nthreads))
+    return E, G, H
+
+
+def smooth_batch_mt(B, old_q=None, sf=None, bt=None, eps2=0.0, d=1.0, nthreads=0):
+    """smooth_batch over host threads; also returns the line-search near-tie
+    counts (comparisons, within band, |dE| < 1e-5 tests, within band)"""
+    x, y, z = _f64(B["x"]), _f64(B["y"]), _f64(B["z"])
+    n = len(B["center"])
+    rt = np.ascontiguousarray(B["ring_tets"], dtype=np.int32)
+    nr = len(rt)
+    oq = np.zeros(max(nr, 1)) if old_q is None else _f64(old_q)
+    ab = np.ascontiguousarray(B["tri_ab"], dtype=np.int32).reshape(-1)
+    if len(ab) == 0:
+        ab = np.zeros(2, np.int32)
+    p_out = np.empty((n, 3))
+    status = np.empty(n, dtype=np.uint8)
+    new_q = np.zeros(max(nr, 1))
+    risk = np.zeros(4, dtype=np.int64)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    kind = np.ascontiguousarray(B["kind"], dtype=np.uint8)
+    center, ro, to = i32(B["center"]), i32(B["ring_off"]), i32(B["tri_off"])
+    lib().orc_smooth_batch_mt(n, _p(kind), _p(center), _p(ro), _p(rt), _p(x), _p(y), _p(z),
+                              _p(oq), _p(to), _p(ab), sf.h if sf is not None else None,
+                              bt.h if bt is not None else None, float(eps2), float(d),
+                              _p(p_out), _p(status), _p(new_q), _p(risk), int(nthreads))
+    return p_out, status, new_q, tuple(int(v) for v in risk)
+
+
+def collapse_batch_mt(B, sf=None, bt=None, eps2=0.0, d=1.0, check_quality=True, check_eps=True,
+                      soft_energy=None, nthreads=0):
+    """collapse_batch over host threads"""
+    x, y, z = _f64(B["x"]), _f64(B["y"]), _f64(B["z"])
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    rt = i32(B["ring_tets"]).reshape(-1)
+    tri = i32(B["tri_ids"]).reshape(-1)
+    nr = len(rt) // 4
+    oq = _f64(B["old_q"]) if nr else np.zeros(1)
+    n = len(B["v1"])
+    status = np.empty(n, dtype=np.uint8)
+    new_q = np.zeros(max(nr, 1))
+    lib().orc_collapse_batch_mt(
+        n, _p(i32(B["v1"])), _p(i32(B["v2"])), _p(np.ascontiguousarray(B["flags"], np.uint8)),
+        _p(i32(B["ring_off"])), _p(rt if nr else np.zeros(4, np.int32)), _p(x), _p(y), _p(z),
+        _p(oq), _p(i32(B["tri_off"])), _p(tri if len(tri) else np.zeros(3, np.int32)),
+        sf.h if sf is not None else None, bt.h if bt is not None else None, float(eps2),
+        float(d), int(bool(check_quality)), int(bool(check_eps)), int(soft_energy is not None),
+        float(soft_energy) if soft_energy is not None else 0.0, _p(status), _p(new_q),
+        int(nthreads))
+    return status, new_q
 
 
 def collapse_batch(B, sf=None, bt=None, eps2=0.0, d=1.0, check_quality=True, check_eps=True,

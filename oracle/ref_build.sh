@@ -8,6 +8,9 @@
 # Energy definition and LocalOperations::check) and feeds them to g++ on stdin
 # after the static member declarations of src/tetwild/LocalOperations.h:109-111.
 # Nothing of the reference is written to disk; only the .so lands in _ref/.
+# The batch entry points loop over tets on one core (ref_amips_batch_*) or
+# over host threads with OpenMP (ref_amips_batch_ejh_mt): the multi-core CPU
+# baseline.
 # The rest of LocalOperations.cpp needs CGAL, geogram, Eigen and libigl, which
 # are not in the image: that part is unbuildable here.
 set -euo pipefail
@@ -19,6 +22,7 @@ OUT="$HERE/_ref"
 mkdir -p "$OUT"
 {
   echo '#include <cmath>'
+  echo '#include <omp.h>'
   echo 'using std::pow;'
   echo 'namespace tetwild { struct LocalOperations {'
   echo '  static double comformalAMIPSEnergy_new(const double* T);'
@@ -36,6 +40,12 @@ mkdir -p "$OUT"
   echo '    tetwild::LocalOperations::comformalAMIPSHessian_new(T + 12*i, H + 9*i); } }'
   echo 'void ref_amips_batch_e(const double* T, long n, double* E) {'
   echo '  for (long i = 0; i < n; ++i) E[i] = tetwild::LocalOperations::comformalAMIPSEnergy_new(T + 12*i); }'
+  echo 'void ref_amips_batch_ejh_mt(const double* T, long n, double* E, double* G, double* H, int nt) {'
+  echo '  if (nt <= 0) nt = omp_get_max_threads();'
+  echo '  _Pragma("omp parallel for schedule(static, 4096) num_threads(nt)")'
+  echo '  for (long i = 0; i < n; ++i) { E[i] = tetwild::LocalOperations::comformalAMIPSEnergy_new(T + 12*i);'
+  echo '    tetwild::LocalOperations::comformalAMIPSJacobian_new(T + 12*i, G + 3*i);'
+  echo '    tetwild::LocalOperations::comformalAMIPSHessian_new(T + 12*i, H + 9*i); } }'
   echo '}'
-} | g++ -x c++ -std=c++11 -O3 -DNDEBUG -ffp-contract=off -fPIC -shared -o "$OUT/libref_amips.so" -
+} | g++ -x c++ -std=c++11 -O3 -DNDEBUG -ffp-contract=off -fopenmp -fPIC -shared -o "$OUT/libref_amips.so" -
 echo "built $OUT/libref_amips.so"

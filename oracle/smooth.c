@@ -188,8 +188,35 @@ double orc_new_energy(const int32_t* ring_tets, int m, const double* x, const do
     return ring_energy(ring_tets, m, &w);
 }
 
-int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, const double* y,
-                       const double* z, double* p_out)
+/* Near-tie accounting for the line search (risk may be NULL).  The reference
+ * sums a ring in unordered_set order (src/tetwild/VertexSmoother.cpp:145-150),
+ * so its energies may differ from these ring-order sums by up to the
+ * recursive-summation bound (m - 1) u sum|E_i| each (u = 2^-53; the terms
+ * are positive, so sum|E_i| is the sum itself).  A comparison whose operands
+ * lie within the sum of both bounds of a tie is one the reference could
+ * decide the other way.
+ *   risk[0] energy comparisons new >= old (finite operands)
+ *   risk[1] of them within the band: |new - old| <= 2 (m - 1) u (new + old)
+ *   risk[2] |dE| < 1e-5 tests of an accepted step
+ *   risk[3] of them within the band: ||dE| - 1e-5| <= 2 (m - 1) u (new + old) */
+static void note_energy_compare(int64_t* risk, int m, double new_e, double old_e)
+{
+    if (!risk || !isfinite(new_e) || !isfinite(old_e)) return;
+    double band = 2.0 * (double)(m > 1 ? m - 1 : 1) * 0x1p-53 * (fabs(new_e) + fabs(old_e));
+    risk[0]++;
+    if (fabs(new_e - old_e) <= band) risk[1]++;
+}
+
+static void note_threshold(int64_t* risk, int m, double new_e, double old_e)
+{
+    if (!risk) return;
+    double band = 2.0 * (double)(m > 1 ? m - 1 : 1) * 0x1p-53 * (fabs(new_e) + fabs(old_e));
+    risk[2]++;
+    if (fabs(fabs(new_e - old_e) - 1e-5) <= band) risk[3]++;
+}
+
+static int newtons(const int32_t* rt, int m, int32_t v_id, const double* x, const double* y,
+                   const double* z, double* p_out, int64_t* risk)
 {
     view w = {x, y, z, v_id, {x[v_id], y[v_id], z[v_id]}};
     int is_moved = 0;
@@ -219,6 +246,7 @@ int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, 
                 continue;
             }
             new_energy = ring_energy(rt, m, &w);
+            note_energy_compare(risk, m, new_energy, old_energy);
             if (new_energy >= old_energy || isinf(new_energy) || isnan(new_energy)) {
                 memcpy(w.p, old_p, sizeof old_p);
                 a /= 2.0;
@@ -227,6 +255,7 @@ int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, 
             step_taken = 1;
             break;
         }
+        if (step_taken) note_threshold(risk, m, new_energy, old_energy);
         if (step_taken && fabs(new_energy - old_energy) < 1e-5) step_taken = 0;
         if (!step_taken) {
             is_moved = step != 0;
@@ -238,6 +267,12 @@ int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, 
     p_out[1] = w.p[1];
     p_out[2] = w.p[2];
     return is_moved;
+}
+
+int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, const double* y,
+                       const double* z, double* p_out)
+{
+    return newtons(rt, m, v_id, x, y, z, p_out, NULL);
 }
 
 /* ---- one vertex of smoothSingle / smoothSurface on a frozen mesh -------- */
@@ -266,15 +301,25 @@ int orc_smooth_problem(int kind, int32_t v, const int32_t* rt, int m, const doub
                        const orc_surface* bt, double eps2, double d, double* p_out,
                        double* new_q)
 {
+    return orc_smooth_problem_risk(kind, v, rt, m, x, y, z, old_q, tri_ab, ntri, sf, bt, eps2, d,
+                                   p_out, new_q, NULL);
+}
+
+int orc_smooth_problem_risk(int kind, int32_t v, const int32_t* rt, int m, const double* x,
+                            const double* y, const double* z, const double* old_q,
+                            const int32_t* tri_ab, int ntri, const orc_surface* sf,
+                            const orc_surface* bt, double eps2, double d, double* p_out,
+                            double* new_q, int64_t* risk)
+{
     view w = {x, y, z, v, {x[v], y[v], z[v]}};
     double p0[3] = {x[v], y[v], z[v]};
     p_out[0] = p0[0]; p_out[1] = p0[1]; p_out[2] = p0[2];
     if (kind == 3)
-        return orc_newtons_method(rt, m, v, x, y, z, p_out) ? ORC_SMOOTH_MOVED
-                                                            : ORC_SMOOTH_NOT_MOVED;
+        return newtons(rt, m, v, x, y, z, p_out, risk) ? ORC_SMOOTH_MOVED
+                                                       : ORC_SMOOTH_NOT_MOVED;
     if (ring_flip(rt, m, &w)) return ORC_SMOOTH_INVALID;
     double p[3];
-    if (!orc_newtons_method(rt, m, v, x, y, z, p)) return ORC_SMOOTH_NOT_MOVED;
+    if (!newtons(rt, m, v, x, y, z, p, risk)) return ORC_SMOOTH_NOT_MOVED;
     if (kind == 0) {
         memcpy(w.p, p, sizeof p);
         if (ring_flip(rt, m, &w)) return ORC_SMOOTH_FLIP_END;

@@ -50,7 +50,7 @@ def checker(tmp_path_factory):
     oracle = [os.path.join(ROOT, "oracle", f) for f in sorted(os.listdir(os.path.join(ROOT, "oracle")))
               if f.endswith(".c")]
     exe = str(d / "headers_check")
-    cmd = ["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-fsanitize=address,undefined",
+    cmd = ["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-fsanitize=address,undefined",
            "-fno-sanitize-recover=undefined", "-I", str(d),
            os.path.join(ROOT, "tests", "host", "headers_check.cpp")] + oracle + ["-o", exe, "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
