@@ -182,8 +182,10 @@ int twg_sample_write(twg_ctx* ctx, const double* tri9, int64_t n, double d,
  * slim_energy of each ring tet, tri_ab[tri_off[p] .. tri_off[p+1]) the other
  * two vertex ids of each one-ring surface face in the reference's order
  * (:395-402), and new_q receives the ring's new qualities when the vertex
- * moves.  Vertices must be rounded (posf == pos); the caller keeps rings with
- * unrounded vertices on its exact-arithmetic path. */
+ * moves.  Flip tests are taken on x/y/z: exact for a ring whose vertices
+ * other than center[p] are rounded (pos == posf).  A ring with another
+ * unrounded vertex needs exact rational orientations the device does not
+ * have: the caller runs it through twg_newton_step with its own tests. */
 typedef struct twg_smooth_desc {
     const double* x;
     const double* y;
@@ -232,6 +234,33 @@ int twg_smooth_plan_create(twg_ctx* ctx, const twg_smooth_desc* desc, int precis
 int twg_smooth_plan_run(twg_smooth_plan* plan);
 int twg_smooth_plan_info(const twg_smooth_plan* plan, int* tile, int* chunk);
 int twg_smooth_plan_destroy(twg_smooth_plan* plan);
+
+/* One step of NewtonsMethod (src/tetwild/VertexSmoother.cpp:478-528) laid
+ * out for a caller that decides some of its flip tests itself: the tets with
+ * an unrounded vertex other than the moving one, which the reference orients
+ * on exact rational positions (isTetFlip, src/tetwild/LocalOperations.cpp:
+ * 535-553).  With vertex v at (px, py, pz): NewtonsUpdate over the m ring
+ * tets (ok, energy, J, H: the sums twg_smooth_batch forms, bit for bit; ok =
+ * -1 when an index is out of range), then every line-search candidate
+ * X[k] = solve(H, H X0 - 2^-k J), k = 0..19, with finite[k], flip[k] (some
+ * ring tet j with skip[j] == 0 is not positively oriented with v at X[k];
+ * skip may be NULL) and new_energy[k] (getNewEnergy at X[k]; 0 when X[k] is
+ * not finite).  The caller walks k in order as the halving loop does and
+ * accepts the first finite candidate that no flip test rejects and that
+ * lowers the energy.  Arrays are device arrays; out points to device memory. */
+typedef struct twg_newton_step_out {
+    double energy;
+    double J[3];
+    double H[9];
+    double X[20][3];
+    double new_energy[20];
+    int32_t ok;
+    uint8_t finite[20];
+    uint8_t flip[20];
+} twg_newton_step_out;
+int twg_newton_step(twg_ctx* ctx, const double* x, const double* y, const double* z, int64_t nv,
+                    const int32_t* ring_tets, int32_t m, int32_t v, double px, double py,
+                    double pz, const uint8_t* skip, twg_newton_step_out* out);
 
 /* ---- edge-collapse candidates -------------------------------------------- */
 /* Return codes of EdgeCollapser::collapseAnEdge (src/tetwild/EdgeCollapser.h:52-56) */

@@ -158,6 +158,19 @@ int orc_smooth_problem_risk(int kind, int32_t v, const int32_t* ring_tets, int m
                             const orc_surface* sf, const orc_surface* bt, double eps2, double d,
                             double* p_out, double* new_q, int64_t* risk);
 
+/* One vertex of smoothSingle (kind 0) or smoothSurface (kind 1 = surface,
+ * 2 = boundary) with the reference's rounding rules, on a mesh whose vertices
+ * carry posf (x, y, z), an exact position pos (px, py, pz; doubles here) and
+ * is_rounded (rounded[]): the try-to-round step, isTetFlip on posf or pos
+ * per tet, NewtonsMethod moving both, and the reference's commits.  Updates
+ * the vertex's posf, pos and rounded flag in place as the reference leaves
+ * them; new_q as orc_smooth_problem.  Returns ORC_SMOOTH_*. */
+int orc_smooth_vertex(int kind, int32_t v, const int32_t* ring_tets, int m, double* x, double* y,
+                      double* z, double* px, double* py, double* pz, uint8_t* rounded,
+                      const double* old_q, const int32_t* tri_ab, int ntri,
+                      const orc_surface* sf, const orc_surface* bt, double eps2, double d,
+                      double* new_q);
+
 /* ---- edge-collapse candidates, src/tetwild/EdgeCollapser.cpp:284-446 ---- */
 /* return codes: the reference's (src/tetwild/EdgeCollapser.h:52-56) */
 #define ORC_COLLAPSE_SUCCESS 0

@@ -90,6 +90,8 @@ def load(path):
                                        c_int, c_dbl, P]),
         "orc_smooth_problem_risk": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P,
                                             c_int, P, P, c_dbl, c_dbl, P, P, P]),
+        "orc_smooth_vertex": (c_int, [c_int, ctypes.c_int32, P, c_int, P, P, P, P, P, P, P, P,
+                                      P, c_int, P, P, c_dbl, c_dbl, P]),
         "orc_max_threads": (c_int, []),
         "orc_amips_batch_mt": (None, [P, c_i64, P, P, P, c_int, P, P, P, c_int]),
         "orc_tet_qualities_mt": (None, [P, c_i64, P, P, P, P, c_int]),
@@ -379,6 +381,36 @@ def smooth_batch(B, old_q=None, sf=None, bt=None, eps2=0.0, d=1.0):
         if s == 0 and int(B["kind"][p]) in (1, 2):
             new_q[r0:r1] = nq[:r1 - r0]
     return p_out, status, new_q
+
+
+class ExactMesh:
+    """posf, exact positions (doubles) and is_rounded flags of a tet mesh, for
+    the reference's per-vertex smoothing bodies with its rounding rules
+    (orc_smooth_vertex); the arrays are updated in place"""
+
+    def __init__(self, P, Pexact, rounded):
+        self.x, self.y, self.z = (np.ascontiguousarray(P[:, k], dtype=np.float64) for k in range(3))
+        self.px, self.py, self.pz = (np.ascontiguousarray(Pexact[:, k], dtype=np.float64)
+                                     for k in range(3))
+        self.r = np.array(rounded, dtype=np.uint8)
+
+    def smooth_vertex(self, kind, v, ring_tets, old_q, tri_ab, sf=None, bt=None, eps2=0.0, d=1.0):
+        rt = np.ascontiguousarray(ring_tets, dtype=np.int32)
+        m = len(rt)
+        oq = _f64(old_q) if m else np.zeros(1)
+        ab = np.ascontiguousarray(tri_ab, dtype=np.int32).reshape(-1)
+        nt = len(ab) // 2
+        if nt == 0:
+            ab = np.zeros(2, np.int32)
+        nq = np.zeros(max(m, 1))
+        s = lib().orc_smooth_vertex(int(kind), int(v), _p(rt), m, _p(self.x), _p(self.y),
+                                    _p(self.z), _p(self.px), _p(self.py), _p(self.pz), _p(self.r),
+                                    _p(oq), _p(ab), nt, sf.h if sf is not None else None,
+                                    bt.h if bt is not None else None, float(eps2), float(d), _p(nq))
+        return int(s), nq[:m]
+
+    def posf(self):
+        return np.stack([self.x, self.y, self.z], 1)
 
 
 def max_threads():

@@ -188,6 +188,45 @@ double orc_new_energy(const int32_t* ring_tets, int m, const double* x, const do
     return ring_energy(ring_tets, m, &w);
 }
 
+/* A mesh with exact positions: posf (x, y, z), pos (px, py, pz) and
+ * is_rounded (r) per vertex; see orc_smooth_vertex. */
+typedef struct {
+    double *x, *y, *z, *px, *py, *pz;
+    uint8_t* r;
+} emesh;
+
+static void eget(const emesh* M, int32_t u, int exact, double* p)
+{
+    if (exact) { p[0] = M->px[u]; p[1] = M->py[u]; p[2] = M->pz[u]; }
+    else { p[0] = M->x[u]; p[1] = M->y[u]; p[2] = M->z[u]; }
+}
+
+static void eset(double* a, double* b, double* c, int32_t v, const double* p)
+{
+    a[v] = p[0];
+    b[v] = p[1];
+    c[v] = p[2];
+}
+
+static void moved_back(const emesh* M, int32_t v, const double* pf, const double* pos)
+{
+    eset(M->x, M->y, M->z, v, pf);
+    eset(M->px, M->py, M->pz, v, pos);
+}
+
+static int e_is_flip(const emesh* M, const int32_t* rt, int m)
+{
+    for (int i = 0; i < m; ++i) {
+        const int32_t* t = rt + 4 * i;
+        int all = 1;
+        for (int j = 0; j < 4; ++j) all = all && M->r[t[j]];
+        double T[12];
+        for (int j = 0; j < 4; ++j) eget(M, t[j], !all, T + 3 * j);
+        if (orc_orient3d(T, T + 3, T + 6, T + 9) != 1) return 1;
+    }
+    return 0;
+}
+
 /* Near-tie accounting for the line search (risk may be NULL).  The reference
  * sums a ring in unordered_set order (src/tetwild/VertexSmoother.cpp:145-150),
  * so its energies may differ from these ring-order sums by up to the
@@ -215,16 +254,25 @@ static void note_threshold(int64_t* risk, int m, double new_e, double old_e)
     if (fabs(fabs(new_e - old_e) - 1e-5) <= band) risk[3]++;
 }
 
+/* NewtonsMethod.  M == NULL: every vertex rounded, flips on posf with the
+ * vertex at w.p.  M != NULL: the reference's rounding rules -- posf and pos
+ * move together (VertexSmoother.cpp:499-500), isTetFlip per tet on posf or
+ * pos, both restored at the end (:538-539). */
 static int newtons(const int32_t* rt, int m, int32_t v_id, const double* x, const double* y,
-                   const double* z, double* p_out, int64_t* risk)
+                   const double* z, double* p_out, int64_t* risk, const emesh* M)
 {
     view w = {x, y, z, v_id, {x[v_id], y[v_id], z[v_id]}};
+    const double pf0[3] = {x[v_id], y[v_id], z[v_id]};
+    double pos0[3] = {0, 0, 0};
+    if (M) { pos0[0] = M->px[v_id]; pos0[1] = M->py[v_id]; pos0[2] = M->pz[v_id]; }
     int is_moved = 0;
     double old_energy = 0.0;
     double J[3], H[9], X0[3];
     for (int step = 0; step < MAX_STEP; ++step) {
         if (!ring_update(rt, m, &w, &old_energy, J, H, X0)) break;
         double old_p[3] = {w.p[0], w.p[1], w.p[2]};
+        double old_pos[3] = {0, 0, 0};
+        if (M) { old_pos[0] = M->px[v_id]; old_pos[1] = M->py[v_id]; old_pos[2] = M->pz[v_id]; }
         double a = 1.0;
         int step_taken = 0;
         double new_energy = 0.0;
@@ -240,8 +288,13 @@ static int newtons(const int32_t* rt, int m, int32_t v_id, const double* x, cons
                 continue;
             }
             w.p[0] = X[0]; w.p[1] = X[1]; w.p[2] = X[2];
-            if (ring_flip(rt, m, &w)) {
+            if (M) {
+                eset(M->x, M->y, M->z, v_id, X);
+                eset(M->px, M->py, M->pz, v_id, X);
+            }
+            if (M ? e_is_flip(M, rt, m) : ring_flip(rt, m, &w)) {
                 memcpy(w.p, old_p, sizeof old_p);
+                if (M) moved_back(M, v_id, old_p, old_pos);
                 a /= 2.0;
                 continue;
             }
@@ -249,6 +302,7 @@ static int newtons(const int32_t* rt, int m, int32_t v_id, const double* x, cons
             note_energy_compare(risk, m, new_energy, old_energy);
             if (new_energy >= old_energy || isinf(new_energy) || isnan(new_energy)) {
                 memcpy(w.p, old_p, sizeof old_p);
+                if (M) moved_back(M, v_id, old_p, old_pos);
                 a /= 2.0;
                 continue;
             }
@@ -266,13 +320,14 @@ static int newtons(const int32_t* rt, int m, int32_t v_id, const double* x, cons
     p_out[0] = w.p[0];
     p_out[1] = w.p[1];
     p_out[2] = w.p[2];
+    if (M) moved_back(M, v_id, pf0, pos0);
     return is_moved;
 }
 
 int orc_newtons_method(const int32_t* rt, int m, int32_t v_id, const double* x, const double* y,
                        const double* z, double* p_out)
 {
-    return newtons(rt, m, v_id, x, y, z, p_out, NULL);
+    return newtons(rt, m, v_id, x, y, z, p_out, NULL, NULL);
 }
 
 /* ---- one vertex of smoothSingle / smoothSurface on a frozen mesh -------- */
@@ -315,11 +370,11 @@ int orc_smooth_problem_risk(int kind, int32_t v, const int32_t* rt, int m, const
     double p0[3] = {x[v], y[v], z[v]};
     p_out[0] = p0[0]; p_out[1] = p0[1]; p_out[2] = p0[2];
     if (kind == 3)
-        return newtons(rt, m, v, x, y, z, p_out, risk) ? ORC_SMOOTH_MOVED
+        return newtons(rt, m, v, x, y, z, p_out, risk, NULL) ? ORC_SMOOTH_MOVED
                                                        : ORC_SMOOTH_NOT_MOVED;
     if (ring_flip(rt, m, &w)) return ORC_SMOOTH_INVALID;
     double p[3];
-    if (!newtons(rt, m, v, x, y, z, p, risk)) return ORC_SMOOTH_NOT_MOVED;
+    if (!newtons(rt, m, v, x, y, z, p, risk, NULL)) return ORC_SMOOTH_NOT_MOVED;
     if (kind == 0) {
         memcpy(w.p, p, sizeof p);
         if (ring_flip(rt, m, &w)) return ORC_SMOOTH_FLIP_END;
@@ -347,5 +402,97 @@ int orc_smooth_problem_risk(int kind, int32_t v, const int32_t* rt, int m, const
         if (orc_face_out(sf, tri, eps2, d, NULL)) return ORC_SMOOTH_REJECT_ENVELOPE;
     }
     memcpy(p_out, pf, sizeof pf);
+    return ORC_SMOOTH_MOVED;
+}
+
+/* ---- the per-vertex bodies with the reference's rounding rules ---------- */
+/* Each vertex has posf (x, y, z), an exact position pos (px, py, pz: doubles
+ * in these inputs) and is_rounded (rounded[]).  Everything follows the
+ * reference literally: isTetFlip orients on posf when the tet's four
+ * vertices are rounded and on pos otherwise (src/tetwild/LocalOperations.cpp:
+ * 535-553); NewtonsMethod moves posf and pos together (VertexSmoother.cpp:
+ * 499-500) and restores both; smoothSingle tries to round the vertex first
+ * (:152-161), validates the ring on posf (:163-172) and commits pos, posf
+ * and is_rounded = true unless the final isFlip fails (:196-211);
+ * smoothSurface (:272-455) likewise, rounding the vertex at its end. */
+int orc_smooth_vertex(int kind, int32_t v, const int32_t* rt, int m, double* x, double* y,
+                      double* z, double* px, double* py, double* pz, uint8_t* rounded,
+                      const double* old_q, const int32_t* tri_ab, int ntri,
+                      const orc_surface* sf, const orc_surface* bt, double eps2, double d,
+                      double* new_q)
+{
+    emesh M = {x, y, z, px, py, pz, rounded};
+    /* try to round the vertex */
+    if (!rounded[v]) {
+        double old_p[3] = {px[v], py[v], pz[v]}, pf[3] = {x[v], y[v], z[v]};
+        eset(px, py, pz, v, pf);
+        if (e_is_flip(&M, rt, m)) eset(px, py, pz, v, old_p);
+        else rounded[v] = 1;
+    }
+    /* validity on posf */
+    for (int i = 0; i < m; ++i) {
+        double T[12];
+        for (int j = 0; j < 4; ++j) eget(&M, rt[4 * i + j], 0, T + 3 * j);
+        if (orc_orient3d(T, T + 3, T + 6, T + 9) != 1) return ORC_SMOOTH_INVALID;
+    }
+    double pn[3];
+    if (!newtons(rt, m, v, x, y, z, pn, NULL, &M)) return ORC_SMOOTH_NOT_MOVED;
+    double old_pf[3] = {x[v], y[v], z[v]}, old_p[3] = {px[v], py[v], pz[v]};
+    const uint8_t old_r = rounded[v];
+    if (kind == 0) {
+        eset(px, py, pz, v, pn);
+        eset(x, y, z, v, pn);
+        rounded[v] = 1;
+        if (e_is_flip(&M, rt, m)) {
+            eset(px, py, pz, v, old_p);
+            eset(x, y, z, v, old_pf);
+            rounded[v] = old_r;
+            return ORC_SMOOTH_FLIP_END;
+        }
+        return ORC_SMOOTH_MOVED;
+    }
+    double pf[3], d2;
+    orc_nearest(kind == 2 ? bt : sf, pn, pf, &d2);
+    eset(x, y, z, v, pf);
+    eset(px, py, pz, v, pf);
+    if (e_is_flip(&M, rt, m)) {
+        eset(px, py, pz, v, old_p);
+        eset(x, y, z, v, old_pf);
+        return ORC_SMOOTH_REJECT_FLIP;
+    }
+    double om = 0.0, nm = 0.0;
+    for (int i = 0; i < m; ++i)
+        if (old_q[i] > om) om = old_q[i];
+    view w = {x, y, z, -1, {0, 0, 0}};
+    for (int i = 0; i < m; ++i) {
+        new_q[i] = tet_quality_at(&w, rt + 4 * i);
+        if (new_q[i] > nm) nm = new_q[i];
+    }
+    if (!(nm < om)) {
+        eset(px, py, pz, v, old_p);
+        eset(x, y, z, v, old_pf);
+        return ORC_SMOOTH_REJECT_QUALITY;
+    }
+    for (int i = 0; i < ntri; ++i) {
+        double tri[9];
+        memcpy(tri, pf, sizeof pf);
+        coords(x, y, z, tri_ab[2 * i], tri + 3);
+        coords(x, y, z, tri_ab[2 * i + 1], tri + 6);
+        if (orc_collinear3(tri, tri + 3, tri + 6)) continue;
+        if (orc_face_out(sf, tri, eps2, d, NULL)) {
+            eset(px, py, pz, v, old_p);
+            eset(x, y, z, v, old_pf);
+            return ORC_SMOOTH_REJECT_ENVELOPE;
+        }
+    }
+    if (!rounded[v]) {
+        eset(px, py, pz, v, pf);
+        if (e_is_flip(&M, rt, m)) {
+            eset(px, py, pz, v, old_p);
+            rounded[v] = 0;
+        } else {
+            rounded[v] = 1;
+        }
+    }
     return ORC_SMOOTH_MOVED;
 }

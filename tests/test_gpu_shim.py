@@ -5,6 +5,7 @@ per-vertex loops (src/tetwild/VertexSmoother.cpp:114-463) in the same order.
 Bar: final vertex positions, stored qualities and counters bit-exact with
 the replay, for the reference's vertex order (seq) and for the colour-class
 order of the batched variants (batch)."""
+import json
 import os
 import struct
 import subprocess
@@ -321,3 +322,89 @@ def test_shim_remove_matches_oracle(tmp_path):
     kinds = np.array(kinds)
     print("kinds", np.bincount(kinds, minlength=6))
     assert (kinds == 3).any() and (kinds == 4).any() and (kinds == 5).any() and (kinds == 0).any()
+
+
+def replay_exact(P, Pex, rounded, tets, fs, flags, q, S, eps, d, batched):
+    """smoothSingle + smoothSurface with the reference's rounding rules
+    (oracle ExactMesh: try-to-round, isTetFlip on posf or pos per tet)"""
+    M = O.ExactMesh(P, Pex, rounded)
+    q = q.copy()
+    nv = len(P)
+    C = conn(tets, nv)
+    counter = suc = 0
+    none = np.zeros((0, 2), np.int32)
+    interior = [v for v in range(nv) if not flags[v] & 3]
+    for grp in (colour_classes(interior, C, tets) if batched else [[v] for v in interior]):
+        for v in grp:
+            s, _ = M.smooth_vertex(0, v, tets[C[v]], q[C[v]], none, S, S, eps * eps, d)
+            counter += 1
+            suc += s in (0, 3)
+    q = O.tet_qualities(tets, M.x, M.y, M.z)
+    surf = [v for v in range(nv) if flags[v] & 1]
+    for grp in (colour_classes(surf, C, tets) if batched else [[v] for v in surf]):
+        for v in grp:
+            ring = C[v]
+            s, nq = M.smooth_vertex(1, v, tets[ring], q[ring], tri_ab_of(v, ring, tets, fs), S, S,
+                                    eps * eps, d)
+            counter += 1
+            if s == 0:
+                suc += 1
+                q[ring] = nq
+    return M.posf(), M.r.copy(), q, counter, suc
+
+
+@pytest.mark.parametrize("mode", ["seq", "batch"])
+def test_shim_unrounded_matches_replay(ctx, tmp_path, mode):
+    """VertexSmoother with unrounded vertices: 20% of the vertices carry an
+    exact position pos != posf (doubles, offset by up to 0.2 h), handed to the
+    mirror through ExactPositions.  Positions, qualities, counters and the
+    final is_rounded flags are bit-exact with the oracle's literal restatement
+    of the reference's rounding rules, and differ from a run that treats every
+    vertex as rounded (so the exact path decides something)."""
+    assert os.path.exists(DRIVER), "build the shim first (make -C tetwild_amd/shim)"
+    n = 5
+    P, tets, fs, flags, sf = lattice_mesh(n=n, seed=6)
+    eps, d = 2e-3, 2e-3
+    st = W.Stream(12, 0)
+    unr = (st.uniform(len(P)) < 0.2)
+    Pex = P.copy()
+    off = (st.uniform(3 * len(P)).reshape(-1, 3) - 0.5) * (0.4 / n)
+    Pex[unr] += off[unr]
+    # surface vertices keep their exact position on the face plane
+    g = np.round(P * n)
+    for a in range(3):
+        onp = (g[:, a] == 0) | (g[:, a] == n)
+        Pex[onp, a] = P[onp, a]
+    q = O.tet_qualities(tets, *(np.ascontiguousarray(P[:, k]) for k in range(3)))
+    fl = flags | (unr.astype(np.uint8) * 16)
+    mesh = tmp_path / "mesh.bin"
+    out = tmp_path / "out.bin"
+    write_mesh(mesh, P, tets, fs, fl, q, sf, eps, d)
+    with open(mesh, "ab") as f:
+        f.write(np.ascontiguousarray(Pex[unr], dtype=np.float64).tobytes())
+    r = subprocess.run([DRIVER, str(mesh), str(out), mode], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip(), flush=True)
+    raw = np.fromfile(out, dtype=np.float64)
+    nv, nt = len(P), len(tets)
+    Pg = raw[:3 * nv].reshape(nv, 3)
+    qg = raw[3 * nv:3 * nv + nt]
+    cnt = raw[3 * nv + nt:3 * nv + nt + 2].view(np.int64)
+    rg = np.fromfile(str(out) + ".rounded", dtype=np.uint8)
+    Vs = sf.reshape(-1, 3)
+    Fs = np.arange(len(Vs), dtype=np.int32).reshape(-1, 3)
+    surf = ctx.surface(Vs, Fs)
+    S = O.Surface(Vs, Fs[surf.order()])
+    Pr, rr, qr, counter, suc = replay_exact(P, Pex, ~unr, tets, fs, flags, q, S, eps, d,
+                                            mode == "batch")
+    print(f"unrounded {int(unr.sum())} -> {int((rr == 0).sum())}; counter {counter} suc {suc}")
+    assert np.array_equal(Pg.view(np.int64), Pr.view(np.int64))
+    assert np.array_equal(qg.view(np.int64), qr.view(np.int64))
+    assert (cnt[0], cnt[1]) == (counter, suc)
+    assert np.array_equal(rg, rr)
+    assert json.loads(r.stdout)["exact_calls"] > 0
+    # the rounding rules matter here: the all-rounded replay ends elsewhere
+    Pa, _, _, _, _ = replay_exact(P, P, np.ones(len(P), bool), tets, fs, flags, q, S, eps, d,
+                                  mode == "batch")
+    assert not np.array_equal(Pa.view(np.int64), Pr.view(np.int64))

@@ -185,3 +185,48 @@ def test_envelope_rejection_batch(ctx):
             batch.run(precision=prec, tile=tile, chunk=1)
             torch.cuda.synchronize()
             check(batch, ref, B)
+
+
+def test_newton_step_walks_to_newtons_method(ctx, dev):
+    """twg_newton_step: its NewtonsUpdate equals the oracle's, and the host
+    halving loop over its candidates (no tet skipped) ends where the batch
+    kernel's NewtonsMethod (kind 3) and the oracle end, bit for bit"""
+    B = W.smoothing_batch(120, seed=21)
+    B3 = dict(B)
+    B3["kind"] = np.full(len(B["center"]), 3, np.uint8)
+    rp, rs, _ = O.smooth_batch(B3)
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(B[k])).to(dev) for k in "xyz")
+    for p in range(len(B["center"])):
+        r0, r1 = int(B["ring_off"][p]), int(B["ring_off"][p + 1])
+        v = int(B["center"][p])
+        rt_np = np.ascontiguousarray(B["ring_tets"][r0:r1])
+        rt = torch.from_numpy(rt_np).to(dev)
+        pos = np.array([B["x"][v], B["y"][v], B["z"][v]])
+        moved = False
+        for step in range(15):
+            o = ctx.newton_step(rt, v, x, y, z, pos)
+            assert o.ok >= 0
+            if step == 0:
+                ok, E, J, H, _ = O.newtons_update(rt_np, v, B["x"], B["y"], B["z"])
+                assert bool(o.ok) == bool(ok) and o.energy == E
+                assert np.array_equal(np.array(o.J), J) and np.array_equal(np.array(o.H), H.reshape(9))
+            if not o.ok:
+                break
+            taken = tiny = False
+            for k in range(20):
+                if not o.finite[k] or o.flip[k]:
+                    continue
+                ne = o.new_energy[k]
+                if ne >= o.energy or np.isinf(ne) or np.isnan(ne):
+                    continue
+                taken, tiny = True, abs(ne - o.energy) < 1e-5
+                pos = np.array(o.X[k])
+                break
+            if taken and tiny:
+                taken = False
+            if not taken:
+                moved = step != 0
+                break
+            moved = True
+        assert moved == (rs[p] == 0), p
+        assert np.array_equal(bits(pos), bits(rp[p])), p

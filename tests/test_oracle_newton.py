@@ -160,10 +160,11 @@ MUTATIONS = [
      check_threshold),
     ("threshold 1e-4", "fabs(new_energy - old_energy) < 1e-5", "fabs(new_energy - old_energy) < 1e-4",
      check_threshold),
-    ("no flip test", "if (ring_flip(rt, m, &w)) {\n                memcpy(w.p, old_p, sizeof old_p);",
-     "if (0 && ring_flip(rt, m, &w)) {\n                memcpy(w.p, old_p, sizeof old_p);", check_needle),
-    ("halving by 3", "            if (ring_flip(rt, m, &w)) {\n                memcpy(w.p, old_p, sizeof old_p);\n                a /= 2.0;",
-     "            if (ring_flip(rt, m, &w)) {\n                memcpy(w.p, old_p, sizeof old_p);\n                a /= 3.0;",
+    ("no flip test", "if (M ? e_is_flip(M, rt, m) : ring_flip(rt, m, &w)) {",
+     "if (0 && (M ? e_is_flip(M, rt, m) : ring_flip(rt, m, &w))) {", check_needle),
+    ("halving by 3",
+     "                if (M) moved_back(M, v_id, old_p, old_pos);\n                a /= 2.0;\n                continue;\n            }\n            new_energy",
+     "                if (M) moved_back(M, v_id, old_p, old_pos);\n                a /= 3.0;\n                continue;\n            }\n            new_energy",
      check_blocked),
     ("is_moved on step 0", "is_moved = step != 0;", "is_moved = 1;", check_threshold),
 ]

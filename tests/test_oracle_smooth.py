@@ -177,3 +177,107 @@ def test_envelope_rejection_branch():
 ], ids=["no face test"])
 def test_envelope_rejection_catches_mutation(tmp_path, name, old, new):
     assert fails(check_envelope_rejection, tmp_path, "smooth.c", old, new), name
+
+
+# ---- the per-vertex bodies with the reference's rounding rules --------------
+def _lattice(n=4, seed=5):
+    x, y, z, tets, _ = W.kuhn_lattice(n, jitter=0.12, seed=seed)
+    P = np.stack([x, y, z], 1)
+    return P, tets
+
+
+def _rings(tets, nv):
+    C = [[] for _ in range(nv)]
+    for t, tt in enumerate(tets):
+        for v in tt:
+            C[v].append(t)
+    return [sorted(set(c)) for c in C]
+
+
+def _interior(P, n):
+    g = np.round(P * n)
+    return [v for v in range(len(P)) if ((g[v] > 0) & (g[v] < n)).all()]
+
+
+def test_smooth_vertex_all_rounded_is_smooth_problem():
+    """with every vertex rounded (pos == posf) the literal body is the
+    frozen-ring problem: same status, same position, every vertex"""
+    n = 4
+    P, tets = _lattice(n)
+    C = _rings(tets, len(P))
+    q = O.tet_qualities(tets, *(np.ascontiguousarray(P[:, k]) for k in range(3)))
+    for v in _interior(P, n):
+        M = O.ExactMesh(P, P, np.ones(len(P), np.uint8))
+        s, _ = M.smooth_vertex(0, v, tets[C[v]], q[C[v]], np.zeros((0, 2), np.int32))
+        B = {"x": P[:, 0], "y": P[:, 1], "z": P[:, 2], "center": np.array([v], np.int32),
+             "ring_off": np.array([0, len(C[v])], np.int32), "ring_tets": tets[C[v]],
+             "kind": np.array([0], np.uint8), "tri_ab": np.zeros((0, 2), np.int32),
+             "tri_off": np.array([0, 0], np.int32)}
+        p, st, _ = O.smooth_batch(B, q[C[v]])
+        assert s == st[0], v
+        if s == 0:
+            assert np.array_equal(M.posf()[v], p[0]), v
+        else:
+            assert np.array_equal(M.posf()[v], P[v]), v
+
+
+def test_smooth_vertex_try_round():
+    """an unrounded vertex whose pos sits where posf is valid gets rounded
+    (pos = posf, flag set) before it is smoothed; one whose posf would invert a
+    ring tet against its neighbours stays unrounded (and the ring is invalid
+    on posf, so it does not move)"""
+    n = 4
+    P, tets = _lattice(n)
+    C = _rings(tets, len(P))
+    q = O.tet_qualities(tets, *(np.ascontiguousarray(P[:, k]) for k in range(3)))
+    v = _interior(P, n)[3]
+    r = np.ones(len(P), np.uint8)
+    r[v] = 0
+    Pex = P.copy()
+    Pex[v] += 1e-9
+    M = O.ExactMesh(P, Pex, r.copy())
+    s, _ = M.smooth_vertex(0, v, tets[C[v]], q[C[v]], np.zeros((0, 2), np.int32))
+    assert M.r[v] == 1 and s in (0, 2, 3)
+    # posf far outside the ring (through a face): pos stays, flag stays 0
+    Pf = P.copy()
+    Pf[v] += 3.0 / n
+    M = O.ExactMesh(Pf, P, r.copy())
+    s, _ = M.smooth_vertex(0, v, tets[C[v]], q[C[v]], np.zeros((0, 2), np.int32))
+    assert M.r[v] == 0 and s == 1  # ORC_SMOOTH_INVALID
+    assert np.array_equal(np.stack([M.px, M.py, M.pz], 1)[v], P[v])
+
+
+def test_smooth_vertex_exact_neighbour_blocks_moves():
+    """a ring tet with an unrounded neighbour u is oriented on pos (all four
+    vertices): with pos[u] placed so that tet is inverted in the exact
+    geometry, every Newton candidate flips and the vertex does not move,
+    while the same ring with u rounded moves"""
+    n = 4
+    P, tets = _lattice(n)
+    C = _rings(tets, len(P))
+    q = O.tet_qualities(tets, *(np.ascontiguousarray(P[:, k]) for k in range(3)))
+    inner = _interior(P, n)
+    moved = 0
+    for v in inner:
+        M = O.ExactMesh(P, P, np.ones(len(P), np.uint8))
+        s0, _ = M.smooth_vertex(0, v, tets[C[v]], q[C[v]], np.zeros((0, 2), np.int32))
+        if s0 != 0:
+            continue
+        # u: a neighbour; pos[u] reflected through the plane of the opposite
+        # face of one ring tet holding both, which inverts that tet in pos
+        t = C[v][0]
+        u = [w for w in tets[t] if w != v][0]
+        others = [w for w in tets[t] if w not in (u,)]
+        a, b, c = P[others[0]], P[others[1]], P[others[2]]
+        nrm = np.cross(b - a, c - a)
+        nrm /= np.linalg.norm(nrm)
+        Pex = P.copy()
+        Pex[u] = P[u] - 2 * np.dot(P[u] - a, nrm) * nrm
+        r = np.ones(len(P), np.uint8)
+        r[u] = 0
+        M = O.ExactMesh(P, Pex, r)
+        s1, _ = M.smooth_vertex(0, v, tets[C[v]], q[C[v]], np.zeros((0, 2), np.int32))
+        assert s1 == 2, (v, s1)  # NOT_MOVED: every candidate flips in the exact geometry
+        assert np.array_equal(M.posf()[v], P[v])
+        moved += 1
+    assert moved > 0

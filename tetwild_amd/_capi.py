@@ -55,6 +55,7 @@ SIGNATURES = {
     "twg_smooth_plan_run": [P],
     "twg_smooth_plan_info": [P, P, P],
     "twg_smooth_plan_destroy": [P],
+    "twg_newton_step": [P, P, P, P, i64, P, i32, i32, f64, f64, f64, P, P],
     "twg_collapse_batch": [P, P, i32, i32, i32],
     "twg_collapse_autotune": [P, P, i32, P, P, P],
     "twg_collapse_plan_create": [P, P, i32, i32, i32, ctypes.POINTER(P)],
@@ -90,6 +91,13 @@ TWG_CV1_BOUNDARY = 2
 TWG_CV2_SURFACE = 4
 TWG_CV2_BOUNDARY = 8
 TWG_CV1_UNROUNDED = 16
+
+
+class NewtonStepOut(ctypes.Structure):
+    """twg_newton_step_out"""
+    _fields_ = [("energy", f64), ("J", f64 * 3), ("H", f64 * 9), ("X", (f64 * 3) * 20),
+                ("new_energy", f64 * 20), ("ok", ctypes.c_int32), ("finite", ctypes.c_uint8 * 20),
+                ("flip", ctypes.c_uint8 * 20)]
 
 
 class CollapseDesc(ctypes.Structure):

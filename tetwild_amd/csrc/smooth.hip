@@ -22,6 +22,7 @@
 // statuses are identical to the FP64 path.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <tuple>
 
@@ -272,6 +273,23 @@ __device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const int
     for (int s = 0; s < m; s += TILE) {
         int j = s + c.tl;
         if (j < m && !f) {
+            Tet T;
+            load_ring_tet(A, rt[j], v, X, false, T);
+            f = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
+        }
+    }
+    return tile_any(c, f);
+}
+
+// ring_flip over the ring tets j with skip[j] == 0 (skip may be NULL)
+template <int TILE>
+__device__ bool ring_flip_masked(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt,
+                                 int m, int v, V3<double> X, const uint8_t* skip)
+{
+    bool f = false;
+    for (int s = 0; s < m; s += TILE) {
+        int j = s + c.tl;
+        if (j < m && !f && !(skip && skip[j])) {
             Tet T;
             load_ring_tet(A, rt[j], v, X, false, T);
             f = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
@@ -787,6 +805,65 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 
 __global__ void reset_counter(unsigned* c) { *c = 0u; }
 
+// One NewtonsMethod step laid out for a caller that decides some flip tests
+// itself: NewtonsUpdate at p0 (ring_update, the batch kernel's sums), then
+// all twenty line-search candidates with their finiteness, their flip test
+// over the ring tets not skipped and their ring energy (the batch kernel's
+// functions, so every value is the one its loop would see).  One wave.
+__global__ __launch_bounds__(64) void newton_step_kernel(SmoothArgs A, const int4* rt, int m, int v,
+                                                         V3<double> p0, const uint8_t* skip,
+                                                         twg_newton_step_out* out)
+{
+    __shared__ double red[64 * kQ];
+    TileCtx<64> c;
+    c.tl = threadIdx.x;
+    c.tbase = 0;
+    c.wbase = 0;
+    c.red = red;
+    bool bid = v < 0 || v >= A.nv;
+    for (int j = c.tl; j < m; j += 64) {
+        int4 t = rt[j];
+        bid = bid || t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 ||
+              t.z >= A.nv || t.w < 0 || t.w >= A.nv;
+    }
+    if (tile_any(c, bid)) {
+        if (c.tl == 0) out->ok = -1;
+        return;
+    }
+    double E, J[3], H[9];
+    const bool ok = ring_update<64>(A, c, rt, m, v, p0, E, J, H);
+    if (c.tl == 0) {
+        out->ok = ok ? 1 : 0;
+        out->energy = E;
+        for (int k = 0; k < 3; ++k) out->J[k] = J[k];
+        for (int k = 0; k < 9; ++k) out->H[k] = H[k];
+    }
+    if (!ok) return;
+    double a = 1.0;
+    for (int it = 0; it < 20; ++it) {
+        double rhs[3], Xs[3];
+        rhs[0] = (H[0] * p0.x + H[1] * p0.y + H[2] * p0.z) - a * J[0];
+        rhs[1] = (H[3] * p0.x + H[4] * p0.y + H[5] * p0.z) - a * J[1];
+        rhs[2] = (H[6] * p0.x + H[7] * p0.y + H[8] * p0.z) - a * J[2];
+        solve3_qrcp(H, rhs, Xs);
+        const bool fin = is_finite(Xs[0]) && is_finite(Xs[1]) && is_finite(Xs[2]);
+        bool flip = false;
+        double en = 0.0;
+        if (fin) {
+            const V3<double> X = mk3(Xs[0], Xs[1], Xs[2]);
+            flip = ring_flip_masked<64>(A, c, rt, m, v, X, skip);
+            en = ring_energy<64>(A, c, rt, m, v, X);
+        }
+        if (c.tl == 0) {
+            for (int k = 0; k < 3; ++k) out->X[it][k] = Xs[k];
+            out->finite[it] = fin ? 1 : 0;
+            out->flip[it] = flip ? 1 : 0;
+            out->new_energy[it] = en;
+        }
+        a /= 2.0;
+    }
+}
+
 using KernelFn = void (*)(SmoothArgs);
 
 KernelFn pick(int tile, bool mixed)
@@ -1008,6 +1085,29 @@ int twg_smooth_plan_info(const twg_smooth_plan* P, int* tile, int* chunk)
     TWG_REQUIRE(P != nullptr, "twg_smooth_plan_info: NULL plan");
     if (tile) *tile = P->tile;
     if (chunk) *chunk = (int)P->chunk;
+    return TWG_OK;
+}
+
+int twg_newton_step(twg_ctx* ctx, const double* x, const double* y, const double* z, int64_t nv,
+                    const int32_t* ring_tets, int32_t m, int32_t v, double px, double py,
+                    double pz, const uint8_t* skip, twg_newton_step_out* out)
+{
+    TWG_REQUIRE(ctx != nullptr, "twg_newton_step: ctx is NULL");
+    TWG_REQUIRE(x && y && z && out && (m == 0 || ring_tets), "twg_newton_step: NULL array");
+    TWG_REQUIRE(nv > 0 && nv < (int64_t(1) << 31) && m >= 0, "twg_newton_step: bad size");
+    TWG_REQUIRE(v >= 0 && v < nv, "twg_newton_step: v out of range");
+    TWG_REQUIRE(((uintptr_t)ring_tets & 15) == 0, "twg_newton_step: ring_tets must be 16-byte aligned");
+    if (int r = bind(ctx)) return r;
+    SmoothArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.x = x;
+    A.y = y;
+    A.z = z;
+    A.nv = nv;
+    hipLaunchKernelGGL(newton_step_kernel, dim3(1), dim3(64), 0, ctx->cur, A,
+                       reinterpret_cast<const int4*>(ring_tets), (int)m, (int)v, mk3(px, py, pz),
+                       skip, out);
+    TWG_LAUNCHED();
     return TWG_OK;
 }
 

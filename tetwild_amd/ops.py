@@ -184,6 +184,26 @@ class Context:
                                       _ptr(q) if q is not None else None), "twg_tet_groups")
         return fl[:ng], mq[:ng], (q[:n_tets] if q is not None else None)
 
+    def newton_step(self, ring_tets, v, x, y, z, p, skip=None):
+        """One NewtonsMethod step of vertex v at p over ring_tets (device
+        tensors): NewtonsUpdate and the twenty line-search candidates with
+        their finiteness, flip test (ring tets with skip == 0) and energy.
+        Returns a _capi.NewtonStepOut (host copy)."""
+        m = self._tets(ring_tets)
+        nv = self._verts(x, y, z)
+        if not 0 <= int(v) < nv:
+            raise ValueError("newton_step: v out of range")
+        if skip is not None:
+            skip = self._dev(skip, torch.uint8, "skip", (m,))
+        out = torch.empty(ctypes.sizeof(C.NewtonStepOut), dtype=torch.uint8, device=self.device)
+        self._bind()
+        C.check(self.L.twg_newton_step(self.h, _ptr(x), _ptr(y), _ptr(z), nv, _ptr(ring_tets), m,
+                                       int(v), float(p[0]), float(p[1]), float(p[2]),
+                                       _ptr(skip) if skip is not None else None, _ptr(out)),
+                "twg_newton_step")
+        host = out.cpu().numpy()
+        return C.NewtonStepOut.from_buffer_copy(host.tobytes())
+
     def energy_stats(self, slim_energy, removed=None, tets=None, locked=None, thres=10.0):
         """(sum, max, second_max, count, buckets[11]) over device arrays"""
         self._dev(slim_energy, torch.float64, "slim_energy", (None,))

@@ -17,9 +17,14 @@
 //
 // mesh.bin (little-endian): int64 nv, nt, nsf, nbf, is_mesh_closed;
 // double eps, eps_2, sampling_dist; double posf[nv][3]; uint8 flags[nv]
-// (1 surface, 2 bbox, 4 boundary, 8 locked); int32 tets[nt][4];
+// (1 surface, 2 bbox, 4 boundary, 8 locked, 16 unrounded); int32 tets[nt][4];
 // int32 is_surface_fs[nt][4]; double tet_quality[nt];
-// double sf_tri[nsf][9]; double b_tri[nbf][9].
+// double sf_tri[nsf][9]; double b_tri[nbf][9]; then, for the unrounded
+// vertices in vertex order, their exact positions double pos[][3].  The
+// exact positions here are doubles, so the driver's ExactPositions orients
+// them with the library's exact predicate (twg_tet_flip: positive or not).
+// When the mesh has unrounded vertices, seq / batch also write
+// <out.bin>.rounded: uint8 is_rounded[nv] after the run.
 // out.bin: double posf[nv][3]; double slim_energy[nt]; int64 counter,
 // suc_counter; plus, for the first up-to-8 interior vertices, NewtonsUpdate
 // (E, J[3], H[9]) and getNewEnergy at the final positions, and the static
@@ -30,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -42,6 +48,47 @@ static bool rd(FILE* f, T* p, size_t n)
 {
     return fread(p, sizeof(T), n, f) == n;
 }
+
+// Exact positions of the unrounded vertices (doubles in these files): an
+// orientation is the exact sign of the determinant on the device; the
+// mirrors only ask whether it is positive.
+class DoubleExact : public ExactPositions {
+public:
+    DoubleExact(Device& dev, const std::vector<TetVertex>& tv, std::vector<Point_3f> pos)
+        : dev_(dev), tv_(tv), pos_(std::move(pos))
+    {
+    }
+    int orientation(const std::array<int, 4>& t, int moved, const Point_3f& at) override
+    {
+        double xyz[12];
+        for (int j = 0; j < 4; ++j) {
+            const int v = t[j];
+            const Point_3f& p = v == moved ? at : (tv_[v].is_rounded ? tv_[v].posf : pos_[v]);
+            xyz[j] = p[0];
+            xyz[4 + j] = p[1];
+            xyz[8 + j] = p[2];
+        }
+        const int32_t ids[4] = {0, 1, 2, 3};
+        char* b = (char*)dev_.buffer(5, 256);
+        dev_.h2d(b, xyz, 96);
+        dev_.h2d(b + 96, ids, 16);
+        uint8_t* fl = (uint8_t*)(b + 128);
+        if (twg_tet_flip(dev_.ctx(), (const int32_t*)(b + 96), 1, (double*)b, (double*)b + 4,
+                         (double*)b + 8, 4, fl) != TWG_OK)
+            throw std::runtime_error(std::string("twg_tet_flip: ") + twg_last_error());
+        uint8_t f = 1;
+        dev_.d2h(&f, fl, 1);
+        calls++;
+        return f ? -1 : 1;
+    }
+    void rounded(int v) override { pos_[v] = tv_[v].posf; }
+    long calls = 0;
+
+private:
+    Device& dev_;
+    const std::vector<TetVertex>& tv_;
+    std::vector<Point_3f> pos_;
+};
 
 int main(int argc, char** argv)
 {
@@ -64,14 +111,22 @@ int main(int argc, char** argv)
         !rd(f, (int32_t*)tets.data(), 4 * nt) || !rd(f, (int32_t*)fs.data(), 4 * nt) ||
         !rd(f, q.data(), nt) || !rd(f, sf.data(), 9 * nsf) || !rd(f, bf.data(), 9 * nbf))
         return 2;
-    fclose(f);
+    std::vector<Point_3f> exact_pos(nv);
+    int64_t n_unrounded = 0;
     for (int64_t v = 0; v < nv; ++v) {
         tv[v].posf = {{pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]}};
         tv[v].is_on_surface = flags[v] & 1;
         tv[v].is_on_bbox = flags[v] & 2;
         tv[v].is_on_boundary = flags[v] & 4;
         tv[v].is_locked = flags[v] & 8;
+        tv[v].is_rounded = !(flags[v] & 16);
+        exact_pos[v] = tv[v].posf;
+        if (!tv[v].is_rounded) {
+            if (!rd(f, exact_pos[v].data(), 3)) return 2;
+            n_unrounded++;
+        }
     }
+    fclose(f);
     for (int64_t t = 0; t < nt; ++t)
         for (int j = 0; j < 4; ++j) tv[tets[t][j]].conn_tets.insert((int)t);
     std::vector<TetQuality> tq(nt);
@@ -93,6 +148,7 @@ int main(int argc, char** argv)
     try {
         Device dev(0);
         Envelope sf_tree(dev, SV, SF), b_tree(dev, BV, BF);
+        DoubleExact exact(dev, tv, exact_pos);
         State st;
         st.eps = eps[0];
         st.eps_2 = eps[1];
@@ -102,6 +158,7 @@ int main(int argc, char** argv)
         if (mode == "energy") {
             for (int64_t t = 0; t < nt; ++t) t_rm[t] = t % 7 == 3;
             LocalOperations lo(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+            lo.exact = n_unrounded ? &exact : nullptr;
             double avg, mx;
             lo.getAvgMaxEnergy(avg, mx);
             bool clean = false;
@@ -113,6 +170,7 @@ int main(int argc, char** argv)
         }
         if (mode == "remove") {
             EdgeRemover er(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+            er.exact = n_unrounded ? &exact : nullptr;
             std::vector<std::array<int, 2>> edges;
             for (const auto& t : tets)
                 for (int a = 0; a < 4; ++a)
@@ -162,6 +220,7 @@ int main(int argc, char** argv)
         }
         if (mode == "collapse" || mode == "collapse_soft") {
             EdgeCollapser ec(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+            ec.exact = n_unrounded ? &exact : nullptr;
             if (mode == "collapse_soft") {
                 ec.is_soft = true;
                 ec.soft_energy = 40;
@@ -186,16 +245,19 @@ int main(int argc, char** argv)
                 for (const auto& t : q) fwrite(&t.slim_energy, 8, 1, o);
             for (int64_t v = 0; v < nv; ++v) {
                 uint8_t fl = (tv[v].is_on_surface ? 1 : 0) | (tv[v].is_on_bbox ? 2 : 0) |
-                             (tv[v].is_on_boundary ? 4 : 0) | (tv[v].is_locked ? 8 : 0);
+                             (tv[v].is_on_boundary ? 4 : 0) | (tv[v].is_locked ? 8 : 0) |
+                             (tv[v].is_rounded ? 0 : 16);
                 fwrite(&fl, 1, 1, o);
             }
             fclose(o);
             int ok = 0;
             for (int c : codes) ok += c == EdgeCollapser::SUCCESS || c == EdgeCollapser::ENVELOP_SUC;
-            printf("{\"candidates\": %zu, \"collapsible\": %d}\n", edges.size(), ok);
+            printf("{\"candidates\": %zu, \"collapsible\": %d, \"exact_calls\": %ld}\n",
+                   edges.size(), ok, exact.calls);
             return 0;
         }
         VertexSmoother vs(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
+        vs.exact = n_unrounded ? &exact : nullptr;
         int64_t counters[2];
         if (mode == "seq") {
             vs.smooth();
@@ -232,8 +294,18 @@ int main(int argc, char** argv)
         double e0 = LocalOperations::comformalAMIPSEnergy_new(T);
         fwrite(&e0, 8, 1, o);
         fclose(o);
-        printf("{\"counter\": %lld, \"suc_counter\": %lld, \"max_energy\": %.17g}\n",
-               (long long)counters[0], (long long)counters[1], vs.getMaxEnergy());
+        if (n_unrounded) {
+            FILE* r = fopen((std::string(argv[2]) + ".rounded").c_str(), "wb");
+            if (!r) return 2;
+            for (int64_t v = 0; v < nv; ++v) {
+                uint8_t b = tv[v].is_rounded ? 1 : 0;
+                fwrite(&b, 1, 1, r);
+            }
+            fclose(r);
+        }
+        printf("{\"counter\": %lld, \"suc_counter\": %lld, \"max_energy\": %.17g, "
+               "\"exact_calls\": %ld}\n",
+               (long long)counters[0], (long long)counters[1], vs.getMaxEnergy(), exact.calls);
     } catch (const std::exception& e) {
         fprintf(stderr, "shim_driver: %s\n", e.what());
         return 1;

@@ -18,6 +18,17 @@ void check(int status, const char* what)
     if (status != TWG_OK)
         throw std::runtime_error(std::string(what) + ": " + twg_last_error());
 }
+
+void replace_id(std::array<int, 4>& t, int from, int to)
+{
+    for (int& v : t)
+        if (v == from) {
+            v = to;
+            return;
+        }
+}
+
+bool contains(const std::array<int, 4>& t, int v) { return std::find(t.begin(), t.end(), v) != t.end(); }
 }  // namespace
 
 // ---- Device --------------------------------------------------------------
@@ -87,7 +98,8 @@ double Envelope::squared_distance(const Point_3f& p, Point_3f* nearest) const
 }
 
 // ---- LocalOperations -------------------------------------------------------
-void LocalOperations::gather(const std::vector<std::array<int, 4>>& ts, Gathered& g, int pin)
+void LocalOperations::gather(const std::vector<std::array<int, 4>>& ts, Gathered& g, int pin,
+                             int ov, const Point_3f* op)
 {
     if (stamp_.size() < tet_vertices.size()) stamp_.assign(tet_vertices.size(), -1);
     g.tets.resize(4 * ts.size());
@@ -106,7 +118,7 @@ void LocalOperations::gather(const std::vector<std::array<int, 4>>& ts, Gathered
     g.y.resize(g.global.size());
     g.z.resize(g.global.size());
     for (size_t k = 0; k < g.global.size(); ++k) {
-        const Point_3f& p = tet_vertices[g.global[k]].posf;
+        const Point_3f& p = g.global[k] == ov && op ? *op : tet_vertices[g.global[k]].posf;
         g.x[k] = p[0];
         g.y[k] = p[1];
         g.z[k] = p[2];
@@ -139,11 +151,53 @@ void LocalOperations::calTetQualities(const std::vector<std::array<int, 4>>& new
     for (size_t i = 0; i < n; ++i) tet_qs[i].slim_energy = e[i];
 }
 
+// src/tetwild/LocalOperations.cpp:535-573
 bool LocalOperations::isFlip(const std::vector<std::array<int, 4>>& new_tets)
+{
+    return isFlipAt(new_tets, -1, nullptr);
+}
+
+bool LocalOperations::hasUnroundedOther(const std::array<int, 4>& t, int v) const
+{
+    for (int u : t)
+        if (u != v && !tet_vertices[u].is_rounded) return true;
+    return false;
+}
+
+ExactPositions& LocalOperations::requireExact(const char* who) const
+{
+    if (!exact)
+        throw std::logic_error(std::string(who) +
+                               ": a tet has an unrounded vertex and no ExactPositions is set "
+                               "(its orientation needs the exact rational positions)");
+    return *exact;
+}
+
+bool LocalOperations::isFlipAt(const std::vector<std::array<int, 4>>& ts, int v, const Point_3f* at)
+{
+    std::vector<std::array<int, 4>> plain;
+    std::vector<size_t> ex;
+    plain.reserve(ts.size());
+    for (size_t i = 0; i < ts.size(); ++i) {
+        if (hasUnroundedOther(ts[i], v)) ex.push_back(i);
+        else plain.push_back(ts[i]);
+    }
+    if (!plain.empty() && posfFlip(plain, v, at)) return true;
+    if (!ex.empty()) {
+        ExactPositions& X = requireExact("isFlip");
+        const Point_3f here = v >= 0 ? (at ? *at : tet_vertices[v].posf) : Point_3f{{0, 0, 0}};
+        for (size_t i : ex)
+            if (X.orientation(ts[i], v, here) != 1) return true;
+    }
+    return false;
+}
+
+bool LocalOperations::posfFlip(const std::vector<std::array<int, 4>>& new_tets, int v,
+                               const Point_3f* at)
 {
     if (new_tets.empty()) return false;
     Gathered g;
-    gather(new_tets, g);
+    gather(new_tets, g, -1, v, at);
     size_t n = new_tets.size(), nv = g.global.size();
     int32_t* dt = (int32_t*)dev.buffer(0, 16 * n);
     double* dx = (double*)dev.buffer(1, 8 * nv);
@@ -162,7 +216,30 @@ bool LocalOperations::isFlip(const std::vector<std::array<int, 4>>& new_tets)
 
 bool LocalOperations::isTetFlip(const std::array<int, 4>& t)
 {
-    return isFlip(std::vector<std::array<int, 4>>{t});
+    return isFlipAt(std::vector<std::array<int, 4>>{t}, -1, nullptr);
+}
+
+void LocalOperations::qualitiesAt(const std::vector<std::array<int, 4>>& ts, int v,
+                                  const Point_3f& at, std::vector<double>& q)
+{
+    q.assign(ts.size(), 0.0);
+    if (ts.empty()) return;
+    Gathered g;
+    gather(ts, g, -1, v, &at);
+    size_t n = ts.size(), nv = g.global.size();
+    int32_t* dt = (int32_t*)dev.buffer(0, 16 * n);
+    double* dx = (double*)dev.buffer(1, 8 * nv);
+    double* dy = (double*)dev.buffer(2, 8 * nv);
+    double* dz = (double*)dev.buffer(3, 8 * nv);
+    double* de = (double*)dev.buffer(4, 8 * n);
+    dev.h2d(dt, g.tets.data(), 16 * n);
+    dev.h2d(dx, g.x.data(), 8 * nv);
+    dev.h2d(dy, g.y.data(), 8 * nv);
+    dev.h2d(dz, g.z.data(), 8 * nv);
+    check(twg_tet_qualities(dev.ctx(), dt, (int64_t)n, dx, dy, dz, (int64_t)nv, TWG_FP64, 0.0, de,
+                            nullptr),
+          "twg_tet_qualities");
+    dev.d2h(q.data(), de, 8 * n);
 }
 
 // src/tetwild/LocalOperations.cpp:575-603
@@ -544,6 +621,10 @@ bool VertexSmoother::NewtonsMethod(const std::vector<int>& t_ids,
                                    const std::vector<std::array<int, 4>>&, int v_id, Point_3f& p)
 {
     (void)t_ids;  // the ring is the vertex's conn_tets (sorted), as oneRing() gathers it
+    std::vector<int> ring;
+    oneRing(v_id, ring);
+    for (int t : ring)
+        if (hasUnroundedOther(tets[t], v_id)) return newtonExact(v_id, ring, p);
     std::vector<uint8_t> st;
     runBatch({Problem{v_id, 3}}, st);
     p = {{last_pout_[0], last_pout_[1], last_pout_[2]}};
@@ -626,22 +707,186 @@ double VertexSmoother::getNewEnergy(const std::vector<int>& t_ids)
     return s;
 }
 
+// smoothSingle / smoothSurface's try-to-round step
+// (src/tetwild/VertexSmoother.cpp:152-161, 272-281): pos = Point_3(posf), kept
+// when no ring tet flips
+void VertexSmoother::tryRound(int v, const std::vector<std::array<int, 4>>& ring)
+{
+    TetVertex& tv = tet_vertices[v];
+    if (tv.is_rounded) return;
+    const Point_3f at = tv.posf;
+    if (!isFlipAt(ring, v, &at)) {
+        tv.is_rounded = true;
+        if (exact) exact->rounded(v);
+    }
+}
+
+// A moved vertex ends rounded: smoothSingle sets pos = p, posf = pf,
+// is_rounded = true (:196-211); smoothSurface rounds it after the envelope
+// test (:441-447), whose isFlip repeats the one the move already passed.
+void VertexSmoother::afterMove(int v, int status)
+{
+    TetVertex& tv = tet_vertices[v];
+    if (status == TWG_SMOOTH_MOVED && !tv.is_rounded) {
+        tv.is_rounded = true;
+        if (exact) exact->rounded(v);
+    }
+}
+
+int VertexSmoother::smoothOne(int v, int kind)
+{
+    std::vector<int> ring;
+    oneRing(v, ring);
+    std::vector<std::array<int, 4>> rt;
+    rt.reserve(ring.size());
+    for (int t : ring) rt.push_back(tets[t]);
+    tryRound(v, rt);
+    bool needs_exact = false;
+    for (const auto& t : rt) needs_exact = needs_exact || hasUnroundedOther(t, v);
+    int s;
+    if (needs_exact) {
+        s = smoothExact(v, kind, ring);
+    } else {
+        std::vector<uint8_t> st;
+        runBatch({Problem{v, kind}}, st);
+        s = st[0];
+    }
+    afterMove(v, s);
+    return s;
+}
+
+// NewtonsMethod (src/tetwild/VertexSmoother.cpp:465-542) for a ring whose flip
+// tests partly need ExactPositions: one twg_newton_step launch per step (its
+// device flip tests skip the exact tets), the halving loop on the host.
+bool VertexSmoother::newtonExact(int v, const std::vector<int>& ring, Point_3f& p)
+{
+    ExactPositions& X = requireExact("NewtonsMethod");
+    std::vector<std::array<int, 4>> rt;
+    for (int t : ring) rt.push_back(tets[t]);
+    const size_t m = rt.size();
+    std::vector<uint8_t> skip(m);
+    for (size_t j = 0; j < m; ++j) skip[j] = hasUnroundedOther(rt[j], v) ? 1 : 0;
+    Gathered g;
+    gather(rt, g, v);
+    const size_t nv = g.global.size();
+    char* b0 = (char*)dev.buffer(0, 24 * nv + 16 * m + m + sizeof(twg_newton_step_out) + 64);
+    double* dx = (double*)b0;
+    int32_t* dt = (int32_t*)(dx + 3 * nv);  // 24 nv is a multiple of 8: round up to 16
+    dt = (int32_t*)(((uintptr_t)dt + 15) & ~(uintptr_t)15);
+    uint8_t* dskip = (uint8_t*)(dt + 4 * m);
+    twg_newton_step_out* dout =
+        (twg_newton_step_out*)(((uintptr_t)(dskip + m) + 15) & ~(uintptr_t)15);
+    dev.h2d(dx, g.x.data(), 8 * nv);
+    dev.h2d(dx + nv, g.y.data(), 8 * nv);
+    dev.h2d(dx + 2 * nv, g.z.data(), 8 * nv);
+    dev.h2d(dt, g.tets.data(), 16 * m);
+    dev.h2d(dskip, skip.data(), m);
+    Point_3f pos = tet_vertices[v].posf;
+    bool is_moved = false;
+    twg_newton_step_out o;
+    for (int step = 0; step < 15; ++step) {
+        check(twg_newton_step(dev.ctx(), dx, dx + nv, dx + 2 * nv, (int64_t)nv, dt, (int32_t)m,
+                              0, pos[0], pos[1], pos[2], dskip, dout),
+              "twg_newton_step");
+        dev.d2h(&o, dout, sizeof o);
+        if (o.ok < 0) throw std::runtime_error("twg_newton_step: bad input");
+        if (!o.ok) break;
+        bool taken = false, tiny = false;
+        for (int k = 0; k < 20 && !taken; ++k) {
+            if (!o.finite[k] || o.flip[k]) continue;
+            const Point_3f Xk = {{o.X[k][0], o.X[k][1], o.X[k][2]}};
+            bool flip = false;
+            for (size_t j = 0; j < m && !flip; ++j)
+                if (skip[j]) flip = X.orientation(rt[j], v, Xk) != 1;
+            if (flip) continue;
+            const double ne = o.new_energy[k];
+            if (ne >= o.energy || std::isinf(ne) || std::isnan(ne)) continue;
+            taken = true;
+            tiny = std::fabs(ne - o.energy) < 1e-5;
+            pos = Xk;
+        }
+        if (taken && tiny) taken = false;
+        if (!taken) {
+            is_moved = step != 0;
+            break;
+        }
+        is_moved = true;
+    }
+    p = pos;
+    return is_moved;
+}
+
+// smoothSingle's (kind 0) or smoothSurface's (kind 1, 2) body after the
+// try-to-round step, host-driven, for a ring with an unrounded vertex other
+// than v; the same statuses and writes as a twg_smooth_batch problem
+int VertexSmoother::smoothExact(int v, int kind, const std::vector<int>& ring)
+{
+    std::vector<std::array<int, 4>> rt;
+    for (int t : ring) rt.push_back(tets[t]);
+    // the validity test orients on posf whatever the rounding (:163-172)
+    if (posfFlip(rt, -1, nullptr)) return TWG_SMOOTH_INVALID;
+    Point_3f pn;
+    if (!newtonExact(v, ring, pn)) return TWG_SMOOTH_NOT_MOVED;
+    TetVertex& tv = tet_vertices[v];
+    if (kind == 0) {
+        if (isFlipAt(rt, v, &pn)) return TWG_SMOOTH_FLIP_END;
+        tv.posf = pn;
+        return TWG_SMOOTH_MOVED;
+    }
+    // projection (:354-362)
+    Point_3f pf;
+    (kind == 2 ? geo_b_tree : geo_sf_tree).squared_distance(pn, &pf);
+    if (isFlipAt(rt, v, &pf)) return TWG_SMOOTH_REJECT_FLIP;
+    double om = 0.0, nm = 0.0;
+    for (int t : ring) om = std::max(om, tet_qualities[t].slim_energy);
+    std::vector<double> nq;
+    qualitiesAt(rt, v, pf, nq);
+    for (double q : nq) nm = std::max(nm, q);
+    if (!(nm < om)) return TWG_SMOOTH_REJECT_QUALITY;
+    // the one-ring surface faces (:316-327) with v at pf (:403-429)
+    std::vector<std::array<int, 3>> tri_ids;
+    for (int t : ring)
+        for (int j = 0; j < 4; ++j)
+            if (tets[t][j] != v && is_surface_fs[t][j] != State::NOT_SURFACE) {
+                std::array<int, 3> tri = {{tets[t][(j + 1) % 4], tets[t][(j + 2) % 4],
+                                           tets[t][(j + 3) % 4]}};
+                std::sort(tri.begin(), tri.end());
+                tri_ids.push_back(tri);
+            }
+    std::sort(tri_ids.begin(), tri_ids.end());
+    tri_ids.erase(std::unique(tri_ids.begin(), tri_ids.end()), tri_ids.end());
+    std::vector<Triangle_3f> tris;
+    for (const auto& tri : tri_ids) {
+        int k = (int)(std::find(tri.begin(), tri.end(), v) - tri.begin());
+        tris.push_back({{pf, tet_vertices[tri[(k + 1) % 3]].posf, tet_vertices[tri[(k + 2) % 3]].posf}});
+    }
+    std::vector<uint8_t> out;
+    areFacesOutEnvelop(tris, out);
+    for (uint8_t o : out)
+        if (o) return TWG_SMOOTH_REJECT_ENVELOPE;
+    tv.posf = pf;
+    for (size_t i = 0; i < ring.size(); ++i) {
+        TetQuality q;
+        q.slim_energy = nq[i];
+        tet_qualities[ring[i]] = q;
+    }
+    return TWG_SMOOTH_MOVED;
+}
+
 // src/tetwild/VertexSmoother.cpp:114-242 in vertex order (Gauss-Seidel)
 void VertexSmoother::smoothSingle()
 {
     counter = 0;
     suc_counter = 0;
-    std::vector<uint8_t> st;
     for (int v = 0; v < (int)tet_vertices.size(); ++v) {
         if (v_is_removed[v]) continue;
         const TetVertex& tv = tet_vertices[v];
         if (tv.is_on_bbox) continue;
         if (state.eps != State::EPSILON_INFINITE && tv.is_on_surface) continue;
         if (tv.is_locked) continue;
-        if (!tv.is_rounded) continue;  // rational vertex: the caller's exact path
         counter++;
-        runBatch({Problem{v, 0}}, st);
-        if (st[0] == TWG_SMOOTH_MOVED || st[0] == TWG_SMOOTH_FLIP_END) suc_counter++;
+        int s = smoothOne(v, 0);
+        if (s == TWG_SMOOTH_MOVED || s == TWG_SMOOTH_FLIP_END) suc_counter++;
     }
     std::vector<std::array<int, 4>> live;
     std::vector<int> ids;
@@ -658,7 +903,6 @@ void VertexSmoother::smoothSingle()
 // src/tetwild/VertexSmoother.cpp:244-463 in vertex order
 void VertexSmoother::smoothSurface()
 {
-    std::vector<uint8_t> st;
     for (int v = 0; v < (int)tet_vertices.size(); ++v) {
         if (v_is_removed[v]) continue;
         TetVertex& tv = tet_vertices[v];
@@ -670,10 +914,9 @@ void VertexSmoother::smoothSurface()
             continue;
         }
         if (!isBoundaryPoint(v)) tv.is_on_boundary = false;
-        if (!tv.is_rounded) continue;
         counter++;
-        runBatch({Problem{v, tv.is_on_boundary ? 2 : 1}}, st);
-        if (st[0] == TWG_SMOOTH_MOVED) suc_counter++;
+        int s = smoothOne(v, tv.is_on_boundary ? 2 : 1);
+        if (s == TWG_SMOOTH_MOVED) suc_counter++;
     }
 }
 
@@ -705,6 +948,43 @@ std::vector<std::vector<int>> VertexSmoother::colourClasses(const std::vector<in
     return classes;
 }
 
+// One colour class: the try-to-round step per vertex, then one batch for the
+// rings the device decides alone and the host-driven path for the others.
+// The vertices of a class share no tet, so the order inside it is free.
+void VertexSmoother::runClass(const std::vector<int>& cls, bool surface, int& suc)
+{
+    std::vector<Problem> probs;
+    std::vector<int> exact_v;
+    for (int v : cls) {
+        std::vector<int> ring;
+        oneRing(v, ring);
+        std::vector<std::array<int, 4>> rt;
+        for (int t : ring) rt.push_back(tets[t]);
+        tryRound(v, rt);
+        bool needs_exact = false;
+        for (const auto& t : rt) needs_exact = needs_exact || hasUnroundedOther(t, v);
+        const int kind = surface ? (tet_vertices[v].is_on_boundary ? 2 : 1) : 0;
+        if (needs_exact) exact_v.push_back(v);
+        else probs.push_back(Problem{v, kind});
+    }
+    std::vector<uint8_t> st;
+    runBatch(probs, st);
+    auto count = [&](int s) {
+        if (s == TWG_SMOOTH_MOVED || (!surface && s == TWG_SMOOTH_FLIP_END)) suc++;
+    };
+    for (size_t i = 0; i < probs.size(); ++i) {
+        afterMove(probs[i].v, st[i]);
+        count(st[i]);
+    }
+    for (int v : exact_v) {
+        std::vector<int> ring;
+        oneRing(v, ring);
+        int s = smoothExact(v, surface ? (tet_vertices[v].is_on_boundary ? 2 : 1) : 0, ring);
+        afterMove(v, s);
+        count(s);
+    }
+}
+
 void VertexSmoother::smoothSingleBatched()
 {
     counter = 0;
@@ -713,18 +993,13 @@ void VertexSmoother::smoothSingleBatched()
     for (int v = 0; v < (int)tet_vertices.size(); ++v) {
         if (v_is_removed[v]) continue;
         const TetVertex& tv = tet_vertices[v];
-        if (tv.is_on_bbox || tv.is_locked || !tv.is_rounded) continue;
+        if (tv.is_on_bbox || tv.is_locked) continue;
         if (state.eps != State::EPSILON_INFINITE && tv.is_on_surface) continue;
         verts.push_back(v);
     }
-    std::vector<uint8_t> st;
     for (const auto& cls : colourClasses(verts)) {
-        std::vector<Problem> probs;
-        for (int v : cls) probs.push_back(Problem{v, 0});
-        runBatch(probs, st);
+        runClass(cls, false, suc_counter);
         counter += (int)cls.size();
-        for (uint8_t s : st)
-            if (s == TWG_SMOOTH_MOVED || s == TWG_SMOOTH_FLIP_END) suc_counter++;
     }
     std::vector<std::array<int, 4>> live;
     std::vector<int> ids;
@@ -751,17 +1026,11 @@ void VertexSmoother::smoothSurfaceBatched()
             continue;
         }
         if (!isBoundaryPoint(v)) tv.is_on_boundary = false;
-        if (!tv.is_rounded) continue;
         verts.push_back(v);
     }
-    std::vector<uint8_t> st;
     for (const auto& cls : colourClasses(verts)) {
-        std::vector<Problem> probs;
-        for (int v : cls) probs.push_back(Problem{v, tet_vertices[v].is_on_boundary ? 2 : 1});
-        runBatch(probs, st);
+        runClass(cls, true, suc_counter);
         counter += (int)cls.size();
-        for (uint8_t s : st)
-            if (s == TWG_SMOOTH_MOVED) suc_counter++;
     }
 }
 
@@ -779,6 +1048,67 @@ int EdgeCollapser::checkCollapse(int v1_id, int v2_id, std::vector<TetQuality>* 
     checkCollapses({{{v1_id, v2_id}}}, codes, new_qs ? &qs : nullptr);
     if (new_qs) *new_qs = qs[0];
     return codes[0];
+}
+
+// collapseAnEdge's checks (src/tetwild/EdgeCollapser.cpp:284-446) after its
+// flag updates, in the reference's order
+int EdgeCollapser::checkCollapseExact(int v1_id, int v2_id, std::vector<TetQuality>* new_qs)
+{
+    const TetVertex &a = tet_vertices[v1_id], &b = tet_vertices[v2_id];
+    const bool degenerate = a.posf == b.posf;  // the edge length is 0
+    if (a.is_on_boundary && !b.is_on_boundary && !degenerate && isPointOutBoundaryEnvelop(b.posf))
+        return ENVELOP;
+    if (a.is_on_surface && !b.is_on_surface && !degenerate && isPointOutEnvelop(b.posf))
+        return ENVELOP;
+    std::vector<int> ring;
+    oneRing(v1_id, ring);
+    std::vector<std::array<int, 4>> new_tets;
+    for (int t : ring) {
+        if (contains(tets[t], v2_id)) continue;
+        std::array<int, 4> nt = tets[t];
+        replace_id(nt, v1_id, v2_id);
+        new_tets.push_back(nt);
+    }
+    if (isFlip(new_tets)) return FLIP;
+    std::vector<TetQuality> qs;
+    calTetQualities(new_tets, qs);
+    if (energy_type != State::ENERGY_NA && is_check_quality) {
+        TetQuality old_tq, new_tq;
+        getCheckQuality(ring, old_tq);
+        getCheckQuality(qs, new_tq);
+        if (is_soft) old_tq.slim_energy = soft_energy;
+        if (!a.is_rounded) new_tq.slim_energy = 0;  // an unroundable vertex goes anyway (:408)
+        if (!degenerate && !new_tq.isBetterOrEqualThan(old_tq)) return QUALITY;
+    }
+    int code = SUCCESS;
+    if (state.eps != State::EPSILON_NA && state.eps != State::EPSILON_INFINITE && a.is_on_surface) {
+        // isCollapsable_epsilon (:727-770)
+        std::vector<std::array<int, 3>> tri_ids;
+        for (int t : ring)
+            for (int j = 0; j < 4; ++j)
+                if (tets[t][j] != v1_id && is_surface_fs[t][j] != State::NOT_SURFACE) {
+                    std::array<int, 3> f = {{tets[t][(j + 1) % 4], tets[t][(j + 2) % 4],
+                                             tets[t][(j + 3) % 4]}};
+                    std::sort(f.begin(), f.end());
+                    tri_ids.push_back(f);
+                }
+        std::sort(tri_ids.begin(), tri_ids.end());
+        tri_ids.erase(std::unique(tri_ids.begin(), tri_ids.end()), tri_ids.end());
+        std::vector<Triangle_3f> tris;
+        for (auto f : tri_ids) {
+            if (std::find(f.begin(), f.end(), v2_id) != f.end()) continue;
+            *std::find(f.begin(), f.end(), v1_id) = v2_id;
+            tris.push_back({{tet_vertices[f[0]].posf, tet_vertices[f[1]].posf, tet_vertices[f[2]].posf}});
+        }
+        std::vector<uint8_t> out;
+        areFacesOutEnvelop(tris, out);
+        if (!degenerate)
+            for (uint8_t o : out)
+                if (o) return ENVELOP;
+        code = ENVELOP_SUC;
+    }
+    if (new_qs) *new_qs = qs;
+    return code;
 }
 
 void EdgeCollapser::checkCollapses(const std::vector<std::array<int, 2>>& edges,
@@ -916,6 +1246,18 @@ void EdgeCollapser::checkCollapses(const std::vector<std::array<int, 2>>& edges,
         if (st[c] == TWG_COLLAPSE_BAD_INPUT)
             throw std::runtime_error("twg_collapse_batch: bad input for edge " +
                                      std::to_string(edges[c][0]) + "-" + std::to_string(edges[c][1]));
+        // a new tet with an unrounded vertex: the flip test needs ExactPositions
+        bool needs_exact = false;
+        for (int t : rings[c]) {
+            if (contains(tets[t], edges[c][1])) continue;
+            std::array<int, 4> nt = tets[t];
+            replace_id(nt, edges[c][0], edges[c][1]);
+            needs_exact = needs_exact || hasUnroundedOther(nt, -1);
+        }
+        if (needs_exact) {
+            codes[c] = checkCollapseExact(edges[c][0], edges[c][1], new_qs ? &(*new_qs)[c] : nullptr);
+            continue;
+        }
         codes[c] = st[c];
         if (!new_qs || (st[c] != SUCCESS && st[c] != ENVELOP_SUC)) continue;
         for (size_t i = 0; i < rings[c].size(); ++i) {
@@ -930,17 +1272,6 @@ void EdgeCollapser::checkCollapses(const std::vector<std::array<int, 2>>& edges,
 
 // ---- EdgeRemover -----------------------------------------------------------
 namespace {
-
-void replace_id(std::array<int, 4>& t, int from, int to)
-{
-    for (int& v : t)
-        if (v == from) {
-            v = to;
-            return;
-        }
-}
-
-bool contains(const std::array<int, 4>& t, int v) { return std::find(t.begin(), t.end(), v) != t.end(); }
 
 // the two vertices of every ring tet other than v1, v2, chained around the
 // edge (n12_v_ids) and the ring positions in chain order (n12_t_ids),
@@ -1102,6 +1433,14 @@ void EdgeRemover::checkRemovals(const std::vector<std::array<int, 2>>& edges,
         dev.d2h(gq.data(), dq, 8 * ntt);
         for (size_t g = 0; g < ng; ++g)
             if (gflip[g] == TWG_GROUP_BAD_INPUT) throw std::runtime_error("twg_tet_groups: bad input");
+        // a group with an unrounded vertex: its flip test through ExactPositions
+        for (size_t g = 0; g < ng; ++g) {
+            bool ex = false;
+            for (int k = goff[g]; k < goff[g + 1] && !ex; ++k) ex = hasUnroundedOther(gt[k], -1);
+            if (!ex) continue;
+            std::vector<std::array<int, 4>> ts(gt.begin() + goff[g], gt.begin() + goff[g + 1]);
+            gflip[g] = isFlipAt(ts, -1, nullptr) ? 1 : 0;
+        }
     }
     // the reference's decisions, edge by edge
     auto better = [&](double nq, double oq) {

@@ -13,8 +13,17 @@
 //   GEO::MeshFacetsAABBWithEps -> tetwild_amd::Envelope (a device AABB tree)
 // Every call gathers the vertices it touches into compact device arrays, so
 // the GPU always sees the host mesh as it is at the time of the call (the
-// driver may mutate the mesh freely between calls).  Vertices must be rounded
-// (is_rounded): rational positions stay on the caller's exact path.
+// driver may mutate the mesh freely between calls).
+//
+// Unrounded vertices (is_rounded false) have an exact rational position
+// (TetVertex::pos, a CGAL Epeck Point_3, in the reference) that this mirror
+// does not hold.  The reference orients a tet on pos whenever one of its
+// vertices is unrounded (isTetFlip, src/tetwild/LocalOperations.cpp:535-553);
+// the mirrors send exactly those tests to the caller's ExactPositions and
+// decide every other one on the device.  A test that needs the exact
+// positions with no ExactPositions set throws std::logic_error rather than
+// falling back to posf.  For a rounded vertex pos == Point_3(posf), as the
+// reference keeps it, so posf stands for pos there.
 #pragma once
 
 #include <array>
@@ -68,6 +77,18 @@ struct State {
     bool use_energy_max = true;
     // Args::filter_energy_thres (include/tetwild/Args.h:63), read by getFilterEnergy
     double filter_energy_thres = 10;
+};
+
+// The caller's exact positions of the unrounded vertices.
+class ExactPositions {
+public:
+    virtual ~ExactPositions() = default;
+    // CGAL::orientation(pos[t[0]], pos[t[1]], pos[t[2]], pos[t[3]]) as +1
+    // (POSITIVE), 0 or -1, where pos of a rounded vertex is its posf and,
+    // when moved >= 0, vertex `moved` sits at the double point `at`.
+    virtual int orientation(const std::array<int, 4>& t, int moved, const Point_3f& at) = 0;
+    // v has just become rounded: pos[v] = Point_3(posf[v]) from now on
+    virtual void rounded(int v) { (void)v; }
 };
 
 // GPU context shared by the objects of one mesh (one device, one stream).
@@ -127,6 +148,9 @@ public:
     int suc_counter = 0;
     // TWG_FP64 (default) or TWG_MIXED: results are identical, mixed screens in FP32
     int precision = TWG_FP64;
+    // exact orientations for tets with unrounded vertices (may stay null on a
+    // mesh whose vertices are all rounded)
+    ExactPositions* exact = nullptr;
 
     LocalOperations(std::vector<TetVertex>& t_vs, std::vector<std::array<int, 4>>& ts,
                     std::vector<std::array<int, 4>>& is_sf_fs, std::vector<bool>& v_is_rm,
@@ -174,8 +198,24 @@ protected:
         std::vector<double> x, y, z;
         std::vector<int> global;  // local -> global vertex id
     };
-    void gather(const std::vector<std::array<int, 4>>& ts, Gathered& g, int pin = -1);
+    // ov >= 0: vertex ov is gathered at *op instead of its posf
+    void gather(const std::vector<std::array<int, 4>>& ts, Gathered& g, int pin = -1, int ov = -1,
+                const Point_3f* op = nullptr);
     std::vector<int> stamp_;  // per-vertex local id (or -1), sized lazily
+
+    // isFlip with vertex v at *at (v < 0: no vertex moved), the reference's
+    // rule per tet: on posf when every vertex other than the moved one is
+    // rounded (the moved one sits at a double point, so that is its exact
+    // orientation), else through ExactPositions
+    bool isFlipAt(const std::vector<std::array<int, 4>>& ts, int v, const Point_3f* at);
+    // some vertex of t other than v is unrounded
+    bool hasUnroundedOther(const std::array<int, 4>& t, int v) const;
+    ExactPositions& requireExact(const char* who) const;
+    // the device part of isFlipAt: orientations on posf (v at *at)
+    bool posfFlip(const std::vector<std::array<int, 4>>& ts, int v, const Point_3f* at);
+    // calTetQualities with v at *at
+    void qualitiesAt(const std::vector<std::array<int, 4>>& ts, int v, const Point_3f& at,
+                     std::vector<double>& q);
 };
 
 class VertexSmoother : public LocalOperations {
@@ -208,6 +248,20 @@ private:
         int v;
         int kind;
     };
+    // one vertex of smoothSingle (kind 0) or smoothSurface (kind 1 / 2) from
+    // the try-to-round step on; returns the TWG_SMOOTH_* status
+    int smoothOne(int v, int kind);
+    // smoothSingle / smoothSurface's try-to-round step
+    // (src/tetwild/VertexSmoother.cpp:152-161, 272-281)
+    void tryRound(int v, const std::vector<std::array<int, 4>>& ring);
+    // a ring with an unrounded vertex other than v: its flip tests go to
+    // ExactPositions, so the problem runs step by step (twg_newton_step)
+    int smoothExact(int v, int kind, const std::vector<int>& ring);
+    bool newtonExact(int v, const std::vector<int>& ring, Point_3f& p);
+    // rounded-vertex bookkeeping after a problem's outcome
+    void afterMove(int v, int status);
+    // one colour class of the batched variants; suc counts its successes
+    void runClass(const std::vector<int>& cls, bool surface, int& suc);
     // runs the problems (all independent) as one batch; returns statuses and
     // writes moved positions / qualities back into the mesh
     void runBatch(const std::vector<Problem>& probs, std::vector<uint8_t>& status);
@@ -247,6 +301,12 @@ public:
 
     // v1's one-ring in the order the mirrors use (sorted tet ids)
     void oneRing(int v_id, std::vector<int>& t_ids) const;
+
+private:
+    // collapseAnEdge's checks in the reference's order on the host, each
+    // through the mirrors' device calls: the candidates whose new tets hold
+    // an unrounded vertex (their flip test needs ExactPositions)
+    int checkCollapseExact(int v1_id, int v2_id, std::vector<TetQuality>* new_qs);
 };
 
 // The decision of EdgeRemover::swap for an edge (src/tetwild/EdgeRemover.h,
