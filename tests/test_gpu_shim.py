@@ -408,3 +408,58 @@ def test_shim_unrounded_matches_replay(ctx, tmp_path, mode):
     Pa, _, _, _, _ = replay_exact(P, P, np.ones(len(P), bool), tets, fs, flags, q, S, eps, d,
                                   mode == "batch")
     assert not np.array_equal(Pa.view(np.int64), Pr.view(np.int64))
+
+
+def test_shim_collapse_unrounded_matches_oracle(tmp_path):
+    """EdgeCollapser::checkCollapses with 25% of the vertices unrounded (exact
+    positions equal to posf here): every candidate whose new tets hold an
+    unrounded vertex takes the host-driven check sequence with its flip test
+    through ExactPositions; codes and new qualities are bit-exact with the
+    oracle (whose quality rule counts an unrounded v1's new quality as 0,
+    src/tetwild/EdgeCollapser.cpp:408)"""
+    assert os.path.exists(DRIVER), "build the shim first (make -C tetwild_amd/shim)"
+    M = W.sheet_mesh(4, seed=8)
+    P = np.stack([M["x"], M["y"], M["z"]], 1)
+    tets = M["tets"]
+    q = O.tet_qualities(tets, M["x"], M["y"], M["z"])
+    eps = 0.05 / 4
+    unr = W.Stream(13, 0).uniform(len(P)) < 0.25
+    flags = (M["on_surface"] * 1 + M["on_boundary"] * 4 + unr * 16).astype(np.uint8)
+    sf = M["sf_V"][M["sf_F"]].reshape(-1, 9)
+    bf = M["b_V"][M["b_F"]].reshape(-1, 9)
+    mesh = tmp_path / "mesh.bin"
+    out = tmp_path / "out.bin"
+    with open(mesh, "wb") as f:
+        f.write(struct.pack("<5q", len(P), len(tets), len(sf), len(bf), 0))
+        f.write(struct.pack("<3d", eps, eps * eps, eps))
+        f.write(np.ascontiguousarray(P).tobytes())
+        f.write(flags.tobytes())
+        f.write(np.ascontiguousarray(tets, dtype=np.int32).tobytes())
+        f.write(np.ascontiguousarray(M["is_surface_fs"], dtype=np.int32).tobytes())
+        f.write(q.tobytes())
+        f.write(np.ascontiguousarray(sf).tobytes())
+        f.write(np.ascontiguousarray(bf).tobytes())
+        f.write(np.ascontiguousarray(P[unr]).tobytes())
+    r = subprocess.run([DRIVER, str(mesh), str(out), "collapse"], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip(), flush=True)
+    assert json.loads(r.stdout)["exact_calls"] > 0
+    B = W.collapse_batch(M, q)
+    B["flags"] = B["flags"] | (unr[B["v1"]] * 16).astype(np.uint8)
+    n = len(B["v1"])
+    raw = open(out, "rb").read()
+    codes = np.frombuffer(raw[:4 * n], dtype=np.int32)
+    cnt = np.frombuffer(raw[4 * n:8 * n], dtype=np.int32)
+    nq = np.frombuffer(raw[8 * n:8 * n + 8 * int(cnt.sum())], dtype=np.float64)
+    fl_after = np.frombuffer(raw[8 * n + 8 * int(cnt.sum()):], dtype=np.uint8)
+    assert np.array_equal(fl_after, flags)
+    so, nqo = O.collapse_batch(B, O.Surface(M["sf_V"], M["sf_F"]), O.Surface(M["b_V"], M["b_F"]),
+                               eps * eps, eps)
+    assert np.array_equal(codes, so.astype(np.int32))
+    ok = np.isin(so, [0, 4])
+    ro = B["ring_off"]
+    expect = [nqo[ro[c]:ro[c + 1]] for c in np.nonzero(ok)[0]]
+    expect = np.concatenate([e[~np.isnan(e)] for e in expect]) if expect else np.zeros(0)
+    assert np.array_equal(nq.view(np.int64), expect.view(np.int64))
+    print("codes", np.bincount(so, minlength=5))
