@@ -20,6 +20,9 @@ Secondary measurements in the same JSON line ("secondary"):
                  jittered 80^3 Kuhn lattice carrying an open surface sheet,
                  autotuned tiles, the batch replayed from a hipGraph, FP64 and
                  mixed
+  config1        the end-to-end workload's icosphere (2562 v / 5120 f, eps 1e-3
+                 diag) in a tet mesh through the C++ mirrors: smoothing,
+                 collapse and removal checks, reductions, envelope
   reduce         the energy reductions (twg_energy_stats) over 10.1M stored
                  qualities, the bulk quality pass (twg_tet_qualities, FP64
                  and mixed) and isFlip (twg_tet_flip) over the 10.1M lattice
@@ -32,7 +35,7 @@ restatement and, for the envelope and the smoothing line search, the counts
 of decisions resting on a near-tie.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
-                  [--only amips,envelope,slivers,smoothing,collapse,reduce]
+                  [--only amips,envelope,slivers,smoothing,collapse,reduce,config1]
 
 Prints ONE JSON line.  The roofline object covers the headline kernel:
 algorithmic bytes per launch / average launch time from HIP events on the
@@ -83,6 +86,43 @@ def cpu_entry(value, value_1core, unit, kind, sample, host):
     return {"value": value, "unit": unit, "cores": host["threads"], "kind": kind,
             "sample": sample, "value_1core": value_1core, "nproc": host["nproc"],
             "cpu_model": host["model"]}
+
+
+L2_PEAK_GBS = 8 * 2048 * 2.4  # 8 XCDs x 2 KiB per clock (CDNA3 figure) x 2.4 GHz
+L2_REQUEST_BYTES = 128        # an L2 (TCC) request moves at most one 128-byte line
+
+
+def counters(kernel):
+    """per-launch counter values of `kernel` measured by rocprofv3 --pmc on
+    this workload (profiles/kernel_counters.json, written from the runs that
+    profiles/kernel_counters.md lists), or None"""
+    path = os.path.join(ROOT, "profiles", "kernel_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("kernels", {}).get(kernel)
+
+
+def counted_roofline(c, seconds):
+    """rates of a launch of `seconds` from its counted work: issued FP64 /
+    FP32 lane-FLOPs (VALU FLOP counters x 64 lanes: inactive lanes of an
+    issued instruction count too), HBM-side traffic (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a plain copy) and L2 requests"""
+    if c is None:
+        return {}
+    return {
+        "counted_fp64_flops_per_launch": c["fp64_flops"],
+        "counted_fp64_tflops": c["fp64_flops"] / seconds / 1e12,
+        "counted_fp64_frac": c["fp64_flops"] / seconds / 1e12 / FP64_PEAK_TFLOPS,
+        "counted_fp32_tflops": c["fp32_flops"] / seconds / 1e12,
+        "valu_lane_utilisation": c["valu_lane_util"],
+        "traffic": c["fabric_bytes"],
+        "hbm_side_gbs": c["fabric_bytes"] / seconds / 1e9,
+        "l2_gbs": c["l2_requests"] * L2_REQUEST_BYTES / seconds / 1e9,
+        "l2_peak_gbs": L2_PEAK_GBS,
+        "l2_hit_rate": c["l2_hit_rate"],
+        "counters": "profiles/kernel_counters.json",
+    }
 
 
 def log(msg):
@@ -148,6 +188,11 @@ def bench_amips(ctx, args, result):
     if os.path.exists(prof):
         with open(prof) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    c = counters("amips_eval_kernel<2>")
+    if c is not None:
+        roofline["counted_fp64_flops_per_launch"] = c["fp64_flops"]
+        roofline["counted_fp64_tflops"] = c["fp64_flops"] / kern / 1e12
+        roofline["counted_fp64_frac"] = c["fp64_flops"] / kern / 1e12 / FP64_PEAK_TFLOPS
     result.update({
         "value": n / wall, "unit": "tet-evals/s", "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3,
@@ -278,6 +323,17 @@ def bench_envelope(ctx, args, result, surf_cache):
         "alg_bytes_per_check": alg,
         "alg_gbs_fp64": alg * nq / rates["fp64"][1] / 1e9,
     }
+    # the dominant kernel (faces_list_kernel): HBM-side and L2 rates from its
+    # counted traffic over the HIP-event time of the whole call (Morton
+    # ordering included); its FP64 lane-FLOP rate
+    kern64 = rates["fp64"][1]
+    rl = {"bound": "latency (dependent box / facet reads)", "kernel": "faces_list_kernel<false, 80>",
+          "avg_launch_ms": kern64 * 1e3, "unit": "GB/s", "peak": HBM_PEAK_GBS}
+    rl.update(counted_roofline(counters("faces_list_kernel<false, 80>"), kern64))
+    if "hbm_side_gbs" in rl:
+        rl["achieved"] = rl["hbm_side_gbs"]
+        rl["frac"] = rl["achieved"] / HBM_PEAK_GBS
+    sec["roofline"] = rl
     if not args.no_cpu:
         log("[cpu] timing the CPU baseline sample")
         from oracle import oracle as O
@@ -402,6 +458,14 @@ def bench_smoothing(ctx, args, result, surf_cache):
         sec[f"tile_{name}"] = tile
         sec[f"chunk_{name}"] = chunk
         sec[f"graph_ms_{name}"] = kern * 1e3
+        if prec == 0:
+            rl = {"bound": "fp64 valu", "kernel": "smooth_kernel<64, false>",
+                  "avg_launch_ms": kern * 1e3, "unit": "TFLOP/s", "peak": FP64_PEAK_TFLOPS}
+            rl.update(counted_roofline(counters("smooth_kernel<64, false>"), kern))
+            if "counted_fp64_tflops" in rl and tile == 64:
+                rl["achieved"] = rl["counted_fp64_tflops"]
+                rl["frac"] = rl["counted_fp64_frac"]
+            sec["roofline"] = rl
         plan.close()
     torch.cuda.synchronize()
     st = batch.status.cpu().numpy()
@@ -538,6 +602,89 @@ def bench_collapse(ctx, args, result):
     result["secondary"]["collapse"] = sec
 
 
+def bench_config1(ctx, args, result):
+    """The end-to-end CPU workload's surface -- icosphere subdivided 4x (2562
+    vertices, 5120 triangles, radius 1) jittered radially by N(0, 0.002),
+    eps = 1e-3 bbox diagonal, stage-1 eps = eps (1 - 1/sqrt 3), sampling
+    distance eps, ideal edge diagonal / 20 -- embedded in a tet mesh
+    (workloads.icosphere_tetmesh), through the C++ mirrors of the in-scope
+    operators of the refinement loop: smooth() in vertex order and the
+    batched colour-class variant, EdgeCollapser checks on every directed edge,
+    EdgeRemover decisions on every interior edge of 3..5 tets, the energy
+    reductions, plus the envelope test of every surface-vertex face and every
+    vertex.  The reference pipeline itself (CGAL, geogram, Eigen, libigl)
+    cannot be built here, so its own numbers are not measured."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    from tetwild_amd import driver as D
+    from tetwild_amd import workloads as W
+
+    M = W.icosphere_tetmesh(levels=4, sigma=0.002, seed=0)
+    P = np.stack([M["x"], M["y"], M["z"]], 1)
+    tets = M["tets"]
+    tt = torch.from_numpy(tets).to(ctx.device)
+    tx, ty, tz = (torch.from_numpy(M[k]).to(ctx.device) for k in "xyz")
+    q = ctx.tet_qualities(tt, tx, ty, tz)[0].cpu().numpy()
+    flags = (M["on_surface"] * 1 + M["on_bbox"] * 2).astype(np.uint8)
+    sf = M["sf_V"][M["sf_F"]].reshape(-1, 9)
+    bf = M["b_V"][M["b_F"]].reshape(-1, 9)
+    d = D.tempdir()
+    mesh = os.path.join(d, "config1.bin")
+    D.write_mesh(mesh, P, tets, M["is_surface_fs"], flags, q, sf, bf, M["eps"], M["eps_2"],
+                 M["sampling_dist"], closed=True)
+    sec = {"surface_vertices": int(len(M["sf_V"])), "surface_triangles": int(len(M["sf_F"])),
+           "vertices": int(len(P)), "tets": int(len(tets)), "bbox_diag": M["diag"],
+           "eps_input": M["eps_input"], "eps": M["eps"], "sampling_dist": M["sampling_dist"],
+           "ideal_edge": M["edge_len"], "energy_before": {"max": float(q.max()),
+                                                          "avg": float(q.mean())},
+           "reference": "not measurable: the reference pipeline needs CGAL, geogram, Eigen "
+                        "and libigl, absent here"}
+    nv, nt = len(P), len(tets)
+    for mode in ("seq", "batch"):
+        info, out, el = D.run(mesh, mode)
+        raw = np.fromfile(out, dtype=np.float64)
+        qn = raw[3 * nv:3 * nv + nt]
+        Pn = raw[:3 * nv].reshape(nv, 3)
+        sec[f"smooth_{mode}"] = {"counter": info["counter"], "suc_counter": info["suc_counter"],
+                                 "moved": int((Pn != P).any(1).sum()),
+                                 "energy_after": {"max": float(qn.max()), "avg": float(qn.mean())},
+                                 "seconds": el}
+    info, out, el = D.run(mesh, "collapse")
+    codes = np.fromfile(out, dtype=np.int32, count=info["candidates"])
+    sec["collapse"] = {"candidates": info["candidates"], "seconds": el,
+                       "codes": dict(zip(["success", "flip", "quality", "envelop", "envelop_suc"],
+                                         np.bincount(codes, minlength=5).tolist()))}
+    info, out, el = D.run(mesh, "remove")
+    sec["remove"] = {"edges": info["edges"], "removable": info["removable"], "seconds": el}
+    info, _, el = D.run(mesh, "energy")
+    sec["energy_stats_tets_t_mod_7_ne_3"] = info
+    # envelope decisions: faces with three surface vertices, every vertex
+    surf = ctx.surface(M["sf_V"], M["sf_F"])
+    on = M["on_surface"]
+    faces = set()
+    for j in range(4):
+        f = np.sort(np.stack([tets[:, (j + k) % 4] for k in (1, 2, 3)], 1), 1)
+        faces |= set(map(tuple, f[on[f].all(1)]))
+    faces = np.array(sorted(faces), dtype=np.int64)
+    Q = np.ascontiguousarray(P[faces].reshape(-1, 9))
+    fo = ctx.envelope_faces(surf, torch.from_numpy(Q).to(ctx.device), M["eps_2"],
+                            M["sampling_dist"]).cpu().numpy()
+    po = ctx.envelope_points(surf, torch.from_numpy(np.ascontiguousarray(P)).to(ctx.device),
+                             M["eps_2"]).cpu().numpy()
+    surf.close()
+    sec["envelope"] = {"faces": int(len(Q)), "faces_out": int(fo.sum()), "points": int(nv),
+                       "points_out": int(po.sum())}
+    if not args.no_cpu:
+        S = O.Surface(M["sf_V"], M["sf_F"])
+        fr, _ = S.face_out(Q, M["eps_2"], M["sampling_dist"])
+        sec["envelope"]["decision_mismatch_vs_cpu"] = int(np.count_nonzero(fr != fo)) + \
+            int(np.count_nonzero(S.point_out(P, M["eps_2"]) != po))
+        sec["quality_bit_mismatch_vs_cpu"] = bits_mismatch(q, O.tet_qualities(tets, M["x"], M["y"],
+                                                                               M["z"]))
+    result["secondary"]["config1"] = sec
+
+
 def bench_reduce(ctx, args, result):
     """twg_energy_stats over the 119^3 lattice's qualities (10.1M tets, 10%
     removed, 1% of vertices locked) and twg_tet_groups over the same tets in
@@ -630,7 +777,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller secondary workloads")
     ap.add_argument("--cubes", type=int, default=119)
-    ap.add_argument("--only", default="amips,envelope,slivers,smoothing,collapse,reduce")
+    ap.add_argument("--only", default="amips,envelope,slivers,smoothing,collapse,reduce,config1")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -689,6 +836,8 @@ def main():
         stage("collapse", bench_collapse)
     if "reduce" in only:
         stage("reduce", bench_reduce)
+    if "config1" in only:
+        stage("config1", bench_config1)
     ctx.close()
     print(json.dumps(result), flush=True)
 
