@@ -144,6 +144,16 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* bvh, const double* tri9, int
                        double eps2, double sampling_dist, int precision, uint8_t* out,
                        unsigned long long* stats);
 
+/* Debug counters of the dense envelope batches (no counterpart in the
+ * reference): while enabled, the LDS-list kernels of twg_envelope_faces and
+ * twg_envelope_points count their waves that tested the wave's facet list
+ * (out[0]) and those that walked the tree instead (out[1]: the list
+ * overflowed, or a coordinate is not finite).  enable = 1 (re)starts the
+ * counts at zero (16 bytes of device memory, allocated once per context);
+ * enable = 0 stops counting, waits for the context's stream and writes the
+ * counts to out (host, may be NULL).  Decisions do not depend on it. */
+int twg_ctx_list_stats(twg_ctx* ctx, int enable, unsigned long long* out);
+
 /* MeshFacetsAABBWithEps::nearest_facet (src/tetwild/geogram/mesh_AABB.h:130-141,
  * mesh_AABB.cpp:381-480) with the reference's traversal order: closest point
  * cp3[i], squared distance d2[i] and facet id (caller's numbering after
@@ -220,6 +230,18 @@ typedef struct twg_smooth_desc {
  * identical to TWG_FP64. */
 int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* desc, int precision, int tile,
                      int chunk);
+/* twg_smooth_batch without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as twg_smooth_batch; its only scratch, the persistent
+ * kernel's work-queue counter, lives in a 256-byte file-scope device array
+ * (one per device, held until the process ends) that every launch clears
+ * with a kernel first.  Limit: n below 2^31 - 2^24 problems, as for
+ * twg_smooth_batch.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU (several streams or contexts) race; serialise
+ * them. */
+int twg_smooth_batch_noalloc(twg_ctx* ctx, const twg_smooth_desc* desc, int precision, int tile,
+                             int chunk);
 /* Times every (tile, chunk) pair on this batch and remembers the fastest for
  * batches of a similar size on this device.  The outputs are written (by the
  * timing runs) exactly as twg_smooth_batch would write them. */

@@ -162,10 +162,15 @@ def test_dense_faces_wave_lists(ctx, dev):
     ref, _ = O.Surface(V, F).face_out(Q, eps * eps, eps)
     assert 0.2 < ref.mean() < 0.8
     for precision in (0, 1):
+        ctx.list_stats(True)
         out = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision).cpu().numpy()
+        listed, walked = ctx.list_stats(False)
         walk, _ = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision, stats=True)
         assert np.array_equal(out, ref), precision
         assert np.array_equal(walk.cpu().numpy(), ref), precision
+        # the list kernel ran, and most of its waves tested their lists
+        print(f"precision {precision}: {listed} waves listed, {walked} walked", flush=True)
+        assert listed + walked == (n + 255) // 256 * 4 and listed > 0.9 * (listed + walked)
 
 
 def test_dense_points_wave_lists(ctx, dev):
@@ -184,10 +189,17 @@ def test_dense_points_wave_lists(ctx, dev):
     Pn = P.copy()
     Pn[::9973, 1] = np.nan
     for precision in (0, 1):
-        for Q in (P, Pn):
+        for Q, nan in ((P, False), (Pn, True)):
+            ctx.list_stats(True)
             out = ctx.envelope_points(surf, dev_t(dev, Q), eps * eps, precision=precision).cpu().numpy()
+            listed, walked = ctx.list_stats(False)
             walk, _ = ctx.envelope_points(surf, dev_t(dev, Q), eps * eps, precision=precision, stats=True)
             assert np.array_equal(out, walk.cpu().numpy()), precision
+            # most waves tested their lists; the NaN points' waves walked the tree
+            print(f"precision {precision} nan {nan}: {listed} listed, {walked} walked", flush=True)
+            assert listed + walked == (n + 255) // 256 * 4 and listed > 0.9 * (listed + walked)
+            if nan:
+                assert walked > 0
         assert np.array_equal(ctx.envelope_points(surf, dev_t(dev, P), eps * eps,
                                                   precision=precision).cpu().numpy(), ref), precision
 

@@ -47,9 +47,11 @@ SIGNATURES = {
     "twg_envelope_points": [P, P, P, i64, f64, i32, P, P],
     "twg_envelope_faces": [P, P, P, i64, f64, f64, i32, P, P],
     "twg_nearest_points": [P, P, P, i64, P, P, P],
+    "twg_ctx_list_stats": [P, i32, P],
     "twg_sample_count": [P, P, i64, f64, P],
     "twg_sample_write": [P, P, i64, f64, P, i64, P],
     "twg_smooth_batch": [P, P, i32, i32, i32],
+    "twg_smooth_batch_noalloc": [P, P, i32, i32, i32],
     "twg_smooth_autotune": [P, P, i32, P, P, P],
     "twg_smooth_plan_create": [P, P, i32, i32, i32, ctypes.POINTER(P)],
     "twg_smooth_plan_run": [P],

@@ -84,6 +84,12 @@ int twg_ctx_create(int device, twg_ctx** out)
         return twg::hip_fail(e, "twg_ctx_create", __FILE__, __LINE__);
     }
     c->cur = c->own;
+    if (int r = twg::noalloc_scratch_init(c)) {
+        (void)hipEventDestroy(c->order);
+        (void)hipStreamDestroy(c->own);
+        delete c;
+        return r;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     *out = c;
@@ -98,6 +104,7 @@ int twg_ctx_destroy(twg_ctx* ctx)
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (int i = 0; i < 4; ++i)
         if (ctx->slot[i]) (void)hipFree(ctx->slot[i]);
+    if (ctx->list_stats) (void)hipFree(ctx->list_stats);
     if (ctx->order) (void)hipEventDestroy(ctx->order);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;

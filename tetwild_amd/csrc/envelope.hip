@@ -61,7 +61,8 @@ template <bool F32, int CAP>
 __global__ __launch_bounds__(kBlock) void points_list_kernel(BvhView B, const double* __restrict__ p3,
                                                              int64_t n, double eps2, double T,
                                                              double sqrtT, uint8_t* __restrict__ out,
-                                                             const int32_t* __restrict__ order)
+                                                             const int32_t* __restrict__ order,
+                                                             unsigned long long* lstat)
 {
     constexpr int kWaveBytes = CAP * (32 + 8 + 72);
     __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
@@ -100,6 +101,7 @@ __global__ __launch_bounds__(kBlock) void points_list_kernel(BvhView B, const do
     // boxes (min / max ignore NaN): then the whole wave walks the tree
     const bool fin = !act || (is_finite(p.x) && is_finite(p.y) && is_finite(p.z));
     const int nl = __ballot(!fin) ? -1 : collect_facets<64>(W, lo, hi, lst, fr, CAP, lane, 0);
+    if (lstat && lane == 0) atomicAdd(lstat + (nl >= 0 ? 0 : 1), 1ull);
     for (int e = lane; e < nl; e += 64) {
         const double* f = B.tri + 9 * (int64_t)lst[e].leaf;
 #pragma unroll
@@ -159,7 +161,7 @@ template <bool F32, int CAP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void faces_list_kernel(
     BvhView B, const double* __restrict__ tri9, int64_t n, double eps2, double d, double T,
     double sqrtT, uint8_t* __restrict__ out, unsigned* n_pending, int64_t* pending,
-    const int32_t* __restrict__ order)
+    const int32_t* __restrict__ order, unsigned long long* lstat)
 {
     constexpr int kWaveBytes = CAP * (32 + 8 + 72);
         __shared__ __align__(16) unsigned char lists[(kBlock / 64) * kWaveBytes];
@@ -213,6 +215,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 #pragma unroll
         for (int k = 0; k < 9; ++k) fin = fin && is_finite(t[k]);
     const int nl = __ballot(!fin) ? -1 : collect_facets<64>(W, lo, hi, lst, fr, CAP, lane, 0);
+    if (lstat && lane == 0) atomicAdd(lstat + (nl >= 0 ? 0 : 1), 1ull);
     for (int e = lane; e < nl; e += 64) {
         const double* f = B.tri + 9 * (int64_t)lst[e].leaf;
 #pragma unroll
@@ -311,6 +314,11 @@ __global__ __launch_bounds__(kBlock) void faces_exact_kernel(BvhView B,
 }
 
 __global__ void zero_u32(unsigned* p) { *p = 0u; }
+__global__ void zero_u64x2(unsigned long long* p)
+{
+    p[0] = 0ull;
+    p[1] = 0ull;
+}
 
 // batches at least this large are walked in Morton order of their queries
 constexpr int64_t kSortMin = 1 << 16;
@@ -616,9 +624,9 @@ int twg_envelope_points(twg_ctx* ctx, const twg_bvh* B, const double* p3, int64_
         if (int r = morton_order(ctx, p3, n, 3, B->lo, B->hi, &order)) return r;
     if (order && !stats && n >= kPointListDensity * B->nf) {
         if (precision == TWG_FP64)
-            hipLaunchKernelGGL((points_list_kernel<false, kPointListCap>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, order);
+            hipLaunchKernelGGL((points_list_kernel<false, kPointListCap>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, order, ctx->list_stats_on ? ctx->list_stats : nullptr);
         else
-            hipLaunchKernelGGL((points_list_kernel<true, kPointListCap>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, order);
+            hipLaunchKernelGGL((points_list_kernel<true, kPointListCap>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, order, ctx->list_stats_on ? ctx->list_stats : nullptr);
     } else if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((points_kernel<false, true>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
         else hipLaunchKernelGGL((points_kernel<false, false>), g, b, 0, ctx->cur, v, p3, n, eps2, T, sT, out, stats, order);
@@ -658,9 +666,9 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
             hipLaunchKernelGGL((faces_group_kernel<true, 64>), gw, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
     } else if (order && !stats && n >= kFaceListDensity * B->nf) {
         if (precision == TWG_FP64)
-            hipLaunchKernelGGL((faces_list_kernel<false, kFaceListCap>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
+            hipLaunchKernelGGL((faces_list_kernel<false, kFaceListCap>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order, ctx->list_stats_on ? ctx->list_stats : nullptr);
         else
-            hipLaunchKernelGGL((faces_list_kernel<true, kFaceListCap>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order);
+            hipLaunchKernelGGL((faces_list_kernel<true, kFaceListCap>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, n_pending, pending, order, ctx->list_stats_on ? ctx->list_stats : nullptr);
     } else if (precision == TWG_FP64) {
         if (stats) hipLaunchKernelGGL((faces_kernel<false, true>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
         else hipLaunchKernelGGL((faces_kernel<false, false>), g, b, 0, ctx->cur, v, tri9, n, eps2, sampling_dist, T, sT, out, stats, n_pending, pending, order);
@@ -675,6 +683,30 @@ int twg_envelope_faces(twg_ctx* ctx, const twg_bvh* B, const double* tri9, int64
     else
         hipLaunchKernelGGL((faces_exact_kernel<true>), dim3(eg), b, 0, ctx->cur, v, tri9, eps2, sampling_dist, T, sT, out, n_pending, pending);
     TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+int twg_ctx_list_stats(twg_ctx* ctx, int enable, unsigned long long* out)
+{
+    TWG_REQUIRE(ctx != nullptr, "twg_ctx_list_stats: ctx is NULL");
+    if (int r = bind(ctx)) return r;
+    if (enable) {
+        if (!ctx->list_stats) TWG_HIP(hipMalloc(&ctx->list_stats, 2 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(zero_u64x2, dim3(1), dim3(1), 0, ctx->cur, ctx->list_stats);
+        TWG_LAUNCHED();
+        ctx->list_stats_on = true;
+        return TWG_OK;
+    }
+    unsigned long long h[2] = {0, 0};
+    if (ctx->list_stats) {
+        TWG_HIP(hipStreamSynchronize(ctx->cur));
+        TWG_HIP(hipMemcpy(h, ctx->list_stats, sizeof h, hipMemcpyDeviceToHost));
+    }
+    ctx->list_stats_on = false;
+    if (out) {
+        out[0] = h[0];
+        out[1] = h[1];
+    }
     return TWG_OK;
 }
 

@@ -944,9 +944,26 @@ int enqueue(twg_ctx* ctx, const SmoothArgs& A, int tile, bool mixed, hipStream_t
 // autotuner cache: (device, mixed, size class) -> (tile, chunk)
 TuneCache g_tuned;
 
+// Scratch of twg_smooth_batch_noalloc: the persistent kernel's work-queue
+// counter (4 bytes; 256 reserved).  One copy per device, held until the
+// process ends.  In host code the name is a stand-in, not a device address:
+// noalloc_scratch_init looks the address up (hipGetSymbolAddress) and every
+// launch takes that address.
+constexpr size_t kNoallocScratchBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 int size_class(int64_t n) { return n < 4096 ? 0 : (n < 65536 ? 1 : 2); }
 
 }  // namespace
+
+int noalloc_scratch_init(twg_ctx* ctx)
+{
+    void* ptr = nullptr;
+    TWG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    ctx->noalloc_scratch = ptr;
+    return TWG_OK;
+}
+
 }  // namespace twg
 
 using namespace twg;
@@ -1023,6 +1040,32 @@ int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int 
     unsigned ch = chunk ? (unsigned)chunk : tc;
     SmoothArgs A;
     if (int r = make_args(ctx, d, (unsigned*)ctx->scratch, ch, A)) return r;
+    return enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur);
+}
+
+int twg_smooth_batch_noalloc(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int tile,
+                             int chunk)
+{
+    TWG_REQUIRE(ctx != nullptr, "twg_smooth_batch_noalloc: ctx is NULL");
+    TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED,
+                "twg_smooth_batch_noalloc: bad precision");
+    TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64,
+                "twg_smooth_batch_noalloc: tile must be 0, 16, 32 or 64");
+    TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_batch_noalloc: chunk out of range");
+    TWG_REQUIRE(d != nullptr, "twg_smooth_batch_noalloc: NULL descriptor");
+    // the scratch holds one 32-bit queue counter: problems popped in chunks
+    // of up to 1024 must stay below 2^31 (make_args checks n against that)
+    TWG_REQUIRE(ctx->noalloc_scratch != nullptr, "twg_smooth_batch_noalloc: no scratch address");
+    if (d->n == 0) return TWG_OK;
+    if (int r = bind(ctx)) return r;
+    int tt = 32;
+    unsigned tc = 1u;
+    g_tuned.get(ctx->device, precision == TWG_MIXED, size_class(d->n), tt, tc);
+    int t = tile ? tile : tt;
+    unsigned ch = chunk ? (unsigned)chunk : tc;
+    SmoothArgs A;
+    if (int r = make_args(ctx, d, (unsigned*)ctx->noalloc_scratch, ch, A)) return r;
+    // enqueue zeroes the counter with a kernel (no memset node under capture)
     return enqueue(ctx, A, t, precision == TWG_MIXED, ctx->cur);
 }
 

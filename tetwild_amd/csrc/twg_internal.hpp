@@ -28,6 +28,13 @@ struct twg_ctx {
     // undecided, 1 query ordering keys and values, 2 sort temporary storage
     void* slot[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t slot_bytes[4] = {0, 0, 0, 0};
+    // device address of smooth.hip's file-scope scratch array on this
+    // context's device (twg_smooth_batch_noalloc), looked up at creation
+    void* noalloc_scratch = nullptr;
+    // debug counters of the envelope list kernels (twg_ctx_list_stats):
+    // waves that tested their LDS facet list, waves that walked the tree
+    unsigned long long* list_stats = nullptr;
+    bool list_stats_on = false;
 };
 
 namespace twg {
@@ -49,6 +56,10 @@ int morton_order(twg_ctx* ctx, const double* items, int64_t n, int stride, const
 
 // Activates ctx->device for the calling thread.
 int bind(twg_ctx* ctx);
+
+// With ctx->device current: the device address of the no-allocation
+// smoothing entry point's scratch array, into ctx->noalloc_scratch.
+int noalloc_scratch_init(twg_ctx* ctx);
 
 constexpr int kBlock = 256;
 

@@ -184,6 +184,15 @@ class Context:
                                       _ptr(q) if q is not None else None), "twg_tet_groups")
         return fl[:ng], mq[:ng], (q[:n_tets] if q is not None else None)
 
+    def list_stats(self, enable):
+        """enable=True: start counting the waves of the envelope list kernels;
+        enable=False: stop and return (waves that tested their LDS facet
+        list, waves that walked the tree)"""
+        out = (ctypes.c_ulonglong * 2)()
+        self._bind()
+        C.check(self.L.twg_ctx_list_stats(self.h, 1 if enable else 0, out), "twg_ctx_list_stats")
+        return None if enable else (int(out[0]), int(out[1]))
+
     def newton_step(self, ring_tets, v, x, y, z, p, skip=None):
         """One NewtonsMethod step of vertex v at p over ring_tets (device
         tensors): NewtonsUpdate and the twenty line-search candidates with
