@@ -375,6 +375,19 @@ def test_shim_unrounded_matches_replay(ctx, tmp_path, mode):
     for a in range(3):
         onp = (g[:, a] == 0) | (g[:, a] == n)
         Pex[onp, a] = P[onp, a]
+    # five interior vertices get an exact position reflected through the face
+    # opposite them in one of their tets: that tet is inverted in the exact
+    # geometry, so a neighbour smoothed while such a vertex is unrounded sees
+    # every Newton candidate flip there (it orients on pos)
+    C = conn(tets, len(P))
+    inner = [v for v in range(len(P)) if not flags[v] & 3]
+    for u in inner[5::9][:5]:
+        unr[u] = True
+        t = tets[C[u][0]]
+        a, b, c = (P[w] for w in t if w != u)
+        nrm = np.cross(b - a, c - a)
+        nrm /= np.linalg.norm(nrm)
+        Pex[u] = P[u] - 2 * np.dot(P[u] - a, nrm) * nrm
     q = O.tet_qualities(tets, *(np.ascontiguousarray(P[:, k]) for k in range(3)))
     fl = flags | (unr.astype(np.uint8) * 16)
     mesh = tmp_path / "mesh.bin"
