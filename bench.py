@@ -218,6 +218,16 @@ def bench_amips(ctx, args, result):
         "alg_bytes_per_launch": n * (16 + 8 + 12 + 36 + 1) + nv * 24,
         "hbm_gbs": (n * (16 + 8 + 12 + 36 + 1) + nv * 24) / kern_m / 1e9,
     }
+    # at the energy tolerance 1e-9: FP32 (unit roundoff 6e-8) certifies no
+    # tet, so every tet falls back to FP64
+    tol9 = 1e-9
+    wall_9, kern_9 = events_time(
+        lambda: ctx.amips_eval_mixed(tt, tx, ty, tz, mode=2, tol=tol9, out=(Em, Gm, Hm, fb)),
+        max(2, args.steps // 4), 1)
+    result["secondary"]["amips_mixed"]["at_tol_1e-9"] = {
+        "tol": tol9, "tet_evals_per_s": n / wall_9, "fallback_rate": fb.float().mean().item(),
+        "max_rel_err_E_vs_fp64": ((Em - E).abs() / E.abs()).max().item(),
+        "speedup_vs_fp64": wall / wall_9}
 
     if not args.no_cpu:
         log("[cpu] timing the CPU baseline sample")
@@ -398,6 +408,12 @@ def bench_slivers(ctx, args, result, surf_cache):
         "max_rel_err_fallback_part": rel[fbm].max().item() if fbm.any() else 0.0,
         "quality_sentinel_mismatch": sent_mm,
     }
+    w9, _ = events_time(lambda: ctx.amips_eval_mixed(tt, tx, ty, tz, 2, 1e-9, out=(Em, Gm, Hm, fb)),
+                        steps, 1)
+    sec["at_tol_1e-9"] = {"tol": 1e-9, "tet_evals_per_s": n / w9, "speedup_vs_fp64": w64 / w9,
+                          "fallback_rate": fb.float().mean().item(),
+                          "max_rel_err_E_vs_fp64":
+                              ((Em - E).abs() / E.abs())[torch.isfinite(E)].max().item()}
     del E, G, H, Em, Gm, Hm, tt, tx, ty, tz
     # point envelope samples clustered within +-1% of eps
     V, F = surf_cache["VF"]

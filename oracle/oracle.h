@@ -20,9 +20,12 @@
  *                         rational brute force on small inputs.  Bit-level
  *                         agreement with geogram's point_triangle_squared_distance
  *                         (not vendored in the reference): parity unpinned.
- *   orc_sample_triangle   invariants (vertices present, samples in the
- *                         triangle, spacing, count formula).  Bit-level
- *                         agreement with geogram vec3 arithmetic: parity unpinned.
+ *   orc_sample_triangle   an independent 50-digit model of the hexagonal
+ *                         lattice (rows, even-row bump, break rule, side
+ *                         edges, integer-ratio and tie rules), sample by
+ *                         sample (tests/test_oracle_sampling.py, with planted
+ *                         mistakes), plus invariants.  Bit-level agreement
+ *                         with geogram vec3 arithmetic: parity unpinned.
  *   orc_envelope_*        brute force over all facets (exact-rational decisions
  *                         away from the eps^2 rounding band).
  *   orc_collapse_check    composition of the pinned parts above (orient3d,
@@ -41,9 +44,18 @@
  *                         and the decision the quality comparison
  *                         (tests/test_oracle_remove.py).
  *   orc_newton_*          invariants (monotone energy, no flips, step bounds,
- *                         is_moved semantics, X = X0 - a H^-1 J).  Bit-level
- *                         agreement with Eigen's ColPivHouseholderQR:
- *                         parity unpinned.
+ *                         is_moved semantics, X = X0 - a H^-1 J); MAX_STEP,
+ *                         MAX_IT and the 1e-5 stop by constructed rings
+ *                         (tests/test_oracle_newton.py, with planted
+ *                         mistakes).  Bit-level agreement with Eigen's
+ *                         ColPivHouseholderQR: parity unpinned.
+ *   orc_smooth_vertex     the rounding rules: equal to orc_smooth_problem when
+ *                         every vertex is rounded, constructed try-to-round
+ *                         and inverted-exact-tet cases
+ *                         (tests/test_oracle_smooth.py).
+ *   orc_*_mt, risk counts the sequential loops' results bit for bit; the
+ *                         band counts on constructed distances
+ *                         (tests/test_oracle_baseline.py).
  */
 #ifndef TWG_ORACLE_H
 #define TWG_ORACLE_H
