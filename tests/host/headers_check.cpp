@@ -281,6 +281,19 @@ static void check_lists(std::mt19937_64& rng)
                 for (int f = 0; f < nf; ++f) ok = ok && (!exact[f] || in_list[f]);
                 CHECK(ok, "lists: %d facets, mode %d, query %d: list differs from the box query\n", nf, mode, q);
                 lists++;
+                // half the lists record edge neighbours for in_envelope_list's
+                // second try (any list positions: a facet within eps decides
+                // "in" whichever facet it is, so arbitrary ones must not change
+                // a decision)
+                if (q % 2 == 1 && n < 256)
+                    for (int j = 0; j < n; ++j) {
+                        uint32_t pk = 0xff000000u;
+                        for (int k = 0; k < 3; ++k) {
+                            const uint32_t v = rng() % 4 == 0 ? 0xffu : (uint32_t)(rng() % n);
+                            pk |= v << (8 * k);
+                        }
+                        lst[j].pad = (int32_t)pk;
+                    }
                 // points of the unpadded box against brute force
                 std::vector<double> tri(9 * (size_t)std::max(n, 1));
                 for (int j = 0; j < n; ++j)
@@ -292,7 +305,9 @@ static void check_lists(std::mt19937_64& rng)
                     for (int k = 0; k < 3; ++k) *pc[k] = lo[k] + pad + (hi[k] - lo[k] - 2 * pad) * U(rng);
                     double best = INFINITY;
                     for (int f = 0; f < nf; ++f) best = std::min(best, facet_d2(W, f, p));
-                    const bool got = in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hl);
+                    const bool got = q % 2 == 1
+                                         ? in_envelope_list<true>(lst.data(), tri.data(), n, p, eps2, sqrtT, hl)
+                                         : in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hl);
                     CHECK(got == (best <= eps2), "lists: %d facets, mode %d, query %d, point %d\n", nf, mode, q, r);
                     if (r % 7 == 0) hl = -1;
                     points++;

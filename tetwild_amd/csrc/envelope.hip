@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 #pragma unroll
         for (int k = 0; k < 9; ++k) tri[9 * e + k] = f[k];
     }
-    wave_sync();
+    list_neighbours<64>(W, lst, nl, lane);
     if (act) out[i] = face_out_listed<F32>(W, t, eps2, d, T, sqrtT, lst, tri, nl) ? 1 : 0;
 }
 
@@ -290,9 +290,9 @@ __global__ __launch_bounds__(kBlock) void faces_group_kernel(BvhView B, const do
 #pragma unroll
             for (int q = 0; q < 9; ++q) tri[9 * e + q] = f[q];
         }
-        wave_sync();
+        list_neighbours<64>(walk(B), lst, nl, r);
     }
-    const bool o = face_out_group<F32, G>(W, ld3(t), ld3(t + 3), ld3(t + 6), eps2, d, T, sqrtT, r,
+    const bool o = face_out_group<F32, G, true, true>(W, ld3(t), ld3(t + 3), ld3(t + 6), eps2, d, T, sqrtT, r,
                                           gbase, lst, tri, nl);
     if (r == 0) out[i] = o ? 1 : 0;
 }
@@ -514,6 +514,59 @@ int twg_bvh_build(twg_ctx* ctx, const double* V, int64_t nv, const int32_t* F, i
         }
     }
     B->rmax = rmax;
+    // edge adjacency in tree order: corners identified by their coordinates'
+    // bits (facets of a triangle soup share edges too); edges of one vertex
+    // twice are skipped; an edge of more than two facets links them in a cycle
+    std::vector<int32_t> adj(3 * nf, -1);
+    {
+        const int64_t nc = 3 * nf;
+        auto bits = [&](int64_t c, int k) {
+            uint64_t u;
+            std::memcpy(&u, &tri[3 * c + k], 8);
+            return u;
+        };
+        std::vector<int64_t> corner(nc);
+        for (int64_t c = 0; c < nc; ++c) corner[c] = c;
+        std::sort(corner.begin(), corner.end(), [&](int64_t a, int64_t b) {
+            for (int k = 0; k < 3; ++k)
+                if (bits(a, k) != bits(b, k)) return bits(a, k) < bits(b, k);
+            return a < b;
+        });
+        std::vector<int64_t> vid(nc);
+        int64_t id = -1;
+        for (int64_t i = 0; i < nc; ++i) {
+            const int64_t c = corner[i];
+            if (i == 0 || bits(c, 0) != bits(corner[i - 1], 0) || bits(c, 1) != bits(corner[i - 1], 1) ||
+                bits(c, 2) != bits(corner[i - 1], 2))
+                ++id;
+            vid[c] = id;
+        }
+        struct Edge {
+            int64_t a, b;
+            int64_t slot;  // 3 facet + k
+        };
+        std::vector<Edge> es;
+        es.reserve(nc);
+        for (int64_t f = 0; f < nf; ++f)
+            for (int k = 0; k < 3; ++k) {
+                const int64_t u = vid[3 * f + k], w = vid[3 * f + (k + 1) % 3];
+                if (u != w) es.push_back({std::min(u, w), std::max(u, w), 3 * f + k});
+            }
+        std::sort(es.begin(), es.end(), [](const Edge& x, const Edge& y) {
+            return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : x.slot < y.slot);
+        });
+        for (size_t i = 0; i < es.size();) {
+            size_t j = i;
+            while (j < es.size() && es[j].a == es[i].a && es[j].b == es[i].b) ++j;
+            if (j - i >= 2)
+                for (size_t r = i; r < j; ++r) {
+                    const size_t o = r + 1 < j ? r + 1 : i;
+                    adj[es[r].slot] = (int32_t)(es[o].slot / 3);
+                    if (j - i == 2) adj[es[o].slot] = (int32_t)(es[r].slot / 3);
+                }
+            i = j;
+        }
+    }
     B->nnode = max_node(1, 0, nf) + 1;
     std::vector<BvhNode64> nodes(B->nnode);
     for (auto& N : nodes) {
@@ -550,16 +603,20 @@ int twg_bvh_build(twg_ctx* ctx, const double* V, int64_t nv, const int32_t* F, i
     if (!r) e = hipMalloc(&B->node64, sizeof(BvhNode64) * B->nnode);
     if (!r && e == hipSuccess) e = hipMalloc(&B->node32, sizeof(BvhNode32) * B->nnode);
     if (!r && e == hipSuccess) e = hipMalloc(&B->tri, sizeof(double) * 9 * nf);
+    if (!r && e == hipSuccess) e = hipMalloc(&B->adj, sizeof(int32_t) * 3 * nf);
     if (!r && e == hipSuccess)
         e = hipMemcpy(B->node64, nodes.data(), sizeof(BvhNode64) * B->nnode, hipMemcpyHostToDevice);
     if (!r && e == hipSuccess)
         e = hipMemcpy(B->node32, n32.data(), sizeof(BvhNode32) * B->nnode, hipMemcpyHostToDevice);
     if (!r && e == hipSuccess)
         e = hipMemcpy(B->tri, tri.data(), sizeof(double) * 9 * nf, hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess)
+        e = hipMemcpy(B->adj, adj.data(), sizeof(int32_t) * 3 * nf, hipMemcpyHostToDevice);
     if (r || e != hipSuccess) {
         if (B->node64) (void)hipFree(B->node64);
         if (B->node32) (void)hipFree(B->node32);
         if (B->tri) (void)hipFree(B->tri);
+        if (B->adj) (void)hipFree(B->adj);
         delete B;
         return r ? r : hip_fail(e, "twg_bvh_build upload", __FILE__, __LINE__);
     }
@@ -574,6 +631,7 @@ int twg_bvh_destroy(twg_bvh* B)
     (void)hipDeviceSynchronize();
     if (B->node64) (void)hipFree(B->node64);
     if (B->node32) (void)hipFree(B->node32);
+    if (B->adj) (void)hipFree(B->adj);
     if (B->tri) (void)hipFree(B->tri);
     delete B;
     return TWG_OK;

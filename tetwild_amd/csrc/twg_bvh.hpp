@@ -46,6 +46,8 @@ struct twg_bvh {
     BvhNode64* node64 = nullptr;
     BvhNode32* node32 = nullptr;
     double* tri = nullptr;  // nf x 9, tree order
+    // nf x 3, tree order: the facet across edge k (vertices k, k+1), -1 none
+    int32_t* adj = nullptr;
     std::vector<int32_t> order;  // tree position -> caller's facet id
 };
 
@@ -89,8 +91,9 @@ struct BvhView {
     const double* __restrict__ tri;
     int64_t nf;
     int64_t nnode;  // node slots (array length)
+    const int32_t* __restrict__ adj;  // nf x 3 edge neighbours (tree positions)
 };
-static_assert(sizeof(BvhView) == 40, "BvhView: kernel-argument layout");
+static_assert(sizeof(BvhView) == 48, "BvhView: kernel-argument layout");
 
 // What the traversals read: the view plus an optional copy of the top of the
 // tree in LDS (node slots [0, ncache), the levels every traversal walks
@@ -103,6 +106,7 @@ struct BvhWalk {
     const double* __restrict__ tri;
     int64_t nf;
     int64_t nnode;
+    const int32_t* __restrict__ adj;
     LdsNode64* c64;
     LdsNode32* c32;
     int64_t ncache;
@@ -120,6 +124,7 @@ __device__ __forceinline__ BvhWalk walk(const BvhView& B)
     w.tri = B.tri;
     w.nf = B.nf;
     w.nnode = B.nnode;
+    w.adj = B.adj;
     w.c64 = nullptr;
     w.c32 = nullptr;
     w.ncache = 0;
@@ -391,7 +396,7 @@ __device__ __forceinline__ void put_lds(LdsNode32W* d, const BvhNode32& s)
         d->hi[k] = s.hi[k];
     }
     d->leaf = s.leaf;
-    d->pad = 0;
+    d->pad = -1;  // no neighbours recorded (list_neighbours fills them)
 }
 
 // The facets whose boxes meet the box [lo, hi], found by the G lanes of a
@@ -510,6 +515,31 @@ __device__ int collect_facets(const BvhWalk& W, const double lo[3], const double
 typedef __attribute__((address_space(3))) double LdsF64;
 typedef __attribute__((address_space(3))) const double LdsF64c;
 
+// For each entry of a collect_facets list (lanes gl, gl + G, ...), the list
+// positions of its facet's three edge neighbours (the tree's adjacency),
+// packed 8 bits each into the entry's pad field; 0xff where the neighbour is
+// not in the list or the tree has no adjacency.  Lists of up to 255 entries.
+// Every lane of the group calls it; it ends with the group's writes visible.
+template <int G>
+__device__ void list_neighbours(const BvhWalk& W, LdsNode32W* lst, int nl, int gl)
+{
+    if (W.adj && nl > 0 && nl < 256) {
+        for (int e = gl; e < nl; e += G) {
+            const int64_t f = lst[e].leaf;
+            const int a0 = W.adj[3 * f], a1 = W.adj[3 * f + 1], a2 = W.adj[3 * f + 2];
+            uint32_t q0 = 0xffu, q1 = 0xffu, q2 = 0xffu;
+            for (int i = 0; i < nl; ++i) {
+                const int li = lst[i].leaf;
+                q0 = li == a0 ? (uint32_t)i : q0;
+                q1 = li == a1 ? (uint32_t)i : q1;
+                q2 = li == a2 ? (uint32_t)i : q2;
+            }
+            lst[e].pad = (int32_t)(q0 | (q1 << 8) | (q2 << 16) | 0xff000000u);
+        }
+    }
+    wave_sync();
+}
+
 // facet_d2 of a triangle held in LDS (9 doubles): the same computation on
 // the same values
 __device__ __forceinline__ double tri_d2(LdsF64c* t, V3<double> p)
@@ -529,10 +559,28 @@ __device__ __forceinline__ double tri_d2(LdsF64c* t, V3<double> p)
 // triangle distance at every position where some lane's box passes.)  tri
 // holds the listed facets' triangles, 9 doubles per position.  On a hit hl
 // becomes the facet's position.
+template <bool NB = false>
 __device__ __forceinline__ bool in_envelope_list(LdsNode32W* lst, LdsF64c* tri, int nl,
                                                  V3<double> p, double eps2, double sqrtT, int& hl)
 {
-    if (hl >= 0 && tri_d2(tri + 9 * hl, p) <= eps2) return true;
+    int nb[3] = {-1, -1, -1};
+    if (hl >= 0) {
+        if (tri_d2(tri + 9 * hl, p) <= eps2) return true;
+        // a sample that left its predecessor's facet is usually within eps of
+        // one of that facet's edge neighbours (NB: the list records them)
+        const uint32_t pk = NB ? (uint32_t)lst[hl].pad : 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t q = (pk >> (8 * k)) & 0xffu;
+            if (q != 0xffu) {
+                if (tri_d2(tri + 9 * q, p) <= eps2) {
+                    hl = (int)q;
+                    return true;
+                }
+                nb[k] = (int)q;
+            }
+        }
+    }
     const V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
     const float Tf = f32_threshold(p, sqrtT);
     for (int base = 0; base < nl; base += 64) {
@@ -548,6 +596,9 @@ __device__ __forceinline__ bool in_envelope_list(LdsNode32W* lst, LdsF64c* tri, 
                 if (k0 + u < n) m |= (uint64_t)(box_d2(pf, e[u].lo, e[u].hi) <= Tf) << (k0 + u);
         }
         if (hl >= base && hl < base + n) m &= ~(1ull << (hl - base));
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (nb[k] >= base && nb[k] < base + n) m &= ~(1ull << (nb[k] - base));
         while (m) {
             const int i = base + __builtin_ctzll(m);
             m &= m - 1;
@@ -795,7 +846,7 @@ __device__ bool face_out_listed(const BvhWalk& B, const double* t9, double eps2,
     }
     int hl = -1;
     return for_each_sample(a, b, c, d, [&](V3<double> s) {
-        return !in_envelope_list(lst, tri, nl, s, eps2, sqrtT, hl);
+        return !in_envelope_list<true>(lst, tri, nl, s, eps2, sqrtT, hl);
     });
 }
 
@@ -811,7 +862,7 @@ __device__ bool face_out_listed(const BvhWalk& B, const double* t9, double eps2,
 // sample, so it equals face_out_sampled's.  For batches of few triangles
 // with many samples each, where one lane per triangle leaves most of the
 // machine idle.
-template <bool F32, int G, bool SUBTREE = true>
+template <bool F32, int G, bool SUBTREE = true, bool NB = false>
 __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<double> c,
                                double eps2, double d, double T, double sqrtT, int r, int gbase,
                                LdsNode32W* lst = nullptr, LdsF64c* tri = nullptr, int nl = -1)
@@ -838,7 +889,7 @@ __device__ bool face_out_group(const BvhWalk& B, V3<double> a, V3<double> b, V3<
     for (int64_t j = 0; j < K; ++j) {
         V3<double> q = a;
         const bool have = first + j < total && g.next(q);
-        const bool out = have && !(nl >= 0 ? in_envelope_list(lst, tri, nl, q, eps2, sqrtT, hl)
+        const bool out = have && !(nl >= 0 ? in_envelope_list<NB>(lst, tri, nl, q, eps2, sqrtT, hl)
                                            : in_envelope<F32, false, SUBTREE>(B, q, eps2, T, sqrtT, hint, st));
         if ((__ballot(out) >> gbase) & gmask) return true;
     }
@@ -862,12 +913,13 @@ inline BvhView bvh_view(const twg_bvh* b)
     v.tri = b->tri;
     v.nf = b->nf;
     v.nnode = b->nnode;
+    v.adj = b->adj;
     return v;
 }
 
 inline BvhView no_bvh()
 {
-    BvhView v = {nullptr, nullptr, nullptr, 0, 0};
+    BvhView v = {nullptr, nullptr, nullptr, 0, 0, nullptr};
     return v;
 }
 
