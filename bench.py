@@ -28,6 +28,9 @@ Secondary measurements in the same JSON line ("secondary"):
                  and mixed) and isFlip (twg_tet_flip) over the 10.1M lattice
                  tets, and the configuration check (twg_tet_groups) over
                  2.5M groups of four tets
+  amips_sequence (--only sequence; not in the default run) per-launch times
+                 of the headline kernel in back-to-back sequences started on
+                 an idle GPU, beside torch's copy and fill kernels
 Each carries its CPU baseline (the reference's compiled AMIPS functions, or
 the CPU restatement, over every host thread OpenMP is given, with the
 one-core rate beside it; bounded samples), its parity counts against that
@@ -35,11 +38,13 @@ restatement and, for the envelope and the smoothing line search, the counts
 of decisions resting on a near-tie.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
-                  [--only amips,envelope,slivers,smoothing,collapse,reduce,config1]
+                  [--only amips,envelope,slivers,smoothing,collapse,reduce,config1,sequence]
 
 Prints ONE JSON line.  The roofline object covers the headline kernel:
 algorithmic bytes per launch / average launch time from HIP events on the
-launch stream.
+launch stream over the timed steps; roofline.sequence holds the median launch
+of a further 300 back-to-back launches (the steady state: on an idle GPU the
+first ~70 launches of this kernel run up to 40 % slower).
 """
 import argparse
 import json
@@ -149,6 +154,23 @@ def events_time(fn, steps, warmup):
     return wall / steps, ev0.elapsed_time(ev1) / 1e3 / steps
 
 
+def launch_sequence(fn, reps):
+    """GPU seconds of each of `reps` back-to-back launches of fn (HIP events
+    around every launch on the current stream)"""
+    import numpy as np
+    import torch
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    torch.cuda.synchronize()
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return np.array([a.elapsed_time(b) / 1e3 for a, b in ev])
+
+
 def bits_mismatch(a, b):
     import numpy as np
     return int(np.count_nonzero(np.ascontiguousarray(a).view(np.int64) !=
@@ -193,6 +215,19 @@ def bench_amips(ctx, args, result):
         roofline["counted_fp64_flops_per_launch"] = c["fp64_flops"]
         roofline["counted_fp64_tflops"] = c["fp64_flops"] / kern / 1e12
         roofline["counted_fp64_frac"] = c["fp64_flops"] / kern / 1e12 / FP64_PEAK_TFLOPS
+    # A longer back-to-back sequence: some 2 ms after memory-heavy work starts
+    # on an idle GPU, launches slow down by up to 40 % and recover over the
+    # next ~15 ms; the median launch of the sequence is the steady state.
+    seq = launch_sequence(launch, 300)
+    med = float(np.median(seq))
+    roofline["sequence"] = {
+        "launches": len(seq), "median_launch_ms": med * 1e3,
+        "mean_launch_ms": float(seq.mean()) * 1e3,
+        "slowest_launch_ms": float(seq.max()) * 1e3,
+        "mean_last_100_ms": float(seq[-100:].mean()) * 1e3,
+        "median_achieved_gbs": alg_bytes / med / 1e9,
+        "median_frac": alg_bytes / med / 1e9 / HBM_PEAK_GBS,
+    }
     result.update({
         "value": n / wall, "unit": "tet-evals/s", "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3,
@@ -535,6 +570,67 @@ def bench_smoothing(ctx, args, result, surf_cache):
     result["secondary"]["smoothing"] = sec
 
 
+def bench_sequence(ctx, args, result):
+    """Per-launch times of amips_eval_kernel<2> on the 119^3 lattice in
+    back-to-back sequences started on an idle GPU (200 ms host sleep before
+    each), after a 10 ms spin kernel, with a 2 ms host pause between launches,
+    and of torch's copy (1.25 GB moved) and fill (1.05 GB written) kernels.
+    Not part of the default run (--only sequence)."""
+    import numpy as np
+    import torch
+    from tetwild_amd import workloads as W
+
+    dev = ctx.device
+    x, y, z, tets, _ = W.kuhn_lattice(args.cubes, jitter=0.15, seed=0)
+    n, nv = len(tets), len(x)
+    tx, ty, tz = (torch.from_numpy(a).to(dev) for a in (x, y, z))
+    tt = torch.from_numpy(tets).to(dev)
+    out = (torch.empty(n, dtype=torch.float64, device=dev),
+           torch.empty((n, 3), dtype=torch.float64, device=dev),
+           torch.empty((n, 9), dtype=torch.float64, device=dev))
+    launch = ctx.amips_launcher(tt, tx, ty, tz, 2, out)
+    alg = n * BYTES_PER_TET + nv * BYTES_PER_VERTEX
+    half = alg // 16
+    src = torch.ones(half, dtype=torch.float64, device=dev)
+    dst = torch.empty(half, dtype=torch.float64, device=dev)
+    wr = torch.empty(n * 13, dtype=torch.float64, device=dev)
+
+    def idle():
+        torch.cuda.synchronize()
+        time.sleep(0.2)
+
+    def paused(fn, reps):
+        t = []
+        for _ in range(reps):
+            t.append(launch_sequence(fn, 1)[0])
+            time.sleep(0.002)
+        return np.array(t)
+
+    for _ in range(3):
+        launch()
+    seqs = {}
+    for r in range(3):
+        idle()
+        seqs[f"amips_after_idle_{r}"] = (launch_sequence(launch, 150), alg)
+    idle()
+    torch.cuda._sleep(20_000_000)
+    seqs["amips_after_spin"] = (launch_sequence(launch, 150), alg)
+    idle()
+    seqs["amips_2ms_pauses"] = (paused(launch, 60), alg)
+    idle()
+    seqs["copy_after_idle"] = (launch_sequence(lambda: dst.copy_(src), 150), 2 * half * 8)
+    idle()
+    seqs["fill_after_idle"] = (launch_sequence(lambda: wr.fill_(2.0), 150), n * 104)
+    res = {}
+    for k, (t, nbytes) in seqs.items():
+        us = np.round(t * 1e6, 1)
+        res[k] = {"launches": len(t), "bytes": int(nbytes), "median_us": float(np.median(us)),
+                  "mean_us": float(us.mean()), "max_us": float(us.max()),
+                  "mean_launches_6_to_25_us": float(us[5:25].mean()),
+                  "median_gbs": nbytes / float(np.median(t)) / 1e9, "us": us.tolist()}
+    result["secondary"]["amips_sequence"] = res
+
+
 # ------------------------------------------------------------ (f) --------
 def bench_collapse(ctx, args, result):
     """collapseAnEdge's checks for 1M directed-edge candidates of a jittered
@@ -854,6 +950,8 @@ def main():
         stage("reduce", bench_reduce)
     if "config1" in only:
         stage("config1", bench_config1)
+    if "sequence" in only:
+        stage("sequence", bench_sequence)
     ctx.close()
     print(json.dumps(result), flush=True)
 
