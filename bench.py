@@ -485,6 +485,7 @@ def bench_slivers(ctx, args, result, surf_cache):
 def bench_smoothing(ctx, args, result, surf_cache):
     import numpy as np
     import torch
+    from tetwild_amd import _capi as C
     from tetwild_amd import workloads as W
     from tetwild_amd.ops import SmoothBatch
 
@@ -505,15 +506,17 @@ def bench_smoothing(ctx, args, result, surf_cache):
         tile, chunk, ms = batch.autotune(precision=prec)
         plan = batch.plan(precision=prec)
         wall, kern = events_time(plan.run, max(2, args.steps // 4), 1)
+        split = tile == C.TWG_SMOOTH_TILE_SPLIT
         sec[f"problems_per_s_{name}"] = n / wall
-        sec[f"tile_{name}"] = tile
+        sec[f"tile_{name}"] = "split: 64 lanes per surface problem, 16 per other" if split else tile
         sec[f"chunk_{name}"] = chunk
         sec[f"graph_ms_{name}"] = kern * 1e3
         if prec == 0:
-            rl = {"bound": "fp64 valu", "kernel": "smooth_kernel<64, false>",
+            kname = "smooth_split_kernel<false>" if split else f"smooth_kernel<{tile}, false>"
+            rl = {"bound": "fp64 valu", "kernel": kname,
                   "avg_launch_ms": kern * 1e3, "unit": "TFLOP/s", "peak": FP64_PEAK_TFLOPS}
-            rl.update(counted_roofline(counters("smooth_kernel<64, false>"), kern))
-            if "counted_fp64_tflops" in rl and tile == 64:
+            rl.update(counted_roofline(counters(kname), kern))
+            if "counted_fp64_tflops" in rl:
                 rl["achieved"] = rl["counted_fp64_tflops"]
                 rl["frac"] = rl["counted_fp64_frac"]
             sec["roofline"] = rl

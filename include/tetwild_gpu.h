@@ -224,17 +224,20 @@ typedef struct twg_smooth_desc {
  * bodies of VertexSmoother::smoothSingle / smoothSurface with NewtonsMethod,
  * NewtonsUpdate and getNewEnergy (src/tetwild/VertexSmoother.h:23-30,
  * src/tetwild/VertexSmoother.cpp:114-702).  Results are bit-identical to the
- * CPU restatement for any tile (16, 32, 64 lanes per problem; 0 = tuned or
- * default) and chunk (problems per queue pop; 0 = tuned or default).
+ * CPU restatement for any tile (16, 32, 64 lanes per problem, or
+ * TWG_SMOOTH_TILE_SPLIT: 64 lanes for each problem of kind 1 or 2, which
+ * the launch takes first, and 16 for the others; 0 = tuned or default) and
+ * chunk (problems per queue pop; 0 = tuned or default).
  * precision TWG_MIXED screens line-search energies in FP32; results are
  * identical to TWG_FP64. */
+#define TWG_SMOOTH_TILE_SPLIT 1
 int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* desc, int precision, int tile,
                      int chunk);
 /* twg_smooth_batch without any device allocation: no counterpart in the
  * reference; for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
  * kernels and results as twg_smooth_batch; its only scratch, the persistent
- * kernel's work-queue counter, lives in a 256-byte file-scope device array
+ * kernel's two work-queue counters, lives in a 256-byte file-scope device array
  * (one per device, held until the process ends) that every launch clears
  * with a kernel first.  Limit: n below 2^31 - 2^24 problems, as for
  * twg_smooth_batch.  All calls on one GPU share that array: calls that

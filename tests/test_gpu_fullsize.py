@@ -6,7 +6,7 @@
 * envelope points: 2M samples in a +-1 % band around eps, every decision;
 * smoothing: the 200k one-ring batch (20 % surface) in full -- status and
   final position bit-exact, new ring qualities bit-exact for moved surface
-  vertices, FP64 and mixed.
+  vertices, FP64 and mixed, 64-lane tiles and the 64/16 split.
 
 Bar: identical decisions and bit-identical positions / qualities."""
 import numpy as np
@@ -14,6 +14,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
+from tetwild_amd import _capi as C
 from tetwild_amd import workloads as W
 from tetwild_amd.ops import SmoothBatch
 
@@ -75,10 +76,10 @@ def test_fullsize_smoothing(ctx, dev, sphere):
     batch = SmoothBatch(ctx, B, sf=surf, bt=surf, eps2=EPS * EPS, sampling_dist=EPS, old_q=old_q)
     moved_surf = np.nonzero((rs == 0) & (B["kind"] > 0))[0]
     assert (B["kind"] > 0).mean() > 0.15 and moved_surf.size > 0.1 * n
-    for prec in (0, 1):
-        print(f"smoothing n={n} precision={prec}", flush=True)
+    for prec, tile in ((0, 64), (1, 64), (0, C.TWG_SMOOTH_TILE_SPLIT)):
+        print(f"smoothing n={n} precision={prec} tile={tile}", flush=True)
         batch.status.fill_(255)
-        batch.run(precision=prec, tile=64, chunk=1)
+        batch.run(precision=prec, tile=tile, chunk=1)
         torch.cuda.synchronize()
         p, s, nq = (t.cpu().numpy() for t in (batch.p_out, batch.status, batch.new_q))
         mism = np.nonzero(s != rs)[0]

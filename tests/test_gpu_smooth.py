@@ -44,7 +44,7 @@ def test_interior_batch_all_tiles(ctx):
     B = W.smoothing_batch(600, seed=11)
     ref = O.smooth_batch(B)
     batch = SmoothBatch(ctx, B)
-    for tile in (16, 32, 64):
+    for tile in (16, 32, 64, C.TWG_SMOOTH_TILE_SPLIT):
         for chunk in (1, 4):
             for prec in (0, 1):
                 print(f"tile={tile} chunk={chunk} precision={prec}", flush=True)
@@ -68,7 +68,8 @@ def test_surface_batch(ctx, surf_data):
     # the oracle projects with trees in the device's facet order (ties)
     ref = O.smooth_batch(B, q, S_sf_order, Sb, EPS2, D)
     batch = SmoothBatch(ctx, B, sf=sf, bt=bt, eps2=EPS2, sampling_dist=D, old_q=q)
-    for tile, prec in ((16, 0), (32, 1), (64, 0)):
+    for tile, prec in ((16, 0), (32, 1), (64, 0), (C.TWG_SMOOTH_TILE_SPLIT, 1),
+                       (C.TWG_SMOOTH_TILE_SPLIT, 0)):
         print(f"tile={tile} precision={prec}", flush=True)
         batch.status.fill_(255)
         batch.new_q.zero_()
@@ -115,19 +116,21 @@ def test_graph_plan_replays(ctx, surf_data):
     q = O.tet_qualities(B["ring_tets"], B["x"], B["y"], B["z"])
     ref = O.smooth_batch(B, q, S, S, EPS2, D)
     batch = SmoothBatch(ctx, B, sf=sf, bt=sf, eps2=EPS2, sampling_dist=D, old_q=q)
-    plan = batch.plan(precision=1, tile=32, chunk=4)
-    for rep in range(3):
-        batch.status.fill_(255)
-        batch.p_out.zero_()
-        torch.cuda.synchronize()
-        plan.run()
-        torch.cuda.synchronize()
-        check(batch, ref, B)
-        # eager work between replays
-        batch.run(precision=0, tile=16, chunk=1)
-        torch.cuda.synchronize()
-        check(batch, ref, B)
-    plan.close()
+    for tile, chunk in ((32, 4), (C.TWG_SMOOTH_TILE_SPLIT, 2)):
+        print(f"plan tile={tile} chunk={chunk}", flush=True)
+        plan = batch.plan(precision=1, tile=tile, chunk=chunk)
+        for rep in range(3):
+            batch.status.fill_(255)
+            batch.p_out.zero_()
+            torch.cuda.synchronize()
+            plan.run()
+            torch.cuda.synchronize()
+            check(batch, ref, B)
+            # eager work between replays
+            batch.run(precision=0, tile=16, chunk=1)
+            torch.cuda.synchronize()
+            check(batch, ref, B)
+        plan.close()
 
 
 def test_newtons_method_only(ctx):

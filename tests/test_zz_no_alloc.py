@@ -46,7 +46,8 @@ def test_noalloc_matches_smooth_batch(ctx):
     B2["kind"] = B["kind"][:k]
     B2["ring_off"] = B["ring_off"][:k + 1]
     B2["tri_off"] = B["tri_off"][:k + 1]
-    for arrays, tile, chunk in ((B, 64, 4), (B2, 32, 1)):
+    for arrays, tile, chunk in ((B, 64, 4), (B2, 32, 1), (B, C.TWG_SMOOTH_TILE_SPLIT, 2),
+                                (B2, C.TWG_SMOOTH_TILE_SPLIT, 1)):
         (s0, p0, q0), (s1, p1, q1) = run_both(ctx, arrays, surf, tile, chunk)
         assert (s0 != 255).all()
         assert np.array_equal(s0, s1)

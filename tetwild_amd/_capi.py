@@ -13,6 +13,8 @@ TWG_EDEVICE = 2
 TWG_ENOMEM = 3
 TWG_FP64 = 0
 TWG_MIXED = 1
+# twg_smooth_batch tile: 64 lanes for each surface problem, 16 for the others
+TWG_SMOOTH_TILE_SPLIT = 1
 TWG_AMIPS_E = 0
 TWG_AMIPS_EG = 1
 TWG_AMIPS_EGH = 2

@@ -14,7 +14,10 @@
 // sequentially -- so every result is independent of TILE and bit-identical
 // to the host restatement.  Tiles pull problems from an atomic queue in
 // chunks, so rings of 4..200 tets balance across the machine.  The 3x3 solve
-// runs redundantly in every lane of the tile (no broadcast needed).
+// runs redundantly in every lane of the tile (no broadcast needed).  With
+// TWG_SMOOTH_TILE_SPLIT the launch runs the problems with an envelope stage
+// on 64-lane tiles first and then the others on 16-lane tiles (the widths
+// each kind runs fastest on).
 //
 // Mixed precision: the line-search energy sums are screened in FP32 with a
 // forward error bound; only sums whose accept/reject or |dE| < 1e-5 decision
@@ -58,7 +61,7 @@ struct SmoothArgs {
     double* __restrict__ p_out;
     uint8_t* __restrict__ status;
     double* __restrict__ new_q;
-    unsigned* counter;
+    unsigned* counter;  // the work queues: [0], and [1] for a split launch
     unsigned chunk;
 };
 
@@ -803,7 +806,75 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
 }
 
-__global__ void reset_counter(unsigned* c) { *c = 0u; }
+// The tile-width split: a problem with the envelope stage (kind 1 or 2) on
+// a 64-lane tile, every other problem on a 16-lane tile, in one launch.  The
+// queue of counter[0] is passed first, its surface problems taken and the
+// rest skipped; then the queue of counter[1], taking the others.  The long
+// surface problems start first and the short ones fill in behind them.  A
+// 16-lane tile's LDS slice lies inside its wave's 64-lane slice, so a wave
+// that moved on never touches a slice another wave still uses.
+__device__ __forceinline__ bool surface_kind(const SmoothArgs& A, int64_t p)
+{
+    const int k = A.kind ? A.kind[p] : 0;
+    return k == 1 || k == 2;
+}
+
+template <int TILE>
+__device__ __forceinline__ void tile_setup(TileCtx<TILE>& c, unsigned char* pool)
+{
+    c.tl = threadIdx.x % TILE;
+    c.tbase = threadIdx.x - c.tl;
+    c.wbase = (threadIdx.x % 64) - c.tl;
+    constexpr int cap = ring_cap<TILE>();
+    unsigned char* slice = pool + (threadIdx.x / TILE) * tile_bytes<TILE>();
+    c.red = (double*)slice;
+    c.lst = (LdsNode32W*)slice;
+    c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
+    c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
+}
+
+template <int TILE, bool MIXED, int WANT>  // WANT 1: surface kinds, 0: the others
+__device__ __forceinline__ void drain(const SmoothArgs& A, const TileCtx<TILE>& c, unsigned* counter)
+{
+    for (;;) {
+        unsigned start = 0;
+        if (c.tl == 0) start = atomicAdd(counter, A.chunk);
+        start = __shfl(start, 0, TILE);
+        if ((int64_t)start >= A.n) break;
+        int64_t end = (int64_t)start + A.chunk;
+        if (end > A.n) end = A.n;
+        for (int64_t p = start; p < end; ++p)
+            if (surface_kind(A, p) == (WANT == 1)) solve_problem<TILE, MIXED>(A, c, p);
+    }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void smooth_split_kernel(SmoothArgs A)
+{
+    __shared__ __align__(16) unsigned char tcache[kSmoothTreeCacheBytes];
+    __shared__ __align__(16) unsigned char pool[kTilePoolBytes];
+    static_assert(tile_bytes<16>() * 4 == tile_bytes<64>(), "16-lane slices inside the wave's");
+    BvhWalk sfw = A.has_sf ? cache_tree_top<false, kSmoothTreeCacheBytes>(A.sf, tcache) : walk(A.sf);
+    {
+        TileCtx<64> c;
+        c.sfw = sfw;
+        tile_setup<64>(c, pool);
+        drain<64, MIXED, 1>(A, c, A.counter);
+    }
+    wave_sync();  // the 64-lane slice's last reads before its 16-lane reuse
+    {
+        TileCtx<16> c;
+        c.sfw = sfw;
+        tile_setup<16>(c, pool);
+        drain<16, MIXED, 0>(A, c, A.counter + 1);
+    }
+}
+
+__global__ void reset_counter(unsigned* c)
+{
+    c[0] = 0u;
+    c[1] = 0u;
+}
 
 // One NewtonsMethod step laid out for a caller that decides some flip tests
 // itself: NewtonsUpdate at p0 (ring_update, the batch kernel's sums), then
@@ -868,6 +939,7 @@ using KernelFn = void (*)(SmoothArgs);
 
 KernelFn pick(int tile, bool mixed)
 {
+    if (tile == TWG_SMOOTH_TILE_SPLIT) return mixed ? smooth_split_kernel<true> : smooth_split_kernel<false>;
     if (tile == 16) return mixed ? smooth_kernel<16, true> : smooth_kernel<16, false>;
     if (tile == 32) return mixed ? smooth_kernel<32, true> : smooth_kernel<32, false>;
     return mixed ? smooth_kernel<64, true> : smooth_kernel<64, false>;
@@ -932,7 +1004,8 @@ int enqueue(twg_ctx* ctx, const SmoothArgs& A, int tile, bool mixed, hipStream_t
 {
     KernelFn k = pick(tile, mixed);
     int blocks = grid_blocks(ctx, k);
-    int64_t need = (A.n * tile + kBlock - 1) / kBlock;  // no more tiles than problems
+    const int lanes = tile == TWG_SMOOTH_TILE_SPLIT ? 64 : tile;
+    int64_t need = (A.n * lanes + kBlock - 1) / kBlock;  // no more tiles than problems
     if (need < blocks) blocks = (int)std::max<int64_t>(need, 1);
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, A.counter);
     TWG_LAUNCHED();
@@ -944,15 +1017,17 @@ int enqueue(twg_ctx* ctx, const SmoothArgs& A, int tile, bool mixed, hipStream_t
 // autotuner cache: (device, mixed, size class) -> (tile, chunk)
 TuneCache g_tuned;
 
-// Scratch of twg_smooth_batch_noalloc: the persistent kernel's work-queue
-// counter (4 bytes; 256 reserved).  One copy per device, held until the
-// process ends.  In host code the name is a stand-in, not a device address:
+// Scratch of twg_smooth_batch_noalloc: the persistent kernel's two
+// work-queue counters (8 bytes; 256 reserved).  One copy per device, held
+// until the process ends.  In host code the name is a stand-in, not a device address:
 // noalloc_scratch_init looks the address up (hipGetSymbolAddress) and every
 // launch takes that address.
 constexpr size_t kNoallocScratchBytes = 256;
 static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
 
 int size_class(int64_t n) { return n < 4096 ? 0 : (n < 65536 ? 1 : 2); }
+
+bool valid_tile(int t) { return t == 0 || t == 16 || t == 32 || t == 64 || t == TWG_SMOOTH_TILE_SPLIT; }
 
 }  // namespace
 
@@ -991,7 +1066,7 @@ int twg_smooth_autotune(twg_ctx* ctx, const twg_smooth_desc* d, int precision, i
     if (int r = ensure_scratch(ctx, 256)) return r;
     SmoothArgs A;
     if (int r = make_args(ctx, d, (unsigned*)ctx->scratch, 1, A)) return r;
-    const int tiles[3] = {16, 32, 64};
+    const int tiles[4] = {16, 32, 64, TWG_SMOOTH_TILE_SPLIT};
     const unsigned chunks[3] = {1, 4, 16};
     EventPair ev;
     TWG_HIP(ev.create());
@@ -1027,7 +1102,7 @@ int twg_smooth_batch(twg_ctx* ctx, const twg_smooth_desc* d, int precision, int 
 {
     TWG_REQUIRE(ctx != nullptr, "twg_smooth_batch: ctx is NULL");
     TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED, "twg_smooth_batch: bad precision");
-    TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64, "twg_smooth_batch: tile must be 0, 16, 32 or 64");
+    TWG_REQUIRE(valid_tile(tile), "twg_smooth_batch: tile must be 0, 16, 32, 64 or TWG_SMOOTH_TILE_SPLIT");
     TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_batch: chunk out of range");
     TWG_REQUIRE(d != nullptr, "twg_smooth_batch: NULL descriptor");
     if (d->n == 0) return TWG_OK;
@@ -1049,12 +1124,12 @@ int twg_smooth_batch_noalloc(twg_ctx* ctx, const twg_smooth_desc* d, int precisi
     TWG_REQUIRE(ctx != nullptr, "twg_smooth_batch_noalloc: ctx is NULL");
     TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED,
                 "twg_smooth_batch_noalloc: bad precision");
-    TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64,
-                "twg_smooth_batch_noalloc: tile must be 0, 16, 32 or 64");
+    TWG_REQUIRE(valid_tile(tile),
+                "twg_smooth_batch_noalloc: tile must be 0, 16, 32, 64 or TWG_SMOOTH_TILE_SPLIT");
     TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_batch_noalloc: chunk out of range");
     TWG_REQUIRE(d != nullptr, "twg_smooth_batch_noalloc: NULL descriptor");
-    // the scratch holds one 32-bit queue counter: problems popped in chunks
-    // of up to 1024 must stay below 2^31 (make_args checks n against that)
+    // the scratch holds the two 32-bit queue counters: problems popped in
+    // chunks of up to 1024 must stay below 2^31 (make_args checks n)
     TWG_REQUIRE(ctx->noalloc_scratch != nullptr, "twg_smooth_batch_noalloc: no scratch address");
     if (d->n == 0) return TWG_OK;
     if (int r = bind(ctx)) return r;
@@ -1074,7 +1149,7 @@ int twg_smooth_plan_create(twg_ctx* ctx, const twg_smooth_desc* d, int precision
 {
     TWG_REQUIRE(ctx && d && out, "twg_smooth_plan_create: NULL argument");
     TWG_REQUIRE(precision == TWG_FP64 || precision == TWG_MIXED, "twg_smooth_plan_create: bad precision");
-    TWG_REQUIRE(tile == 0 || tile == 16 || tile == 32 || tile == 64, "twg_smooth_plan_create: bad tile");
+    TWG_REQUIRE(valid_tile(tile), "twg_smooth_plan_create: bad tile");
     TWG_REQUIRE(chunk >= 0 && chunk <= 1024, "twg_smooth_plan_create: chunk out of range");
     if (int r = bind(ctx)) return r;
     twg_smooth_plan* P = new twg_smooth_plan();
