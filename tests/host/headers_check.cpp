@@ -1,7 +1,8 @@
 // Host build of the device headers (twg_geom.hpp, twg_bvh.hpp) with the GPU
 // qualifiers removed, checking the logic the kernels run:
 //   1. SampleGen against the oracle's sampleTriangle (bit-exact samples and
-//      counts), and SampleGen::skip against stepping one sample at a time;
+//      counts), SampleGen::skip against stepping one sample at a time, and
+//      the stored-record generator SampleGenC against both;
 //   2. in_envelope -- global walker and LDS walker over the top levels, FP64
 //      and FP32 boxes -- against the brute-force minimum over all facets, on
 //      trees of 1..50k facets (the tree built by the library's own host code);
@@ -16,6 +17,7 @@
 #include "twg_bvh.hpp"
 
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -64,6 +66,26 @@ static void check_sampler(std::mt19937_64& rng)
         samples += n;
         SampleGen cnt;
         cnt.init(a, b, c, d);
+        // the stored-record generator: the same sequence, bit for bit
+        double rec[kFaceRec];
+        for (double& v : rec) v = std::nan("");
+        cnt.store(rec);
+        {
+            SampleGenC<const double*> gc;
+            gc.load(rec, d);
+            V3<double> p;
+            int64_t j = 0;
+            while (gc.next(p) && j <= n + 10) {
+                CHECK(j < (int64_t)seq.size() && !memcmp(&p, &seq[j], sizeof p),
+                      "stored sampler: triangle %d sample %lld differs\n", it, (long long)j);
+                ++j;
+            }
+            CHECK(j == (int64_t)seq.size(), "stored sampler: triangle %d gives %lld samples\n", it,
+                  (long long)j);
+            SampleGenC<const double*> gcnt;
+            gcnt.load(rec, d);
+            CHECK(gcnt.skip(INT64_MAX) == (int64_t)seq.size(), "stored skip: count of triangle %d\n", it);
+        }
         CHECK(cnt.skip(INT64_MAX) == (int64_t)seq.size(), "skip: count of triangle %d\n", it);
         const size_t step = it % 5 == 0 ? 1 : 1 + seq.size() / 7;
         for (size_t s0 = 0; s0 <= seq.size(); s0 += step) {
@@ -78,6 +100,18 @@ static void check_sampler(std::mt19937_64& rng)
                 CHECK(more && !memcmp(&p, &seq[s0], sizeof p), "skip: triangle %d sample %zu\n", it, s0);
             else
                 CHECK(!more, "skip: triangle %d past the end\n", it);
+            SampleGenC<const double*> gc;
+            gc.load(rec, d);
+            const int64_t gotc = gc.skip((int64_t)s0);
+            V3<double> pc;
+            const bool morec = gc.next(pc);
+            CHECK(gotc == (int64_t)s0, "stored skip: triangle %d skipped %lld of %zu\n", it,
+                  (long long)gotc, s0);
+            if (s0 < seq.size())
+                CHECK(morec && !memcmp(&pc, &seq[s0], sizeof pc), "stored skip: triangle %d sample %zu\n",
+                      it, s0);
+            else
+                CHECK(!morec, "stored skip: triangle %d past the end\n", it);
         }
     }
     printf("sampler: 6000 triangles, %ld samples, %ld skip positions\n", samples, positions);

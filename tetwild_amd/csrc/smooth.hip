@@ -69,15 +69,21 @@ struct SmoothArgs {
 // (kQ doubles per lane) while it solves a problem, and while it tests the
 // one-ring faces its list of nearby surface facets -- per entry a 32-byte
 // record, two 4-byte work-list slots (collect_facets) and the facet's FP64
-// triangle (72 bytes).  The two uses never overlap in time.
+// triangle (72 bytes) -- followed by the sampler records (SampleGen::store)
+// of the faces being sampled, TILE / 4 faces at a time.  The reductions and
+// the face test never overlap in time.
 constexpr int kTilePoolBytes = 57344;
 constexpr int kListEntryBytes = 32 + 8 + 72;
+constexpr int kFaceRecBytes = kFaceRec * 8;
 template <int TILE>
 constexpr int tile_bytes() { return kTilePoolBytes / (kBlock / TILE); }
 template <int TILE>
-constexpr int ring_cap() { return tile_bytes<TILE>() / kListEntryBytes; }
+constexpr int face_chunk() { return TILE / 4; }
+template <int TILE>
+constexpr int ring_cap() { return (tile_bytes<TILE>() - face_chunk<TILE>() * kFaceRecBytes) / kListEntryBytes; }
 static_assert(tile_bytes<16>() >= 16 * kQ * 8 && tile_bytes<64>() >= 64 * kQ * 8, "reduction rows");
-static_assert(ring_cap<64>() == 128 && ring_cap<16>() >= 16, "list capacity");
+static_assert(ring_cap<64>() == 94 && ring_cap<16>() >= 16, "list capacity");
+static_assert((ring_cap<16>() * kListEntryBytes) % 16 == 0, "record alignment");
 
 template <int TILE>
 struct TileCtx {
@@ -87,8 +93,9 @@ struct TileCtx {
     double* red;       // the tile's slice: kQ doubles per lane of the tile
     BvhWalk sfw;       // the surface tree, its top levels in LDS
     LdsNode32W* lst;   // the same slice: facet list (ring_cap entries),
-    LdsU32* fr;        // work lists (2 ring_cap entries)
-    LdsF64* tri;       // and the listed facets' triangles (9 doubles each)
+    LdsU32* fr;        // work lists (2 ring_cap entries),
+    LdsF64* tri;       // the listed facets' triangles (9 doubles each)
+    LdsF64* frec;      // and face_chunk sampler records (kFaceRec doubles each)
 };
 
 // acc + v[0] + v[1] + ... + v[cnt - 1] in that order (v[r] = base[r * stride]
@@ -521,9 +528,10 @@ __device__ __forceinline__ void ring_face(const SmoothArgs& A, int64_t k, V3<dou
 // smoothSurface's envelope test of the one-ring surface faces
 // (src/tetwild/VertexSmoother.cpp:413-429): true when some face that is not
 // collinear has a sample with no facet within eps (isFaceOutEnvelop for each
-// face).  The samples of all faces are spread over the tile, TILE faces at a
-// time: lane k counts the samples of face k (SampleGen::skip), a scan over
-// the tile gives the faces' offsets, and lane r walks samples [r K, r K + K)
+// face).  The samples of all faces are spread over the tile, TILE / 4 faces
+// at a time: lane k stores face k's sampler record in LDS and counts its
+// samples (SampleGen::store, skip), a scan over the tile gives the faces'
+// offsets, and lane r walks samples [r K, r K + K) (SampleGenC on the records)
 // of the concatenated sequence (K = ceil(total / TILE)), moving on to the
 // next face with samples when one ends and carrying its facet hint; a ballot
 // after each step stops the tile at the first sample outside.  A sample is
@@ -595,8 +603,9 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
             if (m & tm) hl = __shfl(pos, __ffsll((long long)(m & tm)) - 1, TILE);
         }
     }
-    for (int64_t base = t0; base < t1; base += TILE) {
-        const int nfc = t1 - base < TILE ? (int)(t1 - base) : TILE;
+    constexpr int FC = face_chunk<TILE>();
+    for (int64_t base = t0; base < t1; base += FC) {
+        const int nfc = t1 - base < FC ? (int)(t1 - base) : FC;
         int64_t cnt = 0;
         bool huge = false;
         if (c.tl < nfc) {
@@ -610,10 +619,12 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                 } else {
                     SampleGen g;
                     g.init(a, b, cc, A.d);
+                    g.store(c.frec + kFaceRec * c.tl);
                     cnt = g.skip(INT64_MAX);
                 }
             }
         }
+        wave_sync();  // the records, read below by every lane of the tile
         if (tile_any(c, huge)) return true;
         // faces of this chunk with samples (bit k: face base + k)
         const unsigned long long tm = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
@@ -638,11 +649,9 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                 fstart = ik - ck;
             }
         }
-        SampleGen g;
+        SampleGenC<LdsF64*> g;
         if (f >= 0) {
-            double t9[9];
-            ring_face(A, base + f, pf, t9);
-            g.init(ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.d);
+            g.load(c.frec + kFaceRec * f, A.d);
             g.skip(first - fstart);
         }
         for (int64_t j = 0; j < K; ++j) {
@@ -657,9 +666,7 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                     const unsigned long long later = f >= 63 ? 0ull : has & ~((2ull << f) - 1ull);
                     if (later == 0ull) break;
                     f = __ffsll((long long)later) - 1;
-                    double t9[9];
-                    ring_face(A, base + f, pf, t9);
-                    g.init(ld3(t9), ld3(t9 + 3), ld3(t9 + 6), A.d);
+                    g.load(c.frec + kFaceRec * f, A.d);
                     got = g.next(smp);
                 }
                 if (got)
@@ -668,6 +675,7 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
             }
             if (tile_any(c, out)) return true;
         }
+        wave_sync();  // the records' last reads before the next chunk's
     }
     return false;
 }
@@ -794,6 +802,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         c.lst = (LdsNode32W*)slice;
         c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
         c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
+        c.frec = (LdsF64*)(slice + cap * kListEntryBytes);
     }
     for (;;) {
         unsigned start = 0;
@@ -831,6 +840,7 @@ __device__ __forceinline__ void tile_setup(TileCtx<TILE>& c, unsigned char* pool
     c.lst = (LdsNode32W*)slice;
     c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
     c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
+    c.frec = (LdsF64*)(slice + cap * kListEntryBytes);
 }
 
 template <int TILE, bool MIXED, int WANT>  // WANT 1: surface kinds, 0: the others

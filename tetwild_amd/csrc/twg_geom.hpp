@@ -407,6 +407,12 @@ struct SampleGen {
         }
     }
 
+    // Writes what SampleGenC needs after init(): the (rotated) triangle, the
+    // edge directions, the rows' angles and the edge counts, each computed
+    // with the expression next() and skip() use.
+    template <class P>
+    __device__ __forceinline__ void store(P rec) const;
+
     __device__ __forceinline__ bool next(V3<double>& p)
     {
         for (;;) {
@@ -471,6 +477,277 @@ struct SampleGen {
                 }
                 return false;
             case 8:  // done after v2 (no rows)
+            default:
+                return false;
+            }
+        }
+    }
+};
+
+// Layout of a SampleGen::store record (doubles).
+enum : int {
+    kFrV0 = 0, kFrV1 = 3, kFrV2 = 6,        // the rotated triangle
+    kFrN01 = 9, kFrN02 = 12, kFrN12 = 15,   // normalize(v1 - v0), (v2 - v0), (v2 - v1)
+    kFrN20 = 18,                            // normalize(v0 - v2)
+    kFrSin0 = 21, kFrTan0 = 22, kFrSin1 = 23,
+    kFrN = 24,                              // the longest edge's count after init
+    kFrL12 = 25, kFrL20 = 26,               // |v1 - v2| / d, |v2 - v0| / d
+    kFrM = 27, kFrPhase = 28,
+    kFaceRec = 29
+};
+
+template <class P>
+__device__ __forceinline__ void SampleGen::store(P r) const
+{
+    const V3<double> vs[3] = {v0, v1, v2};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        r[kFrV0 + 3 * j] = vs[j].x;
+        r[kFrV0 + 3 * j + 1] = vs[j].y;
+        r[kFrV0 + 3 * j + 2] = vs[j].z;
+    }
+    r[kFrPhase] = (double)phase;
+    r[kFrN] = N;
+    r[kFrM] = phase == 9 ? 0.0 : (double)M;
+    if (phase == 9) return;
+    const RowGeom g = row_geom();
+    const V3<double> n01 = normalize(v1 - v0), n20 = normalize(v0 - v2);
+    const V3<double> ds[4] = {n01, g.n02, g.n12, n20};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r[kFrN01 + 3 * j] = ds[j].x;
+        r[kFrN01 + 3 * j + 1] = ds[j].y;
+        r[kFrN01 + 3 * j + 2] = ds[j].z;
+    }
+    r[kFrSin0] = g.sin_v0;
+    r[kFrTan0] = g.tan_v0;
+    r[kFrSin1] = g.sin_v1;
+    r[kFrL12] = __builtin_sqrt(len2(v1 - v2)) / d;
+    r[kFrL20] = __builtin_sqrt(len2(v2 - v0)) / d;
+}
+
+// SampleGen over a stored record (P: a pointer to its doubles, e.g. in LDS):
+// the same samples bit for bit, the same skip(), with the directions, the
+// rows' angles and the edge lengths read instead of recomputed.  The state
+// between samples is the record pointer, the row start and the counters.
+template <class P>
+struct SampleGenC {
+    P r;
+    V3<double> rowv;
+    double d, N;
+    int phase, k, m, M, N1;
+
+    __device__ __forceinline__ V3<double> at(int o) const { return mk3(r[o], r[o + 1], r[o + 2]); }
+
+    __device__ __forceinline__ void load(P rec, double d_)
+    {
+        r = rec;
+        d = d_;
+        phase = (int)r[kFrPhase];
+        N = r[kFrN];
+        M = (int)r[kFrM];
+        k = 0;
+    }
+
+    __device__ __forceinline__ void row_header()
+    {
+        if (m > M) {
+            phase = 4;
+            return;
+        }
+        const double s32 = SampleGen::sqrt3_2();
+        const double tan_v0 = r[kFrTan0];
+        int n = (int)(s32 / tan_v0 * m + 0.5);
+        int n1 = (int)(s32 / tan_v0 * m);
+        if (m % 2 == 0 && n == n1) n += 1;
+        const V3<double> v0 = at(kFrV0), v1 = at(kFrV1);
+        V3<double> v0_m = v0 + mulv(at(kFrN02), m * s32 * d / r[kFrSin0]);
+        V3<double> v1_m = v1 + mulv(at(kFrN12), m * s32 * d / r[kFrSin1]);
+        if (dist(v0_m, v1_m) <= d) {
+            phase = 4;
+            return;
+        }
+        double delta_d = ((n + (m % 2) / 2.0) - m * s32 / tan_v0) * d;
+        rowv = v0_m + mulv(at(kFrN01), delta_d);
+        N1 = (int)(dist(rowv, v1_m) / d);
+        k = 0;
+        phase = 3;
+    }
+
+    __device__ __forceinline__ void start_edge12()
+    {
+        N = r[kFrL12];
+        if (N > 1) {
+            if (N == (int)N) N -= 1;
+            k = 1;
+            phase = 5;
+        } else {
+            phase = 6;
+        }
+    }
+
+    __device__ __forceinline__ bool start_edge20()
+    {
+        N = r[kFrL20];
+        if (!(N > 1)) return false;
+        if (N == (int)N) N -= 1;
+        k = 1;
+        phase = 7;
+        return true;
+    }
+
+    // SampleGen::skip
+    __device__ __forceinline__ int64_t skip(int64_t s)
+    {
+        int64_t done = 0;
+        for (;;) {
+            if (done >= s) return done;
+            const int64_t want = s - done;
+            switch (phase) {
+            case 9:
+                if (k >= 3) return done;
+                {
+                    const int64_t take = want < 3 - k ? want : 3 - k;
+                    k += (int)take;
+                    done += take;
+                }
+                break;
+            case 0:
+                if (k <= N) {
+                    const int64_t avail = (int64_t)N - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    phase = 1;
+                    done += 1;
+                }
+                break;
+            case 1:
+                if (M < 1) {
+                    phase = 8;
+                    done += 1;
+                } else {
+                    m = 1;
+                    phase = 2;
+                }
+                break;
+            case 2:
+                row_header();
+                break;
+            case 3:
+                if (k <= N1) {
+                    const int64_t avail = (int64_t)N1 - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    m++;
+                    phase = 2;
+                }
+                break;
+            case 4:
+                start_edge12();
+                done += 1;
+                break;
+            case 5:
+                if (k <= N) {
+                    const int64_t avail = (int64_t)N - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    phase = 6;
+                }
+                break;
+            case 6:
+                if (!start_edge20()) {
+                    phase = 8;
+                    return done;
+                }
+                break;
+            case 7:
+                if (k <= N) {
+                    const int64_t avail = (int64_t)N - k + 1;
+                    const int64_t take = want < avail ? want : avail;
+                    k += (int)take;
+                    done += take;
+                } else {
+                    phase = 8;
+                    return done;
+                }
+                break;
+            default:
+                return done;
+            }
+        }
+    }
+
+    // SampleGen::next
+    __device__ __forceinline__ bool next(V3<double>& p)
+    {
+        for (;;) {
+            switch (phase) {
+            case 9:
+                if (k < 3) {
+                    p = at(kFrV0 + 3 * k);
+                    k++;
+                    return true;
+                }
+                return false;
+            case 0:
+                if (k <= N) {
+                    p = at(kFrV0) + mulv(mulv(at(kFrN01), d), (double)k);
+                    k++;
+                    return true;
+                }
+                phase = 1;
+                p = at(kFrV1);
+                return true;
+            case 1:
+                if (M < 1) {
+                    phase = 8;
+                    p = at(kFrV2);
+                    return true;
+                }
+                m = 1;
+                phase = 2;
+                break;
+            case 2:
+                row_header();
+                break;
+            case 3:
+                if (k <= N1) {
+                    p = rowv + mulv(mulv(at(kFrN01), (double)k), d);
+                    k++;
+                    return true;
+                }
+                m++;
+                phase = 2;
+                break;
+            case 4:
+                p = at(kFrV2);
+                start_edge12();
+                return true;
+            case 5:
+                if (k <= N) {
+                    p = at(kFrV1) + mulv(mulv(at(kFrN12), d), (double)k);
+                    k++;
+                    return true;
+                }
+                phase = 6;
+                break;
+            case 6:
+                if (!start_edge20()) return false;
+                break;
+            case 7:
+                if (k <= N) {
+                    p = at(kFrV2) + mulv(mulv(at(kFrN20), d), (double)k);
+                    k++;
+                    return true;
+                }
+                return false;
+            case 8:
             default:
                 return false;
             }
