@@ -756,15 +756,22 @@ def bench_config1(ctx, args, result):
            "reference": "not measurable: the reference pipeline needs CGAL, geogram, Eigen "
                         "and libigl, absent here"}
     nv, nt = len(P), len(tets)
-    for mode in ("seq", "batch"):
+    # seq: the reference's vertex order, one launch per dependency level;
+    # serial: the same results one launch per vertex; batch: colour classes
+    raws = {}
+    for mode in ("seq", "serial", "batch"):
         info, out, el = D.run(mesh, mode)
         raw = np.fromfile(out, dtype=np.float64)
+        raws[mode] = raw
         qn = raw[3 * nv:3 * nv + nt]
         Pn = raw[:3 * nv].reshape(nv, 3)
         sec[f"smooth_{mode}"] = {"counter": info["counter"], "suc_counter": info["suc_counter"],
                                  "moved": int((Pn != P).any(1).sum()),
                                  "energy_after": {"max": float(qn.max()), "avg": float(qn.mean())},
                                  "seconds": el}
+        if mode == "seq":
+            sec["smooth_seq"]["levels"] = [info["levels_single"], info["levels_surface"]]
+    sec["smooth_seq_vs_serial_bit_mismatch"] = bits_mismatch(raws["seq"], raws["serial"])
     info, out, el = D.run(mesh, "collapse")
     codes = np.fromfile(out, dtype=np.int32, count=info["candidates"])
     sec["collapse"] = {"candidates": info["candidates"], "seconds": el,

@@ -143,7 +143,7 @@ def replay(P, tets, fs, flags, q, S, eps, d, batched):
     return P, q, counter, suc
 
 
-@pytest.mark.parametrize("mode", ["seq", "batch"])
+@pytest.mark.parametrize("mode", ["seq", "serial", "batch"])
 def test_shim_smooth_matches_replay(ctx, tmp_path, mode):
     assert os.path.exists(DRIVER), "build the shim first (make -C tetwild_amd/shim)"
     P, tets, fs, flags, sf = lattice_mesh()
@@ -353,7 +353,7 @@ def replay_exact(P, Pex, rounded, tets, fs, flags, q, S, eps, d, batched):
     return M.posf(), M.r.copy(), q, counter, suc
 
 
-@pytest.mark.parametrize("mode", ["seq", "batch"])
+@pytest.mark.parametrize("mode", ["seq", "serial", "batch"])
 def test_shim_unrounded_matches_replay(ctx, tmp_path, mode):
     """VertexSmoother with unrounded vertices: 20% of the vertices carry an
     exact position pos != posf (doubles, offset by up to 0.2 h), handed to the

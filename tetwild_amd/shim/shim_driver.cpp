@@ -3,7 +3,10 @@
 // reference's class surface can be exercised end to end.
 //
 //   shim_driver <mesh.bin> <out.bin> <mode>
-//   mode: seq       smooth() in vertex order (the reference's Gauss-Seidel order)
+//   mode: seq       smooth(): the reference's vertex-order (Gauss-Seidel) results,
+//                   one launch per dependency level
+//         serial    smoothSingleSerial() + smoothSurfaceSerial(): the same
+//                   results, one launch per vertex
 //         batch     smoothSingleBatched() + smoothSurfaceBatched()
 //         collapse  EdgeCollapser::checkCollapses over every directed edge
 //                   (v1, v2 ascending); collapse_soft: is_soft, soft_energy 40
@@ -23,7 +26,7 @@
 // vertices in vertex order, their exact positions double pos[][3].  The
 // exact positions here are doubles, so the driver's ExactPositions orients
 // them with the library's exact predicate (twg_tet_flip: positive or not).
-// When the mesh has unrounded vertices, seq / batch also write
+// When the mesh has unrounded vertices, seq / serial / batch also write
 // <out.bin>.rounded: uint8 is_rounded[nv] after the run.
 // out.bin: double posf[nv][3]; double slim_energy[nt]; int64 counter,
 // suc_counter; plus, for the first up-to-8 interior vertices, NewtonsUpdate
@@ -93,7 +96,7 @@ private:
 int main(int argc, char** argv)
 {
     if (argc != 4) {
-        fprintf(stderr, "usage: shim_driver mesh.bin out.bin seq|batch|collapse|collapse_soft\n");
+        fprintf(stderr, "usage: shim_driver mesh.bin out.bin seq|serial|batch|collapse|collapse_soft|remove|energy\n");
         return 2;
     }
     FILE* f = fopen(argv[1], "rb");
@@ -259,8 +262,18 @@ int main(int argc, char** argv)
         VertexSmoother vs(tv, tets, fs, v_rm, t_rm, tq, sf_tree, b_tree, st, dev);
         vs.exact = n_unrounded ? &exact : nullptr;
         int64_t counters[2];
+        int levels[2] = {0, 0};
         if (mode == "seq") {
-            vs.smooth();
+            // smooth(), recording the levels of its two loops
+            vs.smoothSingle();
+            levels[0] = vs.last_levels;
+            if (st.eps >= 0) {
+                vs.smoothSurface();
+                levels[1] = vs.last_levels;
+            }
+        } else if (mode == "serial") {
+            vs.smoothSingleSerial();
+            if (st.eps >= 0) vs.smoothSurfaceSerial();
         } else {
             vs.smoothSingleBatched();
             if (st.eps >= 0) vs.smoothSurfaceBatched();
@@ -304,8 +317,9 @@ int main(int argc, char** argv)
             fclose(r);
         }
         printf("{\"counter\": %lld, \"suc_counter\": %lld, \"max_energy\": %.17g, "
-               "\"exact_calls\": %ld}\n",
-               (long long)counters[0], (long long)counters[1], vs.getMaxEnergy(), exact.calls);
+               "\"exact_calls\": %ld, \"levels_single\": %d, \"levels_surface\": %d}\n",
+               (long long)counters[0], (long long)counters[1], vs.getMaxEnergy(), exact.calls,
+               levels[0], levels[1]);
     } catch (const std::exception& e) {
         fprintf(stderr, "shim_driver: %s\n", e.what());
         return 1;

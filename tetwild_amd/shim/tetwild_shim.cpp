@@ -873,8 +873,103 @@ int VertexSmoother::smoothExact(int v, int kind, const std::vector<int>& ring)
     return TWG_SMOOTH_MOVED;
 }
 
-// src/tetwild/VertexSmoother.cpp:114-242 in vertex order (Gauss-Seidel)
+// the calTetQualities pass that ends smoothSingle
+void VertexSmoother::finishSingle()
+{
+    std::vector<std::array<int, 4>> live;
+    std::vector<int> ids;
+    for (size_t i = 0; i < tets.size(); ++i)
+        if (!t_is_removed[i]) {
+            live.push_back(tets[i]);
+            ids.push_back((int)i);
+        }
+    std::vector<TetQuality> qs;
+    calTetQualities(live, qs);
+    for (size_t k = 0; k < ids.size(); ++k) tet_qualities[ids[k]] = qs[k];
+}
+
+std::vector<int> VertexSmoother::surfaceCandidates()
+{
+    std::vector<int> verts;
+    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
+        if (v_is_removed[v]) continue;
+        TetVertex& tv = tet_vertices[v];
+        if (!tv.is_on_surface) continue;
+        if (tv.is_locked) continue;
+        if (isIsolated(v)) {
+            tv.is_on_surface = false;
+            tv.is_on_boundary = false;
+            continue;
+        }
+        if (!isBoundaryPoint(v)) tv.is_on_boundary = false;
+        verts.push_back(v);
+    }
+    return verts;
+}
+
+std::vector<std::vector<int>> VertexSmoother::orderLevels(const std::vector<int>& verts) const
+{
+    std::vector<int> level(tet_vertices.size(), -1);
+    std::vector<std::vector<int>> levels;
+    for (int v : verts) {
+        int l = 0;
+        for (int t : tet_vertices[v].conn_tets)
+            for (int j = 0; j < 4; ++j) {
+                const int u = tets[t][j];
+                if (u != v && level[u] >= l) l = level[u] + 1;
+            }
+        level[v] = l;
+        if (l == (int)levels.size()) levels.emplace_back();
+        levels[l].push_back(v);
+    }
+    return levels;
+}
+
+// src/tetwild/VertexSmoother.cpp:114-242 with the vertex order's results.
+// Why the levels give them: the body for v reads only its one-ring (the
+// positions, rounding and qualities of vertices sharing a tet with v) and
+// writes only v and its ring tets.  When v runs, the vertex-order loop has
+// updated exactly the earlier ring vertices; in level order those have lower
+// levels (done) and the later ones higher levels (not started), and no two
+// vertices of a level share a tet.
 void VertexSmoother::smoothSingle()
+{
+    counter = 0;
+    suc_counter = 0;
+    std::vector<int> verts;
+    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
+        if (v_is_removed[v]) continue;
+        const TetVertex& tv = tet_vertices[v];
+        if (tv.is_on_bbox) continue;
+        if (state.eps != State::EPSILON_INFINITE && tv.is_on_surface) continue;
+        if (tv.is_locked) continue;
+        verts.push_back(v);
+    }
+    const auto levels = orderLevels(verts);
+    last_levels = (int)levels.size();
+    for (const auto& lv : levels) {
+        runClass(lv, false, suc_counter);
+        counter += (int)lv.size();
+    }
+    finishSingle();
+}
+
+// src/tetwild/VertexSmoother.cpp:244-463 with the vertex order's results:
+// the flag updates touch no position and run first, in vertex order, as the
+// loop interleaves them; then the levels, as in smoothSingle
+void VertexSmoother::smoothSurface()
+{
+    const std::vector<int> verts = surfaceCandidates();
+    const auto levels = orderLevels(verts);
+    last_levels = (int)levels.size();
+    for (const auto& lv : levels) {
+        runClass(lv, true, suc_counter);
+        counter += (int)lv.size();
+    }
+}
+
+// src/tetwild/VertexSmoother.cpp:114-242 in vertex order, one call per vertex
+void VertexSmoother::smoothSingleSerial()
 {
     counter = 0;
     suc_counter = 0;
@@ -888,20 +983,11 @@ void VertexSmoother::smoothSingle()
         int s = smoothOne(v, 0);
         if (s == TWG_SMOOTH_MOVED || s == TWG_SMOOTH_FLIP_END) suc_counter++;
     }
-    std::vector<std::array<int, 4>> live;
-    std::vector<int> ids;
-    for (size_t i = 0; i < tets.size(); ++i)
-        if (!t_is_removed[i]) {
-            live.push_back(tets[i]);
-            ids.push_back((int)i);
-        }
-    std::vector<TetQuality> qs;
-    calTetQualities(live, qs);
-    for (size_t k = 0; k < ids.size(); ++k) tet_qualities[ids[k]] = qs[k];
+    finishSingle();
 }
 
-// src/tetwild/VertexSmoother.cpp:244-463 in vertex order
-void VertexSmoother::smoothSurface()
+// src/tetwild/VertexSmoother.cpp:244-463 in vertex order, one call per vertex
+void VertexSmoother::smoothSurfaceSerial()
 {
     for (int v = 0; v < (int)tet_vertices.size(); ++v) {
         if (v_is_removed[v]) continue;
@@ -1001,33 +1087,12 @@ void VertexSmoother::smoothSingleBatched()
         runClass(cls, false, suc_counter);
         counter += (int)cls.size();
     }
-    std::vector<std::array<int, 4>> live;
-    std::vector<int> ids;
-    for (size_t i = 0; i < tets.size(); ++i)
-        if (!t_is_removed[i]) {
-            live.push_back(tets[i]);
-            ids.push_back((int)i);
-        }
-    std::vector<TetQuality> qs;
-    calTetQualities(live, qs);
-    for (size_t k = 0; k < ids.size(); ++k) tet_qualities[ids[k]] = qs[k];
+    finishSingle();
 }
 
 void VertexSmoother::smoothSurfaceBatched()
 {
-    std::vector<int> verts;
-    for (int v = 0; v < (int)tet_vertices.size(); ++v) {
-        if (v_is_removed[v]) continue;
-        TetVertex& tv = tet_vertices[v];
-        if (!tv.is_on_surface || tv.is_locked) continue;
-        if (isIsolated(v)) {
-            tv.is_on_surface = false;
-            tv.is_on_boundary = false;
-            continue;
-        }
-        if (!isBoundaryPoint(v)) tv.is_on_boundary = false;
-        verts.push_back(v);
-    }
+    const std::vector<int> verts = surfaceCandidates();
     for (const auto& cls : colourClasses(verts)) {
         runClass(cls, true, suc_counter);
         counter += (int)cls.size();

@@ -222,10 +222,20 @@ class VertexSmoother : public LocalOperations {
 public:
     using LocalOperations::LocalOperations;
 
-    // src/tetwild/VertexSmoother.h:23-30
+    // src/tetwild/VertexSmoother.h:23-30.  smoothSingle / smoothSurface give
+    // the reference's vertex-order (Gauss-Seidel) results, computed level by
+    // level: a vertex's level is one more than the highest level of the
+    // earlier vertices it shares a tet with, so each level is a set of
+    // independent vertices that sees exactly the neighbours' updates the
+    // vertex-order loop would show it; every level is one batch.
     void smooth();
     void smoothSingle();
     void smoothSurface();
+    // the same loops one vertex per call (the same results, one launch each)
+    void smoothSingleSerial();
+    void smoothSurfaceSerial();
+    // levels of the last smoothSingle / smoothSurface call
+    int last_levels = 0;
     bool NewtonsMethod(const std::vector<int>& t_ids,
                        const std::vector<std::array<int, 4>>& new_tets, int v_id, Point_3f& p);
     bool NewtonsUpdate(const std::vector<int>& t_ids, int v_id, double& energy,
@@ -266,6 +276,13 @@ private:
     // writes moved positions / qualities back into the mesh
     void runBatch(const std::vector<Problem>& probs, std::vector<uint8_t>& status);
     std::vector<std::vector<int>> colourClasses(const std::vector<int>& verts) const;
+    // verts (in vertex order) grouped by dependency level
+    std::vector<std::vector<int>> orderLevels(const std::vector<int>& verts) const;
+    // the surface loop's flag updates (isolated vertices leave the surface,
+    // non-boundary points the boundary), in vertex order; returns the
+    // vertices smoothSurface smooths
+    std::vector<int> surfaceCandidates();
+    void finishSingle();
     std::vector<double> last_pout_;  // p_out of the last batch
 };
 
