@@ -7,6 +7,8 @@ resident in HBM, one launch of amips_eval_kernel<2> per step.
 
 Secondary measurements in the same JSON line ("secondary"):
   amips_mixed    the same lattice through the mixed FP32/FP64 kernel
+  amips_packed_hessian  the same with the Hessian returned as its upper
+                 triangle (TWG_AMIPS_EGH_SYM), FP64 and mixed
   envelope       isFaceOutEnvelop on 20M query triangles against a 1,003,520-facet
                  bumpy sphere (eps = sampling_dist = 1e-3), FP64 and mixed
   mixed_slivers  20M tets with aspect ratio 10^U[0,4] (mixed vs FP64 energies:
@@ -263,6 +265,38 @@ def bench_amips(ctx, args, result):
         "tol": tol9, "tet_evals_per_s": n / wall_9, "fallback_rate": fb.float().mean().item(),
         "max_rel_err_E_vs_fp64": ((Em - E).abs() / E.abs()).max().item(),
         "speedup_vs_fp64": wall / wall_9}
+
+    # the packed Hessian (TWG_AMIPS_EGH_SYM: its upper triangle, 96 instead of
+    # 120 output bytes per tet), FP64 and mixed, against the full layout
+    from tetwild_amd import _capi as C
+    H6 = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    E6 = torch.empty(n, dtype=torch.float64, device=dev)
+    G6 = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    launch6 = ctx.amips_launcher(tt, tx, ty, tz, C.TWG_AMIPS_EGH_SYM, (E6, G6, H6))
+    wall6, kern6 = events_time(launch6, args.steps, args.warmup)
+    seq6 = launch_sequence(launch6, 300)
+    upper = torch.tensor([0, 1, 2, 4, 5, 8], device=dev)
+    mism = int((H6.view(torch.int64) != H[:, upper].view(torch.int64)).sum().item() +
+               (E6.view(torch.int64) != E.view(torch.int64)).sum().item() +
+               (G6.view(torch.int64) != G.view(torch.int64)).sum().item())
+    alg6 = n * (BYTES_PER_TET - 24) + nv * BYTES_PER_VERTEX
+    Hm6 = torch.empty((n, 6), dtype=torch.float32, device=dev)
+    wallm6, kernm6 = events_time(
+        lambda: ctx.amips_eval_mixed(tt, tx, ty, tz, mode=C.TWG_AMIPS_EGH_SYM, tol=tol,
+                                     out=(Em, Gm, Hm6, fb)), args.steps, args.warmup)
+    algm6 = n * (16 + 8 + 12 + 24 + 1) + nv * 24
+    result["secondary"]["amips_packed_hessian"] = {
+        "layout": "H as its upper triangle (xx, xy, xz, yy, yz, zz), TWG_AMIPS_EGH_SYM",
+        "tet_evals_per_s_fp64": n / wall6, "avg_launch_ms_fp64": kern6 * 1e3,
+        "sequence_median_launch_ms_fp64": float(np.median(seq6)) * 1e3,
+        "alg_bytes_per_launch_fp64": alg6,
+        "sequence_median_gbs_fp64": alg6 / float(np.median(seq6)) / 1e9,
+        "bit_mismatch_vs_full_layout": mism,
+        "tet_evals_per_s_mixed": n / wallm6, "avg_launch_ms_mixed": kernm6 * 1e3,
+        "alg_bytes_per_launch_mixed": algm6,
+        "speedup_fp64_vs_full_layout": wall / wall6,
+    }
+    del H6, E6, G6, Hm6
 
     if not args.no_cpu:
         log("[cpu] timing the CPU baseline sample")

@@ -44,6 +44,10 @@ extern "C" {
 #define TWG_AMIPS_E 0   /* energy */
 #define TWG_AMIPS_EG 1  /* energy + gradient w.r.t. vertex 0 */
 #define TWG_AMIPS_EGH 2 /* energy + gradient + Hessian w.r.t. vertex 0 */
+/* as TWG_AMIPS_EGH with H[n][6]: the Hessian's upper triangle row by row
+ * (xx, xy, xz, yy, yz, zz); TWG_AMIPS_EGH's H is symmetric bit for bit, so
+ * these are its values, in 96 instead of 120 output bytes per tet */
+#define TWG_AMIPS_EGH_SYM 3
 
 #define TWG_MAX_ENERGY 1e50 /* State::MAX_ENERGY, src/tetwild/State.h:29 */
 
@@ -78,8 +82,8 @@ int twg_memcpy_d2h(twg_ctx* ctx, void* dst, const void* src, size_t bytes);
  * Replaces LocalOperations::comformalAMIPSEnergy_new / Jacobian_new /
  * Hessian_new (src/tetwild/LocalOperations.h:109-111,
  * src/tetwild/LocalOperations.cpp:28-291) and energy_ispc
- * (src/ispc/energy.ispc:7-65), batched.  mode: TWG_AMIPS_E / _EG / _EGH; G and
- * H may be NULL when the mode does not write them.  FP64, bit-exact with the
+ * (src/ispc/energy.ispc:7-65), batched.  mode: TWG_AMIPS_E / _EG / _EGH /
+ * _EGH_SYM; G and H may be NULL when the mode does not write them.  FP64, bit-exact with the
  * CPU restatement; within ~1e-12 relative of the reference's code-generated
  * expressions on well-shaped tets. */
 int twg_amips_eval(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x,
@@ -91,7 +95,8 @@ int twg_amips_eval(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x
  * value, certified to relative error <= tol by a forward error bound, or, where
  * the bound exceeds tol (ill-conditioned tets), the FP64 value.  fallback[i]
  * (may be NULL) is 1 where the FP64 value was used.  E is finite-or-inf exactly
- * where the FP64 energy is. */
+ * where the FP64 energy is.  mode TWG_AMIPS_EGH_SYM: H[n][6], the upper
+ * triangle (the FP32 Hessian is symmetric too). */
 int twg_amips_eval_mixed(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x,
                          const double* y, const double* z, int64_t nv, int mode, double tol,
                          double* E, float* G, float* H, uint8_t* fallback);

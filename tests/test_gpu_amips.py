@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
+from tetwild_amd import _capi as C
 from tetwild_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -126,6 +127,45 @@ def test_orientation_near_degenerate(ctx, dev):
     fl = ctx.tet_flip(tt, tx, ty, tz).cpu().numpy()
     assert np.array_equal(fl, O.tet_flip(tets, x, y, z))
     assert 0 < fl.sum() < len(fl)
+
+
+UPPER = [0, 1, 2, 4, 5, 8]  # (xx, xy, xz, yy, yz, zz) of a row-major 3x3
+
+
+@pytest.mark.parametrize("case", ["lattice", "aspect", "golden", "invalid"])
+def test_packed_hessian(ctx, dev, case):
+    """TWG_AMIPS_EGH_SYM: E and G as TWG_AMIPS_EGH, H its upper triangle bit
+    for bit (TWG_AMIPS_EGH's H is symmetric bit for bit), FP64 and mixed;
+    rows of tets with bad ids are NaN as in the full layout"""
+    print(f"case={case}", flush=True)
+    if case == "lattice":
+        x, y, z, tets, _ = W.kuhn_lattice(24, seed=1)
+    elif case == "aspect":
+        x, y, z, tets = W.aspect_tets(40000, seed=2)
+    elif case == "golden":
+        x, y, z, tets, _ = golden_mesh()
+    else:
+        x, y, z, tets, _ = W.kuhn_lattice(3, seed=1)
+        tets = tets.copy()
+        tets[4, 1] = len(x)
+        tets[7, 3] = -5
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    E2, G2, H2 = (t.cpu().numpy() for t in ctx.amips_eval(tt, tx, ty, tz, mode=C.TWG_AMIPS_EGH))
+    E3, G3, H3 = (t.cpu().numpy()
+                  for t in ctx.amips_eval(tt, tx, ty, tz, mode=C.TWG_AMIPS_EGH_SYM))
+    assert H3.shape == (len(tets), 6)
+    assert np.array_equal(bits(E3), bits(E2)) and np.array_equal(bits(G3), bits(G2))
+    assert np.array_equal(bits(H3), bits(H2[:, UPPER]))
+    T = H2.reshape(-1, 3, 3)
+    assert np.array_equal(bits(T), bits(T.transpose(0, 2, 1).copy()))
+    if case == "invalid":
+        assert np.isnan(H3[[4, 7]]).all() and np.isfinite(np.delete(H3, [4, 7], 0)).all()
+    Em2, Gm2, Hm2, fb2 = (t.cpu().numpy() for t in ctx.amips_eval_mixed(tt, tx, ty, tz, mode=2))
+    Em3, Gm3, Hm3, fb3 = (t.cpu().numpy()
+                          for t in ctx.amips_eval_mixed(tt, tx, ty, tz, mode=C.TWG_AMIPS_EGH_SYM))
+    assert np.array_equal(bits(Em3), bits(Em2)) and np.array_equal(fb3, fb2)
+    assert np.array_equal(Gm3.view(np.int32), Gm2.view(np.int32))
+    assert np.array_equal(Hm3.view(np.int32), np.ascontiguousarray(Hm2[:, UPPER]).view(np.int32))
 
 
 def test_mixed_certified(ctx, dev):

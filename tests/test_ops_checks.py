@@ -56,6 +56,14 @@ def test_amips_out_checked(ctx):
     # mode E only: G and H may be None
     with pytest.raises(Reached):
         ctx.amips_eval(t, x, y, z, C.TWG_AMIPS_E, out=(f64(5), None, None))
+    # the packed Hessian mode takes H (n, 6), not (n, 9)
+    sym = C.TWG_AMIPS_EGH_SYM
+    with pytest.raises(ValueError):
+        ctx.amips_eval(t, x, y, z, sym, out=(f64(5), f64(5, 3), f64(5, 9)))
+    with pytest.raises(Reached):
+        ctx.amips_eval(t, x, y, z, sym, out=(f64(5), f64(5, 3), f64(5, 6)))
+    with pytest.raises(Reached):
+        ctx.amips_eval(t, x, y, z, sym)
 
 
 def test_amips_mixed_out_checked(ctx):

@@ -18,6 +18,7 @@ TWG_SMOOTH_TILE_SPLIT = 1
 TWG_AMIPS_E = 0
 TWG_AMIPS_EG = 1
 TWG_AMIPS_EGH = 2
+TWG_AMIPS_EGH_SYM = 3  # H as its upper triangle, (n, 6)
 MAX_ENERGY = 1e50
 
 P = ctypes.c_void_p

@@ -4,7 +4,8 @@
 // Data path: the tet's four vertex ids come in as one 16-byte load; the twelve
 // coordinates are gathered from the SoA x/y/z arrays (each vertex is shared by
 // ~24 tets of a lattice, so the gathers mostly hit L2).  Results are per-tet
-// rows (E[n], G[n][3], H[n][9]) written through an LDS transpose, so each
+// rows (E[n], G[n][3], H[n][9] or, packed, H[n][6]) written through an LDS
+// transpose, so each
 // store instruction covers 64 consecutive elements and the kernel emits three
 // sequential write streams (planar outputs -- thirteen streams -- measured
 // ~35% slower on MI355X).  Vertex ids outside [0, nv) make the tet's outputs NaN (flip
@@ -92,7 +93,12 @@ __global__ __launch_bounds__(kBlock) void amips_eval_kernel(
         __builtin_nontemporal_store(e, &E[i]);
     }
     if (MODE >= 1) block_store_rows<3>(G, base, n, g, lds);
-    if (MODE >= 2) block_store_rows<9>(H, base, n, h, lds);
+    if (MODE == 2) block_store_rows<9>(H, base, n, h, lds);
+    if (MODE == 3) {
+        // the upper triangle (h is symmetric: amips_hess writes both halves)
+        const double u[6] = {h[0], h[1], h[2], h[4], h[5], h[8]};
+        block_store_rows<6>(H, base, n, u, lds);
+    }
 }
 
 // FP32 gradient / Hessian from FP32 edge vectors (mixed path)
@@ -183,7 +189,12 @@ __global__ __launch_bounds__(kBlock) void amips_mixed_kernel(
         if (fallback) fallback[i] = fb ? 1 : 0;
     }
     if (MODE >= 1) block_store_rows<3>(G, base, n, g, lds);
-    if (MODE >= 2) block_store_rows<9>(H, base, n, h, lds);
+    if (MODE == 2) block_store_rows<9>(H, base, n, h, lds);
+    if (MODE == 3) {
+        // the upper triangle (both Hessian forms above write symmetric h)
+        const float u[6] = {h[0], h[1], h[2], h[4], h[5], h[8]};
+        block_store_rows<6>(H, base, n, u, lds);
+    }
 }
 
 // calTetQuality_AMIPS semantics (src/tetwild/LocalOperations.cpp:862-884)
@@ -382,7 +393,8 @@ int twg_amips_eval(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x
                    double* H)
 {
     if (int r = check_common(ctx, tets, n, x, y, z, nv, "twg_amips_eval")) return r;
-    TWG_REQUIRE(mode >= TWG_AMIPS_E && mode <= TWG_AMIPS_EGH, "twg_amips_eval: bad mode %d", mode);
+    TWG_REQUIRE(mode >= TWG_AMIPS_E && mode <= TWG_AMIPS_EGH_SYM, "twg_amips_eval: bad mode %d",
+                mode);
     TWG_REQUIRE(n == 0 || (E && (mode < 1 || G) && (mode < 2 || H)),
                 "twg_amips_eval: NULL output");
     if (n == 0) return TWG_OK;
@@ -392,8 +404,10 @@ int twg_amips_eval(twg_ctx* ctx, const int32_t* tets, int64_t n, const double* x
         hipLaunchKernelGGL(amips_eval_kernel<0>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
     else if (mode == 1)
         hipLaunchKernelGGL(amips_eval_kernel<1>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
-    else
+    else if (mode == 2)
         hipLaunchKernelGGL(amips_eval_kernel<2>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
+    else
+        hipLaunchKernelGGL(amips_eval_kernel<3>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, E, G, H);
     TWG_LAUNCHED();
     return TWG_OK;
 }
@@ -403,7 +417,7 @@ int twg_amips_eval_mixed(twg_ctx* ctx, const int32_t* tets, int64_t n, const dou
                          double* E, float* G, float* H, uint8_t* fallback)
 {
     if (int r = check_common(ctx, tets, n, x, y, z, nv, "twg_amips_eval_mixed")) return r;
-    TWG_REQUIRE(mode >= TWG_AMIPS_E && mode <= TWG_AMIPS_EGH, "twg_amips_eval_mixed: bad mode %d",
+    TWG_REQUIRE(mode >= TWG_AMIPS_E && mode <= TWG_AMIPS_EGH_SYM, "twg_amips_eval_mixed: bad mode %d",
                 mode);
     TWG_REQUIRE(n == 0 || (E && (mode < 1 || G) && (mode < 2 || H)),
                 "twg_amips_eval_mixed: NULL output");
@@ -418,8 +432,11 @@ int twg_amips_eval_mixed(twg_ctx* ctx, const int32_t* tets, int64_t n, const dou
     else if (mode == 1)
         hipLaunchKernelGGL(amips_mixed_kernel<1>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, t, E, G,
                            H, fallback);
-    else
+    else if (mode == 2)
         hipLaunchKernelGGL(amips_mixed_kernel<2>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, t, E, G,
+                           H, fallback);
+    else
+        hipLaunchKernelGGL(amips_mixed_kernel<3>, g, b, 0, ctx->cur, tets, n, x, y, z, nv, t, E, G,
                            H, fallback);
     TWG_LAUNCHED();
     return TWG_OK;

@@ -76,7 +76,7 @@ class Context:
         if mode >= C.TWG_AMIPS_EGH:
             if H is None:
                 raise ValueError("out H is required for this mode")
-            self._dev(H, gdtype, "out H", (n, 9))
+            self._dev(H, gdtype, "out H", (n, 6 if mode == C.TWG_AMIPS_EGH_SYM else 9))
         if mixed and out[3] is not None:
             self._dev(out[3], torch.uint8, "out fallback", (n,))
         return out
@@ -98,14 +98,16 @@ class Context:
 
     # -- (a) -----------------------------------------------------------
     def amips_eval(self, tets, x, y, z, mode=C.TWG_AMIPS_EGH, out=None):
-        """E (n), G (n, 3), H (n, 9) float64, gradient/Hessian w.r.t. vertex 0."""
+        """E (n), G (n, 3), H (n, 9) float64, gradient/Hessian w.r.t. vertex 0
+        (mode TWG_AMIPS_EGH_SYM: H (n, 6), the upper triangle row by row)."""
         n = self._tets(tets)
         nv = self._verts(x, y, z)
         dev = self.device
         if out is None:
+            hw = 6 if mode == C.TWG_AMIPS_EGH_SYM else 9
             E = torch.empty(n, dtype=torch.float64, device=dev)
             G = torch.empty((n, 3), dtype=torch.float64, device=dev) if mode >= 1 else None
-            H = torch.empty((n, 9), dtype=torch.float64, device=dev) if mode >= 2 else None
+            H = torch.empty((n, hw), dtype=torch.float64, device=dev) if mode >= 2 else None
         else:
             E, G, H = self._amips_out(out, n, mode, torch.float64, False)
         self._bind()
@@ -131,14 +133,16 @@ class Context:
         return launch
 
     def amips_eval_mixed(self, tets, x, y, z, mode=C.TWG_AMIPS_EGH, tol=2.0**-16, out=None):
-        """E (n) float64, G (n, 3) / H (n, 9) float32, fallback (n) uint8."""
+        """E (n) float64, G (n, 3) / H (n, 9) float32 (mode TWG_AMIPS_EGH_SYM:
+        H (n, 6)), fallback (n) uint8."""
         n = self._tets(tets)
         nv = self._verts(x, y, z)
         dev = self.device
         if out is None:
+            hw = 6 if mode == C.TWG_AMIPS_EGH_SYM else 9
             E = torch.empty(n, dtype=torch.float64, device=dev)
             G = torch.empty((n, 3), dtype=torch.float32, device=dev) if mode >= 1 else None
-            H = torch.empty((n, 9), dtype=torch.float32, device=dev) if mode >= 2 else None
+            H = torch.empty((n, hw), dtype=torch.float32, device=dev) if mode >= 2 else None
             fb = torch.empty(n, dtype=torch.uint8, device=dev)
         else:
             E, G, H, fb = self._amips_out(out, n, mode, torch.float32, True)
