@@ -10,7 +10,8 @@
 //      ballot as the OR of the lanes' step results) against face_out_sampled,
 //      and that every sample of an inside triangle is tested exactly once;
 //   4. collect_facets (one-lane group) against the facets whose boxes meet
-//      the query box, and in_envelope_list on its lists against brute force;
+//      the query box, in_envelope_list on its lists against brute force, and
+//      in_envelope_masked with face_list_mask against in_envelope_list;
 //   5. nearest_facet_group (one-lane group) against nearest_facet, bit for
 //      bit, with queries above shared grid edges and vertices (ties).
 // Exit status 0 when every check passes.
@@ -267,7 +268,7 @@ static void check_lists(std::mt19937_64& rng)
 {
     std::uniform_real_distribution<double> U(0, 1);
     std::vector<unsigned char> cache(kTreeCacheBytes);
-    long lists = 0, points = 0, overflow = 0;
+    long lists = 0, points = 0, overflow = 0, masked = 0;
     for (int nf : {2, 3, 7, 100, 1000, 3001, 20001}) {
         Tree T;
         build(T, nf, rng, 0.02);
@@ -346,10 +347,35 @@ static void check_lists(std::mt19937_64& rng)
                     if (r % 7 == 0) hl = -1;
                     points++;
                 }
+                // a face inside the box, its samples tested through the face's
+                // list mask: the decision and the facet left in the hint are
+                // in_envelope_list's (the mask leaves out only facets farther
+                // than sqrt(T) from the face)
+                if (n >= 0 && n <= 128) {
+                    V3<double> fv[3];
+                    for (int j = 0; j < 3; ++j) {
+                        double* pc[3] = {&fv[j].x, &fv[j].y, &fv[j].z};
+                        for (int k = 0; k < 3; ++k) *pc[k] = lo[k] + pad + (hi[k] - lo[k] - 2 * pad) * U(rng);
+                    }
+                    uint64_t mk[2];
+                    face_list_mask(lst.data(), n, fv[0], fv[1], fv[2], sqrtT, mk);
+                    int hm = -1, hu = -1;
+                    for_each_sample(fv[0], fv[1], fv[2], 0.004, [&](V3<double> p) {
+                        const bool gm = in_envelope_masked(lst.data(), tri.data(), n, mk, p, eps2, sqrtT, hm);
+                        const bool gu = in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hu);
+                        double best = INFINITY;
+                        for (int f = 0; f < nf; ++f) best = std::min(best, facet_d2(W, f, p));
+                        CHECK(gm == gu && hm == hu && gm == (best <= eps2),
+                              "masked lists: %d facets, mode %d, query %d\n", nf, mode, q);
+                        masked++;
+                        return false;
+                    });
+                }
             }
         }
     }
-    printf("facet lists: %ld lists (%ld overflowed), %ld points against brute force\n", lists, overflow, points);
+    printf("facet lists: %ld lists (%ld overflowed), %ld points against brute force, %ld face samples "
+           "through face masks\n", lists, overflow, points, masked);
 }
 
 // a grid of n x n squares over [0, 1]^2, each split into two triangles, on a

@@ -36,6 +36,7 @@ namespace twg {
 namespace {
 
 constexpr int kQ = 10;  // E, J(3), H(6 unique) per tet
+typedef __attribute__((address_space(3))) unsigned long long LdsU64;
 // LDS copy of the surface tree's top levels (FP64 nodes: 8 levels), read by
 // the projection and by the envelope test of the one-ring faces
 constexpr int kSmoothTreeCacheBytes = 16384;
@@ -75,14 +76,18 @@ struct SmoothArgs {
 constexpr int kTilePoolBytes = 57344;
 constexpr int kListEntryBytes = 32 + 8 + 72;
 constexpr int kFaceRecBytes = kFaceRec * 8;
+constexpr int kFaceMaskBytes = 16;  // two 64-bit words: list positions near the face
 template <int TILE>
 constexpr int tile_bytes() { return kTilePoolBytes / (kBlock / TILE); }
 template <int TILE>
 constexpr int face_chunk() { return TILE / 4; }
 template <int TILE>
-constexpr int ring_cap() { return (tile_bytes<TILE>() - face_chunk<TILE>() * kFaceRecBytes) / kListEntryBytes; }
+constexpr int ring_cap()
+{
+    return (tile_bytes<TILE>() - face_chunk<TILE>() * (kFaceRecBytes + kFaceMaskBytes)) / kListEntryBytes;
+}
 static_assert(tile_bytes<16>() >= 16 * kQ * 8 && tile_bytes<64>() >= 64 * kQ * 8, "reduction rows");
-static_assert(ring_cap<64>() == 94 && ring_cap<16>() >= 16, "list capacity");
+static_assert(ring_cap<64>() == 92 && ring_cap<64>() <= 128 && ring_cap<16>() >= 16, "list capacity");
 static_assert((ring_cap<16>() * kListEntryBytes) % 16 == 0, "record alignment");
 
 template <int TILE>
@@ -95,7 +100,8 @@ struct TileCtx {
     LdsNode32W* lst;   // the same slice: facet list (ring_cap entries),
     LdsU32* fr;        // work lists (2 ring_cap entries),
     LdsF64* tri;       // the listed facets' triangles (9 doubles each)
-    LdsF64* frec;      // and face_chunk sampler records (kFaceRec doubles each)
+    LdsF64* frec;      // face_chunk sampler records (kFaceRec doubles each)
+    LdsU64* fmask;     // and per face the list positions near it (2 words)
 };
 
 // acc + v[0] + v[1] + ... + v[cnt - 1] in that order (v[r] = base[r * stride]
@@ -621,6 +627,13 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                     g.init(a, b, cc, A.d);
                     g.store(c.frec + kFaceRec * c.tl);
                     cnt = g.skip(INT64_MAX);
+                    if (nl >= 0) {
+                        // the list positions that can hold a facet near this face
+                        uint64_t mk[2];
+                        face_list_mask(c.lst, nl, a, b, cc, A.sqrtT, mk);
+                        c.fmask[2 * c.tl] = mk[0];
+                        c.fmask[2 * c.tl + 1] = mk[1];
+                    }
                 }
             }
         }
@@ -670,7 +683,8 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                     got = g.next(smp);
                 }
                 if (got)
-                    out = nl >= 0 ? !in_envelope_list(c.lst, c.tri, nl, smp, A.eps2, A.sqrtT, hl)
+                    out = nl >= 0 ? !in_envelope_masked(c.lst, c.tri, nl, c.fmask + 2 * f, smp,
+                                                        A.eps2, A.sqrtT, hl)
                                   : !in_envelope<false, false>(W, smp, A.eps2, A.T, A.sqrtT, hint, st);
             }
             if (tile_any(c, out)) return true;
@@ -803,6 +817,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
         c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
         c.frec = (LdsF64*)(slice + cap * kListEntryBytes);
+        c.fmask = (LdsU64*)(slice + cap * kListEntryBytes + face_chunk<TILE>() * kFaceRecBytes);
     }
     for (;;) {
         unsigned start = 0;
@@ -841,6 +856,7 @@ __device__ __forceinline__ void tile_setup(TileCtx<TILE>& c, unsigned char* pool
     c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
     c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
     c.frec = (LdsF64*)(slice + cap * kListEntryBytes);
+    c.fmask = (LdsU64*)(slice + cap * kListEntryBytes + face_chunk<TILE>() * kFaceRecBytes);
 }
 
 template <int TILE, bool MIXED, int WANT>  // WANT 1: surface kinds, 0: the others

@@ -611,6 +611,78 @@ __device__ __forceinline__ bool in_envelope_list(LdsNode32W* lst, LdsF64c* tri, 
     return false;
 }
 
+// The positions of a collect_facets list (nl <= 128 entries) whose FP32 boxes
+// meet the box of triangle (a, b, c) padded as the list's own box is (by
+// 1.0001 sqrt(T) + 1e-9 max |coordinate|): every facet within sqrt(T) of a
+// point of the triangle -- or of a sample of it, up to its rounding -- is at
+// a set position.  mk: two 64-bit words.
+template <class L>
+__device__ __forceinline__ void face_list_mask(L lst, int nl, V3<double> a, V3<double> b,
+                                               V3<double> c, double sqrtT, uint64_t mk[2])
+{
+    double lo[3] = {a.x, a.y, a.z}, hi[3] = {a.x, a.y, a.z};
+    const double q[6] = {b.x, b.y, b.z, c.x, c.y, c.z};
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        lo[j % 3] = fmin(lo[j % 3], q[j]);
+        hi[j % 3] = fmax(hi[j % 3], q[j]);
+    }
+    double amax = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) amax = fmax(amax, fmax(fabs(lo[k]), fabs(hi[k])));
+    const double pad = 1.0001 * sqrtT + 1e-9 * amax;
+    float lof[3], hif[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lof[k] = f32_down(lo[k] - pad);
+        hif[k] = f32_up(hi[k] + pad);
+    }
+    mk[0] = 0ull;
+    mk[1] = 0ull;
+    for (int e = 0; e < nl && e < 128; ++e) {
+        const BvhNode32 r = from_lds(lst + e);
+        const bool meet = r.lo[0] <= hif[0] && r.hi[0] >= lof[0] && r.lo[1] <= hif[1] &&
+                          r.hi[1] >= lof[1] && r.lo[2] <= hif[2] && r.hi[2] >= lof[2];
+        if (meet) mk[e >> 6] |= 1ull << (e & 63);
+    }
+}
+
+// in_envelope_list over the list positions set in mask (two 64-bit words;
+// lists of at most 128 entries), for a p whose caller guarantees that no
+// facet at an unset position lies within sqrt(T) of p: the facet at hl
+// first, then the set positions' FP32 box tests, then the triangles of the
+// boxes that pass, in ascending position as in_envelope_list.  Every facet
+// that could hold p is at a set position, so the decision and the facet left
+// in hl are in_envelope_list's.
+template <class M>
+__device__ __forceinline__ bool in_envelope_masked(LdsNode32W* lst, LdsF64c* tri, int nl, M mask,
+                                                   V3<double> p, double eps2, double sqrtT, int& hl)
+{
+    if (hl >= 0 && tri_d2(tri + 9 * hl, p) <= eps2) return true;
+    const V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
+    const float Tf = f32_threshold(p, sqrtT);
+    for (int w = 0; w < 2 && 64 * w < nl; ++w) {
+        uint64_t mm = mask[w];
+        if (hl >= 64 * w && hl < 64 * w + 64) mm &= ~(1ull << (hl - 64 * w));
+        uint64_t m = 0;
+        while (mm) {
+            const int b = __builtin_ctzll(mm);
+            mm &= mm - 1;
+            const BvhNode32 e = from_lds(lst + 64 * w + b);
+            if (box_d2(pf, e.lo, e.hi) <= Tf) m |= 1ull << b;
+        }
+        while (m) {
+            const int i = 64 * w + __builtin_ctzll(m);
+            m &= m - 1;
+            if (tri_d2(tri + 9 * i, p) <= eps2) {
+                hl = i;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
 // nearest_facet with the reference's traversal order
 // (src/tetwild/geogram/mesh_AABB.h:130-141, mesh_AABB.cpp:381-480):
 // get_nearest_facet_hint, then depth-first search visiting the nearer child
