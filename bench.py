@@ -173,6 +173,21 @@ def launch_sequence(fn, reps):
     return np.array([a.elapsed_time(b) / 1e3 for a, b in ev])
 
 
+def per_call_rate(call, calls, warm=20):
+    """calls/s of call(i), one C-ABI call for item i, with the device
+    synchronised after each: a caller that needs each result before its next
+    call, as the reference's per-element functions are used"""
+    import torch
+    for i in range(warm):
+        call(i)
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(calls):
+        call(i)
+        torch.cuda.synchronize()
+    return calls / (time.perf_counter() - t0)
+
+
 def bits_mismatch(a, b):
     import numpy as np
     return int(np.count_nonzero(np.ascontiguousarray(a).view(np.int64) !=
@@ -265,6 +280,24 @@ def bench_amips(ctx, args, result):
         "tol": tol9, "tet_evals_per_s": n / wall_9, "fallback_rate": fb.float().mean().item(),
         "max_rel_err_E_vs_fp64": ((Em - E).abs() / E.abs()).max().item(),
         "speedup_vs_fp64": wall / wall_9}
+
+    # the plain drop-in form: one tet per call (comformalAMIPS*_new are
+    # per-tet functions), synchronised per call
+    ctx._bind()
+    L, hc = ctx.L, ctx.h
+    pt, px, py, pz = tt.data_ptr(), tx.data_ptr(), ty.data_ptr(), tz.data_ptr()
+    pE, pG, pH = E.data_ptr(), G.data_ptr(), H.data_ptr()
+
+    def one_tet(i):
+        r = L.twg_amips_eval(hc, pt + 16 * i, 1, px, py, pz, nv, 2, pE + 8 * i, pG + 24 * i,
+                             pH + 72 * i)
+        if r:
+            raise RuntimeError("twg_amips_eval failed")
+    per_call = per_call_rate(one_tet, 2000)
+    result["secondary"]["amips_mixed"]["one_tet_per_call_fp64"] = {
+        "calls": 2000, "tet_evals_per_s": per_call,
+        "mixed_batched_vs_it": (n / wall_m) / per_call,
+        "fp64_batched_vs_it": (n / wall) / per_call}
 
     # the packed Hessian (TWG_AMIPS_EGH_SYM: its upper triangle, 96 instead of
     # 120 output bytes per tet), FP64 and mixed, against the full layout
@@ -388,6 +421,17 @@ def bench_envelope(ctx, args, result, surf_cache):
             max(2, args.steps // 4), 1)
         out[name] = o.cpu().numpy()
         rates[name] = (nq / wall, kern)
+    # the plain drop-in form: one triangle per call (isFaceOutEnvelop is a
+    # per-triangle function), synchronised per call
+    ctx._bind()
+    L, hc, hs = ctx.L, ctx.h, surf.h
+    pq, po = tq.data_ptr(), buf.data_ptr()
+
+    def one_tri(i):
+        r = L.twg_envelope_faces(hc, hs, pq + 72 * i, 1, eps * eps, eps, 0, po + i, None)
+        if r:
+            raise RuntimeError("twg_envelope_faces failed")
+    per_call = per_call_rate(one_tri, 1000)
     samples, trav, boxes, leaves = (float(v) / nq for v in stats)
     alg = ENV_QUERY_BYTES + boxes * ENV_BOX_BYTES + leaves * ENV_LEAF_BYTES
     sec = {
@@ -397,6 +441,9 @@ def bench_envelope(ctx, args, result, surf_cache):
         "mixed_speedup": rates["mixed"][0] / rates["fp64"][0],
         "decision_mismatch_mixed_vs_fp64": int(np.count_nonzero(out["fp64"] != out["mixed"])),
         "out_rate": float(out["fp64"].mean()),
+        "one_triangle_per_call_fp64": {"calls": 1000, "checks_per_s": per_call,
+                                       "mixed_batched_vs_it": rates["mixed"][0] / per_call,
+                                       "fp64_batched_vs_it": rates["fp64"][0] / per_call},
         "per_check": {"samples_tested": samples, "full_traversals": trav, "boxes": boxes,
                       "facets": leaves},
         "alg_bytes_per_check": alg,
