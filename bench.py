@@ -849,7 +849,7 @@ def bench_config1(ctx, args, result):
         sec[f"smooth_{mode}"] = {"counter": info["counter"], "suc_counter": info["suc_counter"],
                                  "moved": int((Pn != P).any(1).sum()),
                                  "energy_after": {"max": float(qn.max()), "avg": float(qn.mean())},
-                                 "seconds": el}
+                                 "seconds": el, "smooth_seconds": info["smooth_seconds"]}
         if mode == "seq":
             sec["smooth_seq"]["levels"] = [info["levels_single"], info["levels_surface"]]
     sec["smooth_seq_vs_serial_bit_mismatch"] = bits_mismatch(raws["seq"], raws["serial"])

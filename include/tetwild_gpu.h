@@ -265,6 +265,25 @@ int twg_smooth_plan_run(twg_smooth_plan* plan);
 int twg_smooth_plan_info(const twg_smooth_plan* plan, int* tile, int* chunk);
 int twg_smooth_plan_destroy(twg_smooth_plan* plan);
 
+/* For a caller that keeps the mesh on the device and runs batches of
+ * independent problems one after another (no two problems of a batch share a
+ * tet: the levels of the reference's vertex order, or colour classes), so
+ * that no batch waits on the host.  No counterpart in the reference, whose
+ * positions and qualities are host arrays updated between vertices.
+ * twg_smooth_gather_q: desc->old_q[r] = q[ring_tid[r]] for the batch's
+ * n_ring entries (NaN for an id outside [0, n_q)); desc->old_q must be
+ * writable device memory.
+ * twg_smooth_apply, after twg_smooth_batch on the same descriptor: for each
+ * problem of kind 0, 1 or 2 with status TWG_SMOOTH_MOVED, writes p_out into
+ * x, y, z at its centre and, for kinds 1 and 2 (q non-NULL), its new_q into
+ * q[ring_tid[r]] over its ring -- the reference's updates of posf and
+ * tet_qualities after a move (src/tetwild/VertexSmoother.cpp:196-211,
+ * 441-453).  Enqueued on the context's stream like the batch. */
+int twg_smooth_gather_q(twg_ctx* ctx, const twg_smooth_desc* desc, const int32_t* ring_tid,
+                        const double* q, int64_t n_q);
+int twg_smooth_apply(twg_ctx* ctx, const twg_smooth_desc* desc, const int32_t* ring_tid,
+                     double* x, double* y, double* z, double* q, int64_t n_q);
+
 /* One step of NewtonsMethod (src/tetwild/VertexSmoother.cpp:478-528) laid
  * out for a caller that decides some of its flip tests itself: the tets with
  * an unrounded vertex other than the moving one, which the reference orients

@@ -61,6 +61,8 @@ SIGNATURES = {
     "twg_smooth_plan_info": [P, P, P],
     "twg_smooth_plan_destroy": [P],
     "twg_newton_step": [P, P, P, P, i64, P, i32, i32, f64, f64, f64, P, P],
+    "twg_smooth_gather_q": [P, P, P, P, i64],
+    "twg_smooth_apply": [P, P, P, P, P, P, P, i64],
     "twg_collapse_batch": [P, P, i32, i32, i32],
     "twg_collapse_autotune": [P, P, i32, P, P, P],
     "twg_collapse_plan_create": [P, P, i32, i32, i32, ctypes.POINTER(P)],

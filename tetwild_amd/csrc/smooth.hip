@@ -961,6 +961,39 @@ __global__ __launch_bounds__(64) void newton_step_kernel(SmoothArgs A, const int
     }
 }
 
+// twg_smooth_gather_q: old_q[r] = q[ring_tid[r]] (NaN for an id outside q)
+__global__ __launch_bounds__(kBlock) void gather_q_kernel(const int32_t* __restrict__ ring_tid,
+                                                          int64_t n_ring,
+                                                          const double* __restrict__ q, int64_t n_q,
+                                                          double* __restrict__ old_q)
+{
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n_ring) return;
+    const int32_t t = ring_tid[r];
+    old_q[r] = t >= 0 && t < n_q ? q[t] : __builtin_nan("");
+}
+
+// twg_smooth_apply: one thread per problem
+__global__ __launch_bounds__(kBlock) void apply_kernel(SmoothArgs A, const int32_t* __restrict__ ring_tid,
+                                                       double* x, double* y, double* z, double* q,
+                                                       int64_t n_q)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= A.n || A.status[p] != TWG_SMOOTH_MOVED) return;
+    const int kind = A.kind ? A.kind[p] : 0;
+    const int v = A.center[p];
+    if (kind > 2 || v < 0 || v >= A.nv) return;
+    x[v] = A.p_out[3 * p];
+    y[v] = A.p_out[3 * p + 1];
+    z[v] = A.p_out[3 * p + 2];
+    if (kind == 0 || !A.new_q || !q) return;
+    const int64_t r0 = A.ring_off[p], r1 = A.ring_off[p + 1];
+    for (int64_t r = r0; r < r1 && r < A.n_ring; ++r) {
+        const int32_t t = ring_tid[r];
+        if (t >= 0 && t < n_q) q[t] = A.new_q[r];
+    }
+}
+
 using KernelFn = void (*)(SmoothArgs);
 
 KernelFn pick(int tile, bool mixed)
@@ -1229,6 +1262,38 @@ int twg_smooth_plan_info(const twg_smooth_plan* P, int* tile, int* chunk)
     TWG_REQUIRE(P != nullptr, "twg_smooth_plan_info: NULL plan");
     if (tile) *tile = P->tile;
     if (chunk) *chunk = (int)P->chunk;
+    return TWG_OK;
+}
+
+int twg_smooth_gather_q(twg_ctx* ctx, const twg_smooth_desc* d, const int32_t* ring_tid,
+                        const double* q, int64_t n_q)
+{
+    TWG_REQUIRE(ctx != nullptr && d != nullptr, "twg_smooth_gather_q: NULL argument");
+    TWG_REQUIRE(d->n_ring >= 0 && n_q >= 0, "twg_smooth_gather_q: negative size");
+    if (d->n_ring == 0) return TWG_OK;
+    TWG_REQUIRE(ring_tid && q && d->old_q, "twg_smooth_gather_q: NULL array");
+    if (int r = bind(ctx)) return r;
+    hipLaunchKernelGGL(gather_q_kernel, dim3(grid_for(d->n_ring)), dim3(kBlock), 0, ctx->cur, ring_tid,
+                       d->n_ring, q, n_q, const_cast<double*>(d->old_q));
+    TWG_LAUNCHED();
+    return TWG_OK;
+}
+
+int twg_smooth_apply(twg_ctx* ctx, const twg_smooth_desc* d, const int32_t* ring_tid, double* x,
+                     double* y, double* z, double* q, int64_t n_q)
+{
+    TWG_REQUIRE(ctx != nullptr && d != nullptr, "twg_smooth_apply: NULL argument");
+    TWG_REQUIRE(d->n >= 0 && n_q >= 0, "twg_smooth_apply: negative size");
+    if (d->n == 0) return TWG_OK;
+    TWG_REQUIRE(x && y && z && d->center && d->ring_off && d->p_out && d->status,
+                "twg_smooth_apply: NULL array");
+    TWG_REQUIRE(!q || ring_tid, "twg_smooth_apply: q needs ring_tid");
+    if (int r = bind(ctx)) return r;
+    SmoothArgs A;
+    if (int r = make_args(ctx, d, nullptr, 1, A)) return r;
+    hipLaunchKernelGGL(apply_kernel, dim3(grid_for(d->n)), dim3(kBlock), 0, ctx->cur, A, ring_tid, x, y,
+                       z, q, n_q);
+    TWG_LAUNCHED();
     return TWG_OK;
 }
 

@@ -35,6 +35,7 @@
 // collapse out.bin: int32 codes[n]; int32 counts[n] (new qualities per
 // candidate); double new_q[sum counts]; uint8 flags[nv] after the checks.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -263,6 +264,7 @@ int main(int argc, char** argv)
         vs.exact = n_unrounded ? &exact : nullptr;
         int64_t counters[2];
         int levels[2] = {0, 0};
+        const auto t_smooth = std::chrono::steady_clock::now();
         if (mode == "seq") {
             // smooth(), recording the levels of its two loops
             vs.smoothSingle();
@@ -278,6 +280,8 @@ int main(int argc, char** argv)
             vs.smoothSingleBatched();
             if (st.eps >= 0) vs.smoothSurfaceBatched();
         }
+        const double smooth_s =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t_smooth).count();
         counters[0] = vs.counter;
         counters[1] = vs.suc_counter;
         FILE* o = fopen(argv[2], "wb");
@@ -317,9 +321,10 @@ int main(int argc, char** argv)
             fclose(r);
         }
         printf("{\"counter\": %lld, \"suc_counter\": %lld, \"max_energy\": %.17g, "
-               "\"exact_calls\": %ld, \"levels_single\": %d, \"levels_surface\": %d}\n",
+               "\"exact_calls\": %ld, \"levels_single\": %d, \"levels_surface\": %d, "
+               "\"smooth_seconds\": %.6f}\n",
                (long long)counters[0], (long long)counters[1], vs.getMaxEnergy(), exact.calls,
-               levels[0], levels[1]);
+               levels[0], levels[1], smooth_s);
     } catch (const std::exception& e) {
         fprintf(stderr, "shim_driver: %s\n", e.what());
         return 1;

@@ -925,6 +925,142 @@ std::vector<std::vector<int>> VertexSmoother::orderLevels(const std::vector<int>
     return levels;
 }
 
+bool VertexSmoother::allRounded() const
+{
+    for (size_t v = 0; v < tet_vertices.size(); ++v)
+        if (!v_is_removed[v] && !tet_vertices[v].is_rounded) return false;
+    return true;
+}
+
+void VertexSmoother::runLevelsOnDevice(const std::vector<std::vector<int>>& levels, bool surface,
+                                       int& suc)
+{
+    const size_t nv = tet_vertices.size(), nt = tets.size();
+    std::vector<int32_t> center, ring_off{0}, ring_tets, ring_tid, tri_off{0}, tri_ab;
+    std::vector<uint8_t> kind;
+    std::vector<size_t> lv_start{0};
+    std::vector<int> ring;
+    for (const auto& lv : levels) {
+        for (int v : lv) {
+            const int k = surface ? (tet_vertices[v].is_on_boundary ? 2 : 1) : 0;
+            center.push_back(v);
+            kind.push_back((uint8_t)k);
+            oneRing(v, ring);
+            for (int t : ring) {
+                for (int j = 0; j < 4; ++j) ring_tets.push_back(tets[t][j]);
+                ring_tid.push_back(t);
+            }
+            ring_off.push_back((int32_t)(ring_tets.size() / 4));
+            if (k != 0) {
+                // one-ring surface faces, as runBatch lists them
+                std::vector<std::array<int, 3>> tri_ids;
+                for (int t : ring)
+                    for (int j = 0; j < 4; ++j)
+                        if (tets[t][j] != v && is_surface_fs[t][j] != State::NOT_SURFACE) {
+                            std::array<int, 3> tri = {{tets[t][(j + 1) % 4], tets[t][(j + 2) % 4],
+                                                       tets[t][(j + 3) % 4]}};
+                            std::sort(tri.begin(), tri.end());
+                            tri_ids.push_back(tri);
+                        }
+                std::sort(tri_ids.begin(), tri_ids.end());
+                tri_ids.erase(std::unique(tri_ids.begin(), tri_ids.end()), tri_ids.end());
+                for (const auto& tri : tri_ids) {
+                    const int kk = (int)(std::find(tri.begin(), tri.end(), v) - tri.begin());
+                    tri_ab.push_back(tri[(kk + 1) % 3]);
+                    tri_ab.push_back(tri[(kk + 2) % 3]);
+                }
+            }
+            tri_off.push_back((int32_t)(tri_ab.size() / 2));
+        }
+        lv_start.push_back(center.size());
+    }
+    const size_t np = center.size(), nr = ring_tid.size(), ntri = tri_ab.size() / 2;
+    if (np == 0) return;
+    std::vector<double> xyzq(3 * nv + nt);
+    for (size_t v = 0; v < nv; ++v)
+        for (int c = 0; c < 3; ++c) xyzq[c * nv + v] = tet_vertices[v].posf[c];
+    for (size_t t = 0; t < nt; ++t)
+        xyzq[3 * nv + t] = tet_qualities.size() > t ? tet_qualities[t].slim_energy : 0.0;
+    // device copies (runBatch's buffer slots 0..5)
+    double* dx = (double*)dev.buffer(0, 8 * (3 * nv + nt));
+    double *dy = dx + nv, *dz = dy + nv, *dq = dz + nv;
+    int32_t* dring = (int32_t*)dev.buffer(1, 16 * nr + 16);
+    int32_t* dcenter = (int32_t*)dev.buffer(2, 4 * (np + 2 * (np + 1) + 2 * ntri + nr) + 64);
+    int32_t* droff = dcenter + np;
+    int32_t* dtoff = droff + (np + 1);
+    int32_t* dtri = dtoff + (np + 1);
+    int32_t* dtid = dtri + 2 * ntri;
+    uint8_t* dkind = (uint8_t*)dev.buffer(3, 2 * np);
+    uint8_t* dstatus = dkind + np;
+    double* dold = (double*)dev.buffer(4, 16 * nr + 16);
+    double* dnew = dold + nr;
+    double* dpout = (double*)dev.buffer(5, 24 * np);
+    dev.h2d(dx, xyzq.data(), 8 * xyzq.size());
+    dev.h2d(dring, ring_tets.data(), 16 * nr);
+    dev.h2d(dcenter, center.data(), 4 * np);
+    dev.h2d(droff, ring_off.data(), 4 * (np + 1));
+    dev.h2d(dtoff, tri_off.data(), 4 * (np + 1));
+    dev.h2d(dtri, tri_ab.data(), 8 * ntri);
+    dev.h2d(dtid, ring_tid.data(), 4 * nr);
+    dev.h2d(dkind, kind.data(), np);
+    for (size_t l = 0; l + 1 < lv_start.size(); ++l) {
+        const size_t s0 = lv_start[l], n = lv_start[l + 1] - s0;
+        const size_t r0 = ring_off[s0], r1 = ring_off[s0 + n];
+        twg_smooth_desc d;
+        std::memset(&d, 0, sizeof d);
+        d.x = dx;
+        d.y = dy;
+        d.z = dz;
+        d.nv = (int64_t)nv;
+        d.n = (int64_t)n;
+        d.center = dcenter + s0;
+        d.ring_off = droff + s0;  // offsets into the whole ring arrays
+        d.ring_tets = dring;
+        d.n_ring = (int64_t)nr;
+        d.kind = dkind + s0;
+        d.old_q = dold;
+        d.tri_off = dtoff + s0;
+        d.tri_ab = dtri;
+        d.n_tri = (int64_t)ntri;
+        d.sf_tree = geo_sf_tree.bvh();
+        d.b_tree = geo_b_tree.bvh();
+        d.eps2 = state.eps_2;
+        d.sampling_dist = state.sampling_dist > 0 ? state.sampling_dist : 1.0;
+        d.p_out = dpout + 3 * s0;
+        d.status = dstatus + s0;
+        d.new_q = dnew;
+        if (surface) {
+            // this level's old qualities, as the earlier levels left them
+            twg_smooth_desc g = d;
+            g.old_q = dold + r0;
+            g.n_ring = (int64_t)(r1 - r0);
+            check(twg_smooth_gather_q(dev.ctx(), &g, dtid + r0, dq, (int64_t)nt), "twg_smooth_gather_q");
+        }
+        check(twg_smooth_batch(dev.ctx(), &d, precision, TWG_SMOOTH_TILE_SPLIT, 1), "twg_smooth_batch");
+        check(twg_smooth_apply(dev.ctx(), &d, dtid, dx, dy, dz, surface ? dq : nullptr, (int64_t)nt),
+              "twg_smooth_apply");
+    }
+    std::vector<uint8_t> status(np);
+    dev.d2h(status.data(), dstatus, np);
+    dev.d2h(xyzq.data(), dx, 8 * xyzq.size());
+    for (size_t p = 0; p < np; ++p) {
+        const int v = center[p];
+        if (status[p] == TWG_SMOOTH_BAD_INPUT)
+            throw std::runtime_error("twg_smooth_batch: bad input for vertex " + std::to_string(v));
+        if (status[p] == TWG_SMOOTH_MOVED) {
+            tet_vertices[v].posf = {{xyzq[v], xyzq[nv + v], xyzq[2 * nv + v]}};
+            if (kind[p] != 0)
+                for (int32_t r = ring_off[p]; r < ring_off[p + 1]; ++r) {
+                    TetQuality q;
+                    q.slim_energy = xyzq[3 * nv + ring_tid[r]];
+                    tet_qualities[ring_tid[r]] = q;
+                }
+        }
+        afterMove(v, status[p]);
+        if (status[p] == TWG_SMOOTH_MOVED || (!surface && status[p] == TWG_SMOOTH_FLIP_END)) suc++;
+    }
+}
+
 // src/tetwild/VertexSmoother.cpp:114-242 with the vertex order's results.
 // Why the levels give them: the body for v reads only its one-ring (the
 // positions, rounding and qualities of vertices sharing a tet with v) and
@@ -947,9 +1083,14 @@ void VertexSmoother::smoothSingle()
     }
     const auto levels = orderLevels(verts);
     last_levels = (int)levels.size();
-    for (const auto& lv : levels) {
-        runClass(lv, false, suc_counter);
-        counter += (int)lv.size();
+    if (allRounded()) {
+        runLevelsOnDevice(levels, false, suc_counter);
+        counter += (int)verts.size();
+    } else {
+        for (const auto& lv : levels) {
+            runClass(lv, false, suc_counter);
+            counter += (int)lv.size();
+        }
     }
     finishSingle();
 }
@@ -962,9 +1103,14 @@ void VertexSmoother::smoothSurface()
     const std::vector<int> verts = surfaceCandidates();
     const auto levels = orderLevels(verts);
     last_levels = (int)levels.size();
-    for (const auto& lv : levels) {
-        runClass(lv, true, suc_counter);
-        counter += (int)lv.size();
+    if (allRounded()) {
+        runLevelsOnDevice(levels, true, suc_counter);
+        counter += (int)verts.size();
+    } else {
+        for (const auto& lv : levels) {
+            runClass(lv, true, suc_counter);
+            counter += (int)lv.size();
+        }
     }
 }
 

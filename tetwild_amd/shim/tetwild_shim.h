@@ -278,6 +278,12 @@ private:
     std::vector<std::vector<int>> colourClasses(const std::vector<int>& verts) const;
     // verts (in vertex order) grouped by dependency level
     std::vector<std::vector<int>> orderLevels(const std::vector<int>& verts) const;
+    // every live vertex is rounded (no test needs ExactPositions)
+    bool allRounded() const;
+    // the levels with positions and qualities kept on the device: per level
+    // a quality gather, one batch and the write-back, no host round trip
+    // between levels; the same updates and counts as runClass per level
+    void runLevelsOnDevice(const std::vector<std::vector<int>>& levels, bool surface, int& suc);
     // the surface loop's flag updates (isolated vertices leave the surface,
     // non-boundary points the boundary), in vertex order; returns the
     // vertices smoothSurface smooths
