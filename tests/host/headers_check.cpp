@@ -268,7 +268,7 @@ static void check_lists(std::mt19937_64& rng)
 {
     std::uniform_real_distribution<double> U(0, 1);
     std::vector<unsigned char> cache(kTreeCacheBytes);
-    long lists = 0, points = 0, overflow = 0, masked = 0;
+    long lists = 0, points = 0, overflow = 0, masked = 0, listed_faces = 0;
     for (int nf : {2, 3, 7, 100, 1000, 3001, 20001}) {
         Tree T;
         build(T, nf, rng, 0.02);
@@ -360,9 +360,14 @@ static void check_lists(std::mt19937_64& rng)
                     uint64_t mk[2];
                     face_list_mask(lst.data(), n, fv[0], fv[1], fv[2], sqrtT, mk);
                     int hm = -1, hu = -1;
+                    const bool nbr = q % 2 == 1;  // these lists carry neighbour positions
                     for_each_sample(fv[0], fv[1], fv[2], 0.004, [&](V3<double> p) {
-                        const bool gm = in_envelope_masked(lst.data(), tri.data(), n, mk, p, eps2, sqrtT, hm);
-                        const bool gu = in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hu);
+                        const bool gm =
+                            nbr ? in_envelope_masked<true>(lst.data(), tri.data(), n, mk, p, eps2, sqrtT, hm)
+                                : in_envelope_masked(lst.data(), tri.data(), n, mk, p, eps2, sqrtT, hm);
+                        const bool gu =
+                            nbr ? in_envelope_list<true>(lst.data(), tri.data(), n, p, eps2, sqrtT, hu)
+                                : in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hu);
                         double best = INFINITY;
                         for (int f = 0; f < nf; ++f) best = std::min(best, facet_d2(W, f, p));
                         CHECK(gm == gu && hm == hu && gm == (best <= eps2),
@@ -370,12 +375,28 @@ static void check_lists(std::mt19937_64& rng)
                         masked++;
                         return false;
                     });
+                    // the whole decision of face_out_listed (masked scans)
+                    // against the per-sample tree walk
+                    double t9[9];
+                    for (int j = 0; j < 3; ++j) {
+                        t9[3 * j] = fv[j].x;
+                        t9[3 * j + 1] = fv[j].y;
+                        t9[3 * j + 2] = fv[j].z;
+                    }
+                    Stats st;
+                    const bool lo_ = face_out_listed<false>(W, t9, eps2, 0.004, Tt, sqrtT, lst.data(),
+                                                           tri.data(), n);
+                    const bool ref =
+                        face_out_sampled<false, false>(walk(T.view), t9, eps2, 0.004, Tt, sqrtT, st);
+                    CHECK(lo_ == ref, "face_out_listed: %d facets, mode %d, query %d\n", nf, mode, q);
+                    listed_faces++;
                 }
             }
         }
     }
     printf("facet lists: %ld lists (%ld overflowed), %ld points against brute force, %ld face samples "
-           "through face masks\n", lists, overflow, points, masked);
+           "through face masks, %ld face_out_listed decisions\n", lists, overflow, points, masked,
+           listed_faces);
 }
 
 // a grid of n x n squares over [0, 1]^2, each split into two triangles, on a

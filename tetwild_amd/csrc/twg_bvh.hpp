@@ -654,16 +654,35 @@ __device__ __forceinline__ void face_list_mask(L lst, int nl, V3<double> a, V3<d
 // boxes that pass, in ascending position as in_envelope_list.  Every facet
 // that could hold p is at a set position, so the decision and the facet left
 // in hl are in_envelope_list's.
-template <class M>
+template <bool NB = false, class M>
 __device__ __forceinline__ bool in_envelope_masked(LdsNode32W* lst, LdsF64c* tri, int nl, M mask,
                                                    V3<double> p, double eps2, double sqrtT, int& hl)
 {
-    if (hl >= 0 && tri_d2(tri + 9 * hl, p) <= eps2) return true;
+    int nb[3] = {-1, -1, -1};
+    if (hl >= 0) {
+        if (tri_d2(tri + 9 * hl, p) <= eps2) return true;
+        // NB: the facet's edge neighbours next, as in_envelope_list<true>
+        const uint32_t pk = NB ? (uint32_t)lst[hl].pad : 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t q = (pk >> (8 * k)) & 0xffu;
+            if (q != 0xffu) {
+                if (tri_d2(tri + 9 * q, p) <= eps2) {
+                    hl = (int)q;
+                    return true;
+                }
+                nb[k] = (int)q;
+            }
+        }
+    }
     const V3<float> pf = mk3((float)p.x, (float)p.y, (float)p.z);
     const float Tf = f32_threshold(p, sqrtT);
     for (int w = 0; w < 2 && 64 * w < nl; ++w) {
         uint64_t mm = mask[w];
         if (hl >= 64 * w && hl < 64 * w + 64) mm &= ~(1ull << (hl - 64 * w));
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (nb[k] >= 64 * w && nb[k] < 64 * w + 64) mm &= ~(1ull << (nb[k] - 64 * w));
         uint64_t m = 0;
         while (mm) {
             const int b = __builtin_ctzll(mm);
@@ -917,6 +936,14 @@ __device__ bool face_out_listed(const BvhWalk& B, const double* t9, double eps2,
         if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) return true;
     }
     int hl = -1;
+    if (nl <= 128) {
+        // only the listed facets that can lie near this triangle
+        uint64_t mk[2];
+        face_list_mask(lst, nl, a, b, c, sqrtT, mk);
+        return for_each_sample(a, b, c, d, [&](V3<double> s) {
+            return !in_envelope_masked<true>(lst, tri, nl, mk, s, eps2, sqrtT, hl);
+        });
+    }
     return for_each_sample(a, b, c, d, [&](V3<double> s) {
         return !in_envelope_list<true>(lst, tri, nl, s, eps2, sqrtT, hl);
     });
