@@ -168,6 +168,48 @@ def test_packed_hessian(ctx, dev, case):
     assert np.array_equal(Hm3.view(np.int32), np.ascontiguousarray(Hm2[:, UPPER]).view(np.int32))
 
 
+def test_output_alignment(ctx, dev):
+    """outputs that start off a 16-byte boundary (a caller's offset pointers:
+    the rows are then stored one element per lane) hold the same bits as
+    aligned ones (16 bytes per lane), FP64 and mixed, every mode"""
+    x, y, z, tets, _ = W.kuhn_lattice(9, seed=3)
+    tx, ty, tz, tt = to_dev(dev, x, y, z, tets)
+    n = len(tets) - 1
+    sub = tt[1:]
+
+    def host(ts):
+        return [None if t is None else t.cpu().numpy() for t in ts]
+
+    for mode in (1, 2, C.TWG_AMIPS_EGH_SYM):
+        hw = 6 if mode == C.TWG_AMIPS_EGH_SYM else 9
+        print(f"mode={mode}", flush=True)
+        ref = host(ctx.amips_eval(sub, tx, ty, tz, mode=mode))
+        Eb = torch.zeros(n + 1, dtype=torch.float64, device=dev)
+        Gb = torch.zeros((n + 1, 3), dtype=torch.float64, device=dev)
+        Hb = torch.zeros((n + 1, hw), dtype=torch.float64, device=dev) if mode >= 2 else None
+        ctx.amips_eval(sub, tx, ty, tz, mode=mode,
+                       out=(Eb[1:], Gb[1:], Hb[1:] if Hb is not None else None))
+        assert Gb[1:].data_ptr() % 16 == 8
+        got = host((Eb[1:], Gb[1:], Hb[1:] if Hb is not None else None))
+        for a, b in zip(got, ref):
+            assert (a is None) == (b is None)
+            if a is not None:
+                assert np.array_equal(bits(a), bits(b))
+        refm = host(ctx.amips_eval_mixed(sub, tx, ty, tz, mode=mode))
+        Emb = torch.zeros(n + 1, dtype=torch.float64, device=dev)
+        Gmb = torch.zeros((n + 2, 3), dtype=torch.float32, device=dev)
+        Hmb = torch.zeros((n + 2, hw), dtype=torch.float32, device=dev) if mode >= 2 else None
+        fbb = torch.zeros(n + 1, dtype=torch.uint8, device=dev)
+        outm = (Emb[1:], Gmb[1:n + 1], Hmb[1:n + 1] if Hmb is not None else None, fbb[1:])
+        ctx.amips_eval_mixed(sub, tx, ty, tz, mode=mode, out=outm)
+        gotm = host(outm)
+        assert np.array_equal(bits(gotm[0]), bits(refm[0]))
+        assert np.array_equal(gotm[1].view(np.int32), refm[1].view(np.int32))
+        if mode >= 2:
+            assert np.array_equal(gotm[2].view(np.int32), refm[2].view(np.int32))
+        assert np.array_equal(gotm[3], refm[3])
+
+
 def test_mixed_certified(ctx, dev):
     x, y, z, tets = W.aspect_tets(60000, seed=8)
     tx, ty, tz, tt = to_dev(dev, x, y, z, tets)

@@ -37,8 +37,10 @@ __device__ __forceinline__ bool load_tet(const int32_t* __restrict__ tets, int64
 }
 
 // Block-cooperative row store: thread t of the block holds row (base + t) of
-// K values; the rows go out through LDS so that every store instruction
-// writes 64 consecutive elements (dst[(base + t) * K + k], rows >= n skipped).
+// K values; the rows go out through LDS so that the block writes one
+// contiguous run (dst[(base + t) * K + k], rows >= n skipped), 16 bytes per
+// lane and store instruction (1 KiB per wave) when the run is 16-byte
+// aligned, one element per lane otherwise (a caller's offset pointer).
 // Nontemporal: the outputs are streamed and never re-read by this kernel.
 // Every thread of the block must call it.
 template <int K, typename T>
@@ -52,10 +54,28 @@ __device__ __forceinline__ void block_store_rows(T* __restrict__ dst, int64_t ba
     int64_t rows = n - base < kBlock ? n - base : kBlock;
     int total = (int)rows * K;
     T* out = dst + base * K;
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V)));
+    if (((uintptr_t)out & 15) == 0) {
+        const int nvec = total / V;
 #pragma unroll
-    for (int r = 0; r < K; ++r) {
-        int j = r * kBlock + t;
+        for (int r = 0; r < (K + V - 1) / V; ++r) {
+            const int j = r * kBlock + t;
+            if (j < nvec) {
+                vec w;
+#pragma unroll
+                for (int e = 0; e < V; ++e) w[e] = lds[V * j + e];
+                __builtin_nontemporal_store(w, reinterpret_cast<vec*>(out) + j);
+            }
+        }
+        const int j = nvec * V + t;  // the last total % V elements
         if (j < total) __builtin_nontemporal_store(lds[j], &out[j]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            int j = r * kBlock + t;
+            if (j < total) __builtin_nontemporal_store(lds[j], &out[j]);
+        }
     }
     __syncthreads();
 }
