@@ -614,6 +614,7 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
         const int nfc = t1 - base < FC ? (int)(t1 - base) : FC;
         int64_t cnt = 0;
         bool huge = false;
+        float flo[3] = {0.f, 0.f, 0.f}, fhi[3] = {0.f, 0.f, 0.f};  // the face's mask box
         if (c.tl < nfc) {
             double t9[9];
             ring_face(A, base + c.tl, pf, t9);
@@ -627,18 +628,52 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                     g.init(a, b, cc, A.d);
                     g.store(c.frec + kFaceRec * c.tl);
                     cnt = g.skip(INT64_MAX);
-                    if (nl >= 0) {
-                        // the list positions that can hold a facet near this face
-                        uint64_t mk[2];
-                        face_list_mask(c.lst, nl, a, b, cc, A.sqrtT, mk);
-                        c.fmask[2 * c.tl] = mk[0];
-                        c.fmask[2 * c.tl + 1] = mk[1];
-                    }
+                    face_mask_box(a, b, cc, A.sqrtT, flo, fhi);
                 }
             }
         }
-        wave_sync();  // the records, read below by every lane of the tile
         if (tile_any(c, huge)) return true;
+        if (nl >= 0) {
+            // face_list_mask for every face of the chunk at once: lane j tests
+            // list entries j, j + TILE, ... against each face's box (from the
+            // face's lane), and a ballot collects each face's words; the face's
+            // lane keeps them (faces without samples get an unused mask)
+            constexpr int per_lane = (ring_cap<TILE>() + TILE - 1) / TILE;
+            static_assert(ring_cap<TILE>() <= 128, "two mask words");
+            BvhNode32 er[per_lane];
+#pragma unroll
+            for (int i = 0; i < per_lane; ++i)
+                if (c.tl + TILE * i < nl) er[i] = from_lds(c.lst + c.tl + TILE * i);
+            const unsigned long long tmk = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
+            uint64_t my0 = 0ull, my1 = 0ull;
+            for (int k = 0; k < nfc; ++k) {
+                float kl[3], kh[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    kl[j] = __shfl(flo[j], k, TILE);
+                    kh[j] = __shfl(fhi[j], k, TILE);
+                }
+                uint64_t w0 = 0ull, w1 = 0ull;
+#pragma unroll
+                for (int i = 0; i < per_lane; ++i) {
+                    const int e = c.tl + TILE * i;
+                    const bool meet = e < nl && box_meets(er[i], kl, kh);
+                    const uint64_t bits = (__ballot(meet) >> c.wbase) & tmk;
+                    const int pos = TILE * i;  // a multiple of TILE: bits stay in one word
+                    if (pos < 64) w0 |= bits << pos;
+                    else w1 |= bits << (pos - 64);
+                }
+                if (c.tl == k) {
+                    my0 = w0;
+                    my1 = w1;
+                }
+            }
+            if (c.tl < nfc) {
+                c.fmask[2 * c.tl] = my0;
+                c.fmask[2 * c.tl + 1] = my1;
+            }
+        }
+        wave_sync();  // the records and masks, read below by every lane of the tile
         // faces of this chunk with samples (bit k: face base + k)
         const unsigned long long tm = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
         const unsigned long long has = (__ballot(cnt > 0) >> c.wbase) & tm;

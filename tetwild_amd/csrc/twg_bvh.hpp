@@ -616,9 +616,9 @@ __device__ __forceinline__ bool in_envelope_list(LdsNode32W* lst, LdsF64c* tri, 
 // 1.0001 sqrt(T) + 1e-9 max |coordinate|): every facet within sqrt(T) of a
 // point of the triangle -- or of a sample of it, up to its rounding -- is at
 // a set position.  mk: two 64-bit words.
-template <class L>
-__device__ __forceinline__ void face_list_mask(L lst, int nl, V3<double> a, V3<double> b,
-                                               V3<double> c, double sqrtT, uint64_t mk[2])
+// the FP32 box face_list_mask compares the list's boxes with
+__device__ __forceinline__ void face_mask_box(V3<double> a, V3<double> b, V3<double> c,
+                                              double sqrtT, float lof[3], float hif[3])
 {
     double lo[3] = {a.x, a.y, a.z}, hi[3] = {a.x, a.y, a.z};
     const double q[6] = {b.x, b.y, b.z, c.x, c.y, c.z};
@@ -631,20 +631,48 @@ __device__ __forceinline__ void face_list_mask(L lst, int nl, V3<double> a, V3<d
 #pragma unroll
     for (int k = 0; k < 3; ++k) amax = fmax(amax, fmax(fabs(lo[k]), fabs(hi[k])));
     const double pad = 1.0001 * sqrtT + 1e-9 * amax;
-    float lof[3], hif[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         lof[k] = f32_down(lo[k] - pad);
         hif[k] = f32_up(hi[k] + pad);
     }
-    mk[0] = 0ull;
-    mk[1] = 0ull;
-    for (int e = 0; e < nl && e < 128; ++e) {
-        const BvhNode32 r = from_lds(lst + e);
-        const bool meet = r.lo[0] <= hif[0] && r.hi[0] >= lof[0] && r.lo[1] <= hif[1] &&
-                          r.hi[1] >= lof[1] && r.lo[2] <= hif[2] && r.hi[2] >= lof[2];
-        if (meet) mk[e >> 6] |= 1ull << (e & 63);
+}
+
+__device__ __forceinline__ bool box_meets(const BvhNode32& r, const float lof[3], const float hif[3])
+{
+    return r.lo[0] <= hif[0] && r.hi[0] >= lof[0] && r.lo[1] <= hif[1] && r.hi[1] >= lof[1] &&
+           r.lo[2] <= hif[2] && r.hi[2] >= lof[2];
+}
+
+template <class L>
+__device__ __forceinline__ void face_list_mask(L lst, int nl, V3<double> a, V3<double> b,
+                                               V3<double> c, double sqrtT, uint64_t mk[2])
+{
+    float lof[3], hif[3];
+    face_mask_box(a, b, c, sqrtT, lof, hif);
+    // the two words in registers (an array indexed at run time would go to
+    // scratch memory), eight records read before their tests
+    const int n = nl < 128 ? nl : 128;
+    uint64_t w[2] = {0ull, 0ull};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e1 = n - 64 * h < 64 ? n - 64 * h : 64;
+        uint64_t m = 0ull;
+        for (int e0 = 0; e0 < e1; e0 += 8) {
+            BvhNode32 r[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u < e1) r[u] = from_lds(lst + 64 * h + e0 + u);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool meet = e0 + u < e1 && box_meets(r[u], lof, hif);
+                m |= (uint64_t)meet << (e0 + u);
+            }
+        }
+        w[h] = m;
     }
+    mk[0] = w[0];
+    mk[1] = w[1];
 }
 
 // in_envelope_list over the list positions set in mask (two 64-bit words;
