@@ -2,7 +2,8 @@
 // qualifiers removed, checking the logic the kernels run:
 //   1. SampleGen against the oracle's sampleTriangle (bit-exact samples and
 //      counts), SampleGen::skip against stepping one sample at a time, and
-//      the stored-record generator SampleGenC against both;
+//      the stored-record generator SampleGenC and the segmented layout
+//      (store_seg, seg_row, SegFace) against both;
 //   2. in_envelope -- global walker and LDS walker over the top levels, FP64
 //      and FP32 boxes -- against the brute-force minimum over all facets, on
 //      trees of 1..50k facets (the tree built by the library's own host code);
@@ -42,7 +43,7 @@ static void check_sampler(std::mt19937_64& rng)
 {
     std::uniform_real_distribution<double> U(-1, 1);
     std::vector<double> ps(3 * 100000);
-    long samples = 0, positions = 0;
+    long samples = 0, positions = 0, seg_faces = 0, seg_rows = 0;
     for (int it = 0; it < 6000; ++it) {
         double t[9];
         const double s = (it % 3 == 0) ? 0.01 : (it % 3 == 1 ? 0.003 : 0.02);
@@ -87,6 +88,52 @@ static void check_sampler(std::mt19937_64& rng)
             gcnt.load(rec, d);
             CHECK(gcnt.skip(INT64_MAX) == (int64_t)seq.size(), "stored skip: count of triangle %d\n", it);
         }
+        {
+            // the segmented layout, filled as the smoothing kernel fills it:
+            // every row on its own (seg_row), then the rows before the first
+            // that ends and their samples
+            double srec[kSegRec];
+            int hdr[kSegHdr];
+            for (double& v : srec) v = std::nan("");
+            cnt.store_seg(srec, hdr, 1);  // row-table entry 0 unused
+            std::vector<double> rt(kSegRow * (size_t)(hdr[kShRows] + 1), std::nan(""));
+            for (int m = 1; m <= hdr[kShRows]; ++m) {
+                V3<double> st = mk3(0.0, 0.0, 0.0);
+                const int c = seg_row(srec, d, m, st);
+                double* e = &rt[kSegRow * (size_t)m];
+                e[0] = st.x;
+                e[1] = st.y;
+                e[2] = st.z;
+                e[3] = c;
+            }
+            int rows = 0;
+            int64_t sum = 0;
+            while (rows < hdr[kShRows] && rt[kSegRow * (size_t)(rows + 1) + 3] > 0)
+                sum += (int64_t)rt[kSegRow * (size_t)(++rows) + 3];
+            hdr[kShRows] = rows;
+            hdr[kShSum] = (int)sum;
+            SegFace F;
+            F.load(hdr);
+            CHECK(F.total(sum) == (int64_t)seq.size(), "segments: triangle %d counts %lld of %zu\n", it,
+                  (long long)F.total(sum), seq.size());
+            int64_t j = 0;
+            for (int s = 0; s < F.nseg(); ++s)
+                for (int k = 0; k < F.count(rt.data(), s); ++k, ++j) {
+                    const V3<double> p = F.sample(srec, rt.data(), s, k, d);
+                    CHECK(j < (int64_t)seq.size() && !memcmp(&p, &seq[j], sizeof p),
+                          "segments: triangle %d sample %lld (segment %d, %d) differs\n", it, (long long)j, s,
+                          k);
+                }
+            CHECK(j == (int64_t)seq.size(), "segments: triangle %d walks %lld samples\n", it, (long long)j);
+            for (size_t o = 0; o < seq.size(); o += 1 + seq.size() / 11) {
+                int s, k;
+                F.seek(rt.data(), (int64_t)o, s, k);
+                const V3<double> p = F.sample(srec, rt.data(), s, k, d);
+                CHECK(!memcmp(&p, &seq[o], sizeof p), "segments: triangle %d seek %zu\n", it, o);
+            }
+            seg_faces++;
+            seg_rows += rows;
+        }
         CHECK(cnt.skip(INT64_MAX) == (int64_t)seq.size(), "skip: count of triangle %d\n", it);
         const size_t step = it % 5 == 0 ? 1 : 1 + seq.size() / 7;
         for (size_t s0 = 0; s0 <= seq.size(); s0 += step) {
@@ -115,7 +162,8 @@ static void check_sampler(std::mt19937_64& rng)
                 CHECK(!morec, "stored skip: triangle %d past the end\n", it);
         }
     }
-    printf("sampler: 6000 triangles, %ld samples, %ld skip positions\n", samples, positions);
+    printf("sampler: 6000 triangles, %ld samples, %ld skip positions; segments: %ld triangles, %ld rows\n",
+           samples, positions, seg_faces, seg_rows);
 }
 
 struct Tree {

@@ -70,25 +70,43 @@ struct SmoothArgs {
 // (kQ doubles per lane) while it solves a problem, and while it tests the
 // one-ring faces its list of nearby surface facets -- per entry a 32-byte
 // record, two 4-byte work-list slots (collect_facets) and the facet's FP64
-// triangle (72 bytes) -- followed by the sampler records (SampleGen::store)
-// of the faces being sampled, TILE / 4 faces at a time.  The reductions and
-// the face test never overlap in time.
-constexpr int kTilePoolBytes = 57344;
+// triangle (72 bytes) -- then per face being sampled (TILE / 4 at a time) its
+// list mask, then the faces' sampler state: on a 64-lane tile, when the
+// faces' rows fit, their segment records and headers and a row table
+// (SampleGen::store_seg, seg_row), else -- and on narrower tiles -- their
+// sampler records (SampleGen::store).  The reductions and the face test never
+// overlap in time.  A block takes 80 KiB with the tree cache, two blocks per
+// CU.
+constexpr int kTilePoolBytes = 65536;
 constexpr int kListEntryBytes = 32 + 8 + 72;
 constexpr int kFaceRecBytes = kFaceRec * 8;
 constexpr int kFaceMaskBytes = 16;  // two 64-bit words: list positions near the face
+constexpr int kFaceSegBytes = kSegRec * 8 + kSegHdr * 4;
 template <int TILE>
 constexpr int tile_bytes() { return kTilePoolBytes / (kBlock / TILE); }
 template <int TILE>
 constexpr int face_chunk() { return TILE / 4; }
+// row-table entries of a tile (0: the tile samples with SampleGenC)
+template <int TILE>
+constexpr int seg_rows() { return TILE == 64 ? 94 : 0; }
+template <int TILE>
+constexpr int face_area_bytes()
+{
+    constexpr int rec = face_chunk<TILE>() * kFaceRecBytes;
+    constexpr int seg = face_chunk<TILE>() * kFaceSegBytes + seg_rows<TILE>() * kSegRow * 8;
+    return seg_rows<TILE>() > 0 && seg > rec ? seg : rec;
+}
 template <int TILE>
 constexpr int ring_cap()
 {
-    return (tile_bytes<TILE>() - face_chunk<TILE>() * (kFaceRecBytes + kFaceMaskBytes)) / kListEntryBytes;
+    return (tile_bytes<TILE>() - face_chunk<TILE>() * kFaceMaskBytes - face_area_bytes<TILE>()) / kListEntryBytes;
 }
 static_assert(tile_bytes<16>() >= 16 * kQ * 8 && tile_bytes<64>() >= 64 * kQ * 8, "reduction rows");
 static_assert(ring_cap<64>() == 92 && ring_cap<64>() <= 128 && ring_cap<16>() >= 16, "list capacity");
 static_assert((ring_cap<16>() * kListEntryBytes) % 16 == 0, "record alignment");
+static_assert(kSmoothTreeCacheBytes + kTilePoolBytes <= 81920, "two blocks per CU");
+
+typedef __attribute__((address_space(3))) int LdsI32;
 
 template <int TILE>
 struct TileCtx {
@@ -100,9 +118,44 @@ struct TileCtx {
     LdsNode32W* lst;   // the same slice: facet list (ring_cap entries),
     LdsU32* fr;        // work lists (2 ring_cap entries),
     LdsF64* tri;       // the listed facets' triangles (9 doubles each)
-    LdsF64* frec;      // face_chunk sampler records (kFaceRec doubles each)
-    LdsU64* fmask;     // and per face the list positions near it (2 words)
+    LdsU64* fmask;     // per face the list positions near it (2 words)
+    LdsF64* frec;      // then face_chunk sampler records (kFaceRec doubles each)
+                       // or the segmented layout (SegArea)
 };
+
+// The segmented layout over a tile's sampler area: face_chunk segment
+// records (kSegRec doubles each), their headers (kSegHdr ints each) and
+// seg_rows row-table entries (kSegRow doubles each)
+template <int TILE>
+struct SegArea {
+    LdsF64* rec;
+    LdsI32* hdr;
+    LdsF64* row;
+    __device__ __forceinline__ explicit SegArea(LdsF64* area)
+        : rec(area),
+          hdr((LdsI32*)(area + face_chunk<TILE>() * kSegRec)),
+          row(area + face_chunk<TILE>() * kFaceSegBytes / 8)
+    {
+    }
+};
+static_assert(kFaceSegBytes % 8 == 0, "row table alignment");
+
+template <int TILE>
+__device__ __forceinline__ void tile_setup(TileCtx<TILE>& c, unsigned char* pool)
+{
+    c.tl = threadIdx.x % TILE;
+    c.tbase = threadIdx.x - c.tl;
+    c.wbase = (threadIdx.x % 64) - c.tl;
+    constexpr int cap = ring_cap<TILE>();
+    constexpr int FC = face_chunk<TILE>();
+    unsigned char* slice = pool + (threadIdx.x / TILE) * tile_bytes<TILE>();
+    c.red = (double*)slice;
+    c.lst = (LdsNode32W*)slice;
+    c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
+    c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
+    c.fmask = (LdsU64*)(slice + cap * kListEntryBytes);
+    c.frec = (LdsF64*)(slice + cap * kListEntryBytes + FC * kFaceMaskBytes);
+}
 
 // acc + v[0] + v[1] + ... + v[cnt - 1] in that order (v[r] = base[r * stride]
 // in LDS): the reads go out four at a time before their additions, which
@@ -535,17 +588,23 @@ __device__ __forceinline__ void ring_face(const SmoothArgs& A, int64_t k, V3<dou
 // (src/tetwild/VertexSmoother.cpp:413-429): true when some face that is not
 // collinear has a sample with no facet within eps (isFaceOutEnvelop for each
 // face).  The samples of all faces are spread over the tile, TILE / 4 faces
-// at a time: lane k stores face k's sampler record in LDS and counts its
-// samples (SampleGen::store, skip), a scan over the tile gives the faces'
-// offsets, and lane r walks samples [r K, r K + K) (SampleGenC on the records)
-// of the concatenated sequence (K = ceil(total / TILE)), moving on to the
-// next face with samples when one ends and carrying its facet hint; a ballot
-// after each step stops the tile at the first sample outside.  A sample is
-// tested against the tile's LDS list of the surface facets whose boxes meet
-// the faces' padded box (collect_facets; the tree walk when the list does not
-// fit), starting from the facet that held the lane's previous sample or, at
-// first, from `hint` (a facet near pf, or -1).  The decision depends neither
-// on the list order, the hints, nor on which lane tests which sample.
+// at a time.  Lane k sets up face k's sampler (SampleGen::init); on a 64-lane
+// tile whose faces' rows fit its row table, lane k stores the face's segment
+// record and header, every lane then fills rows of the table (one row per
+// lane, seg_row: a row needs nothing from the rows before it) and lane k sums
+// its face's rows up to the first that ends; otherwise lane k stores the
+// face's sampler record and counts its samples (SampleGen::store, skip).  A
+// scan over the tile gives the faces' offsets, and lane r walks samples
+// [r K, r K + K) of the concatenated sequence (K = ceil(total / TILE)) --
+// segment by segment (SegFace), or with SampleGenC on the records -- moving
+// on to the next face with samples when one ends and carrying its facet hint;
+// a ballot after each step stops the tile at the first sample outside.  A
+// sample is tested against the tile's LDS list of the surface facets whose
+// boxes meet the faces' padded box (collect_facets; the tree walk when the
+// list does not fit), starting from the facet that held the lane's previous
+// sample or, at first, from `hint` (a facet near pf, or -1).  The decision
+// depends neither on the list order, the hints, nor on which lane tests which
+// sample, and both samplers give SampleGen's samples bit for bit.
 template <int TILE>
 __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<double> pf,
                                int64_t t0, int64_t t1, int hint)
@@ -610,11 +669,16 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
         }
     }
     constexpr int FC = face_chunk<TILE>();
+    constexpr int RC = seg_rows<TILE>();
+    const SegArea<TILE> sa(c.frec);
+    const unsigned long long tm = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
     for (int64_t base = t0; base < t1; base += FC) {
         const int nfc = t1 - base < FC ? (int)(t1 - base) : FC;
         int64_t cnt = 0;
-        bool huge = false;
+        bool huge = false, sampled = false;
+        int rows = 0;  // the face's row count M (0 without rows)
         float flo[3] = {0.f, 0.f, 0.f}, fhi[3] = {0.f, 0.f, 0.f};  // the face's mask box
+        SampleGen g;
         if (c.tl < nfc) {
             double t9[9];
             ring_face(A, base + c.tl, pf, t9);
@@ -624,15 +688,34 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                 if (!(__builtin_sqrt(l) / A.d <= kMaxEdgeSamples)) {
                     huge = true;  // reported out without sampling
                 } else {
-                    SampleGen g;
                     g.init(a, b, cc, A.d);
-                    g.store(c.frec + kFaceRec * c.tl);
-                    cnt = g.skip(INT64_MAX);
+                    sampled = true;
+                    rows = g.phase != 9 && g.M >= 1 ? g.M : 0;
                     face_mask_box(a, b, cc, A.sqrtT, flo, fhi);
                 }
             }
         }
         if (tile_any(c, huge)) return true;
+        // the faces' first row-table entries (an exclusive scan of the rows)
+        int row0 = 0;
+        bool seg = false;
+        if (RC > 0) {
+            int incl = rows;
+            for (int off = 1; off < TILE; off <<= 1) {
+                const int o = __shfl_up(incl, off, TILE);
+                if (c.tl >= off) incl += o;
+            }
+            row0 = incl - rows;
+            seg = __shfl(incl, TILE - 1, TILE) <= RC;  // the same on every lane
+        }
+        if (sampled) {
+            if (seg) {
+                g.store_seg(sa.rec + kSegRec * c.tl, sa.hdr + kSegHdr * c.tl, row0);
+            } else {
+                g.store(c.frec + kFaceRec * c.tl);
+                cnt = g.skip(INT64_MAX);
+            }
+        }
         if (nl >= 0) {
             // face_list_mask for every face of the chunk at once: lane j tests
             // list entries j, j + TILE, ... against each face's box (from the
@@ -644,7 +727,6 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
 #pragma unroll
             for (int i = 0; i < per_lane; ++i)
                 if (c.tl + TILE * i < nl) er[i] = from_lds(c.lst + c.tl + TILE * i);
-            const unsigned long long tmk = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
             uint64_t my0 = 0ull, my1 = 0ull;
             for (int k = 0; k < nfc; ++k) {
                 float kl[3], kh[3];
@@ -658,7 +740,7 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                 for (int i = 0; i < per_lane; ++i) {
                     const int e = c.tl + TILE * i;
                     const bool meet = e < nl && box_meets(er[i], kl, kh);
-                    const uint64_t bits = (__ballot(meet) >> c.wbase) & tmk;
+                    const uint64_t bits = (__ballot(meet) >> c.wbase) & tm;
                     const int pos = TILE * i;  // a multiple of TILE: bits stay in one word
                     if (pos < 64) w0 |= bits << pos;
                     else w1 |= bits << (pos - 64);
@@ -673,9 +755,63 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                 c.fmask[2 * c.tl + 1] = my1;
             }
         }
-        wave_sync();  // the records and masks, read below by every lane of the tile
+        wave_sync();  // the records, headers and masks, read below by every lane of the tile
+        if (seg) {
+            // the rows of the chunk's faces, one lane per row: entry i is row
+            // i - row0 + 1 of the face with row0 <= i < row0 + rows
+            // (the faces' starts are read with every lane of the tile active)
+            constexpr int IPL = RC > 0 ? (RC + TILE - 1) / TILE : 1;
+            const int rtot = __shfl(row0 + rows, TILE - 1, TILE);
+            int fi[IPL], ri[IPL];
+#pragma unroll
+            for (int q = 0; q < IPL; ++q) fi[q] = ri[q] = 0;
+            for (int k = 0; k < nfc; ++k) {
+                const int rk = __shfl(row0, k, TILE), mk = __shfl(rows, k, TILE);
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+                    if (mk > 0 && rk <= c.tl + TILE * q) {
+                        fi[q] = k;
+                        ri[q] = rk;
+                    }
+            }
+#pragma unroll 1
+            for (int q = 0; q < IPL; ++q) {
+                const int i = c.tl + TILE * q;
+                if (i >= rtot) continue;
+                int f = fi[0], r0 = ri[0];
+#pragma unroll
+                for (int u = 1; u < IPL; ++u)
+                    if (q == u) {
+                        f = fi[u];
+                        r0 = ri[u];
+                    }
+                V3<double> rs = mk3(0.0, 0.0, 0.0);
+                const int rc = seg_row(sa.rec + kSegRec * f, A.d, i - r0 + 1, rs);
+                LdsF64* e = sa.row + kSegRow * i;
+                e[0] = rs.x;
+                e[1] = rs.y;
+                e[2] = rs.z;
+                e[3] = (double)rc;
+            }
+            wave_sync();  // the row table
+            if (sampled) {
+                // the rows before the first that ends, and their samples
+                int r = 0;
+                int64_t sum = 0;
+                while (r < rows) {
+                    const int rc = (int)sa.row[kSegRow * (row0 + r) + 3];
+                    if (rc == 0) break;
+                    sum += rc;
+                    ++r;
+                }
+                sa.hdr[kSegHdr * c.tl + kShRows] = r;
+                SegFace F;
+                F.load(sa.hdr + kSegHdr * c.tl);
+                cnt = F.total(sum);
+            }
+            wave_sync();  // the headers' row counts
+        }
         // faces of this chunk with samples (bit k: face base + k)
-        const unsigned long long tm = TILE == 64 ? ~0ull : ((1ull << (TILE & 63)) - 1ull);
         const unsigned long long has = (__ballot(cnt > 0) >> c.wbase) & tm;
         if (has == 0ull) continue;
         // inclusive scan of the counts over the tile
@@ -697,25 +833,55 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                 fstart = ik - ck;
             }
         }
-        SampleGenC<LdsF64*> g;
+        // the lane's place: (segment s, sample k, the segment's count sc) of
+        // face f, or the record sampler
+        SampleGenC<LdsF64*> gc;
+        SegFace F;
+        int ss = 0, sk = 0, sc = 0;
         if (f >= 0) {
-            g.load(c.frec + kFaceRec * f, A.d);
-            g.skip(first - fstart);
+            if (seg) {
+                F.load(sa.hdr + kSegHdr * f);
+                F.seek(sa.row, first - fstart, ss, sk);
+                sc = F.count(sa.row, ss);
+            } else {
+                gc.load(c.frec + kFaceRec * f, A.d);
+                gc.skip(first - fstart);
+            }
         }
         for (int64_t j = 0; j < K; ++j) {
             bool out = false;
             if (f >= 0 && first + j < total) {
                 V3<double> smp;
-                bool got = g.next(smp);
-                while (!got) {
-                    // this face is done: the next face of the chunk with samples
-                    // (one exists, since first + j < total and the counts are
-                    // the generator's own)
-                    const unsigned long long later = f >= 63 ? 0ull : has & ~((2ull << f) - 1ull);
-                    if (later == 0ull) break;
-                    f = __ffsll((long long)later) - 1;
-                    g.load(c.frec + kFaceRec * f, A.d);
-                    got = g.next(smp);
+                bool got = true;
+                if (seg) {
+                    // past the segments (and faces) with no samples left; one
+                    // exists, since first + j < total and the counts are the
+                    // segments' own
+                    while (sk >= sc) {
+                        sk = 0;
+                        if (++ss >= F.nseg()) {
+                            const unsigned long long later = f >= 63 ? 0ull : has & ~((2ull << f) - 1ull);
+                            if (later == 0ull) {
+                                got = false;
+                                break;
+                            }
+                            f = __ffsll((long long)later) - 1;
+                            F.load(sa.hdr + kSegHdr * f);
+                            ss = 0;
+                        }
+                        sc = F.count(sa.row, ss);
+                    }
+                    if (got) smp = F.sample(sa.rec + kSegRec * f, sa.row, ss, sk++, A.d);
+                } else {
+                    got = gc.next(smp);
+                    while (!got) {
+                        // this face is done: the next face of the chunk with samples
+                        const unsigned long long later = f >= 63 ? 0ull : has & ~((2ull << f) - 1ull);
+                        if (later == 0ull) break;
+                        f = __ffsll((long long)later) - 1;
+                        gc.load(c.frec + kFaceRec * f, A.d);
+                        got = gc.next(smp);
+                    }
                 }
                 if (got)
                     out = nl >= 0 ? !in_envelope_masked(c.lst, c.tri, nl, c.fmask + 2 * f, smp,
@@ -841,19 +1007,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     TileCtx<TILE> c;
     // every thread of the block, before the loop (the copy synchronises)
     c.sfw = A.has_sf ? cache_tree_top<false, kSmoothTreeCacheBytes>(A.sf, tcache) : walk(A.sf);
-    c.tl = threadIdx.x % TILE;
-    c.tbase = threadIdx.x - c.tl;
-    c.wbase = (threadIdx.x % 64) - c.tl;
-    {
-        constexpr int cap = ring_cap<TILE>();
-        unsigned char* slice = pool + (threadIdx.x / TILE) * tile_bytes<TILE>();
-        c.red = (double*)slice;
-        c.lst = (LdsNode32W*)slice;
-        c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
-        c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
-        c.frec = (LdsF64*)(slice + cap * kListEntryBytes);
-        c.fmask = (LdsU64*)(slice + cap * kListEntryBytes + face_chunk<TILE>() * kFaceRecBytes);
-    }
+    tile_setup<TILE>(c, pool);
     for (;;) {
         unsigned start = 0;
         if (c.tl == 0) start = atomicAdd(A.counter, A.chunk);
@@ -876,22 +1030,6 @@ __device__ __forceinline__ bool surface_kind(const SmoothArgs& A, int64_t p)
 {
     const int k = A.kind ? A.kind[p] : 0;
     return k == 1 || k == 2;
-}
-
-template <int TILE>
-__device__ __forceinline__ void tile_setup(TileCtx<TILE>& c, unsigned char* pool)
-{
-    c.tl = threadIdx.x % TILE;
-    c.tbase = threadIdx.x - c.tl;
-    c.wbase = (threadIdx.x % 64) - c.tl;
-    constexpr int cap = ring_cap<TILE>();
-    unsigned char* slice = pool + (threadIdx.x / TILE) * tile_bytes<TILE>();
-    c.red = (double*)slice;
-    c.lst = (LdsNode32W*)slice;
-    c.fr = (LdsU32*)(slice + cap * sizeof(BvhNode32));
-    c.tri = (LdsF64*)(slice + cap * (sizeof(BvhNode32) + 2 * sizeof(uint32_t)));
-    c.frec = (LdsF64*)(slice + cap * kListEntryBytes);
-    c.fmask = (LdsU64*)(slice + cap * kListEntryBytes + face_chunk<TILE>() * kFaceRecBytes);
 }
 
 template <int TILE, bool MIXED, int WANT>  // WANT 1: surface kinds, 0: the others

@@ -413,6 +413,20 @@ struct SampleGen {
     template <class P>
     __device__ __forceinline__ void store(P rec) const;
 
+    // Writes the triangle's segment record and header after init() (the
+    // segmented layout below); row0: the first row-table entry of its rows.
+    template <class P, class I>
+    __device__ __forceinline__ void store_seg(P rec, I hdr, int row0) const;
+
+    // the samples of an edge of L sampling distances past its first vertex
+    // (start_edge12 / start_edge20: k = 1 .. N)
+    __device__ __forceinline__ static int edge_count(double L)
+    {
+        if (!(L > 1)) return 0;
+        if (L == (int)L) L -= 1;
+        return (int)L;
+    }
+
     __device__ __forceinline__ bool next(V3<double>& p)
     {
         for (;;) {
@@ -752,6 +766,168 @@ struct SampleGenC {
                 return false;
             }
         }
+    }
+};
+
+// ---- sampleTriangle as segments ------------------------------------------
+// SampleGen's sequence laid out for a group of lanes that sample several
+// triangles at once.  A triangle's samples are runs along lines: the longest
+// edge v0 -> v1, the vertex v1, the rows, v2, the edges v1 -> v2 and
+// v2 -> v0, each sample the run's start plus a multiple of its step with the
+// expression SampleGen uses for that run.  A triangle keeps its rotated
+// vertices and unit directions (kSeg* doubles) and a header of counts (kSh*
+// ints); its rows sit in a row table, one entry (the row's start, its sample
+// count) per row, each filled on its own by seg_row (a row does not depend on
+// the rows before it).  Segment s of a triangle with R rows:
+//   short triangle (N <= 1): s = 0, 1, 2 its vertices;
+//   no rows (M < 1):         0 the edge v0 -> v1 (C01 samples), 1 v1, 2 v2;
+//   otherwise:               0 the edge, 1 v1, 2 .. R + 1 the rows, R + 2 v2,
+//                            R + 3 the edge v1 -> v2 (C12), R + 4 v2 -> v0 (C20).
+// R is the rows before the first that SampleGen::row_header ends at (the
+// header's kShRows, M until the table is read).
+enum : int {
+    kSegV0 = 0, kSegV1 = 3, kSegV2 = 6,       // the rotated triangle
+    kSegN01 = 9, kSegN12 = 12, kSegN20 = 15,  // normalize(v1 - v0), (v2 - v1), (v0 - v2)
+    kSegRec = 18
+};
+enum : int {
+    kShKind = 0, kShC01 = 1, kShRows = 2, kShRow0 = 3, kShC12 = 4, kShC20 = 5, kShM = 6,
+    kShSum = 7,  // the rows' samples
+    kSegHdr = 8
+};
+enum : int { kSegShort = 0, kSegNoRows = 1, kSegRows = 2 };
+constexpr int kSegRow = 4;  // doubles per row-table entry: the start, the count
+
+template <class P>
+__device__ __forceinline__ V3<double> seg_at(P r, int o)
+{
+    return mk3(r[o], r[o + 1], r[o + 2]);
+}
+
+template <class P, class I>
+__device__ __forceinline__ void SampleGen::store_seg(P r, I h, int row0) const
+{
+    const V3<double> vs[3] = {v0, v1, v2};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        r[kSegV0 + 3 * j] = vs[j].x;
+        r[kSegV0 + 3 * j + 1] = vs[j].y;
+        r[kSegV0 + 3 * j + 2] = vs[j].z;
+    }
+    h[kShRow0] = row0;
+    h[kShSum] = 0;
+    if (phase == 9) {
+        h[kShKind] = kSegShort;
+        h[kShC01] = h[kShRows] = h[kShC12] = h[kShC20] = h[kShM] = 0;
+        return;
+    }
+    const V3<double> ds[3] = {normalize(v1 - v0), normalize(v2 - v1), normalize(v0 - v2)};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        r[kSegN01 + 3 * j] = ds[j].x;
+        r[kSegN01 + 3 * j + 1] = ds[j].y;
+        r[kSegN01 + 3 * j + 2] = ds[j].z;
+    }
+    h[kShKind] = M < 1 ? kSegNoRows : kSegRows;
+    h[kShC01] = (int)((int64_t)N + 1);
+    h[kShM] = M;
+    h[kShRows] = M < 1 ? 0 : M;
+    h[kShC12] = edge_count(__builtin_sqrt(len2(v1 - v2)) / d);
+    h[kShC20] = edge_count(__builtin_sqrt(len2(v2 - v0)) / d);
+}
+
+// Row m (1 <= m <= M) of the triangle whose segment record is r: writes its
+// start and returns its count (N1 + 1), or returns 0 when the rows end at m.
+template <class P>
+__device__ __forceinline__ int seg_row(P r, double d, int m, V3<double>& start)
+{
+    SampleGen g;
+    g.v0 = seg_at(r, kSegV0);
+    g.v1 = seg_at(r, kSegV1);
+    g.v2 = seg_at(r, kSegV2);
+    g.d = d;
+    g.m = m;
+    g.row_header(g.row_geom());
+    if (g.phase != 3) return 0;
+    start = g.rowv;
+    return g.N1 + 1;
+}
+
+// A triangle's header in registers, and its segments over the record r and
+// the row table rt (kSegRow doubles per entry).
+struct SegFace {
+    int kind, c01, rows, row0, c12, c20;
+
+    template <class I>
+    __device__ __forceinline__ void load(I h)
+    {
+        kind = h[kShKind];
+        c01 = h[kShC01];
+        rows = h[kShRows];
+        row0 = h[kShRow0];
+        c12 = h[kShC12];
+        c20 = h[kShC20];
+    }
+    __device__ __forceinline__ int nseg() const { return kind == kSegRows ? rows + 5 : 3; }
+    // the samples of the triangle, given its rows' samples
+    __device__ __forceinline__ int64_t total(int64_t row_sum) const
+    {
+        return kind == kSegShort ? 3 : kind == kSegNoRows ? (int64_t)c01 + 2 : (int64_t)c01 + 2 + row_sum + c12 + c20;
+    }
+    template <class RP>
+    __device__ __forceinline__ int count(RP rt, int s) const
+    {
+        if (kind == kSegShort || s == 1) return 1;
+        if (s == 0) return c01;
+        if (kind == kSegNoRows) return 1;
+        const int j = s - 2;
+        if (j < rows) return (int)rt[kSegRow * (row0 + j) + 3];
+        return j == rows ? 1 : (j == rows + 1 ? c12 : c20);
+    }
+    // sample k of segment s: a vertex, start + (dir d) k on an edge (k from 1
+    // on the edges from v1 and v2), start + (dir k) d on a row -- SampleGen's
+    // expressions
+    template <class P, class RP>
+    __device__ __forceinline__ V3<double> sample(P r, RP rt, int s, int k, double d) const
+    {
+        int vo = -1, bo = kSegV0, dofs = kSegN01, kk = k;
+        bool row = false;
+        if (kind == kSegShort) {
+            vo = kSegV0 + 3 * s;
+        } else if (s == 1) {
+            vo = kSegV1;
+        } else if (s != 0) {
+            const int j = s - 2;
+            if (kind == kSegNoRows || j == rows) {
+                vo = kSegV2;
+            } else if (j < rows) {
+                row = true;
+            } else {
+                const bool e12 = j == rows + 1;
+                bo = e12 ? kSegV1 : kSegV2;
+                dofs = e12 ? kSegN12 : kSegN20;
+                kk = k + 1;
+            }
+        }
+        if (vo >= 0) return seg_at(r, vo);
+        const V3<double> dir = seg_at(r, dofs);
+        if (row) return seg_at(rt, kSegRow * (row0 + s - 2)) + mulv(mulv(dir, (double)kk), d);
+        return seg_at(r, bo) + mulv(mulv(dir, d), (double)kk);
+    }
+    // (s, k) -> the sample at offset o of the triangle (o below its total;
+    // past it, a place in the last segment at or beyond its end)
+    template <class RP>
+    __device__ __forceinline__ void seek(RP rt, int64_t o, int& s, int& k) const
+    {
+        const int ns = nseg();
+        s = 0;
+        while (s + 1 < ns) {
+            const int c = count(rt, s);
+            if (o < c) break;
+            o -= c;
+            ++s;
+        }
+        k = o < INT32_MAX ? (int)o : INT32_MAX;
     }
 };
 
