@@ -14,7 +14,10 @@
 //      the query box, in_envelope_list on its lists against brute force, and
 //      in_envelope_masked with face_list_mask against in_envelope_list;
 //   5. nearest_facet_group (one-lane group) against nearest_facet, bit for
-//      bit, with queries above shared grid edges and vertices (ties).
+//      bit, with queries above shared grid edges and vertices (ties);
+//   6. point_tri_d2 (squared distance and closest point) against the
+//      oracle's orc_point_tri_d2, bit for bit, with points in every region
+//      of random, thin and degenerate triangles, on vertices and edges.
 // Exit status 0 when every check passes.
 #include "twg_bvh.hpp"
 
@@ -28,6 +31,8 @@
 
 extern "C" int64_t orc_sample_triangle(const double* v0, const double* v1, const double* v2,
                                        double d, double* ps, int64_t cap);
+extern "C" double orc_point_tri_d2(const double* p, const double* a, const double* b, const double* c,
+                                   double* cp);
 using namespace twg;
 
 static long g_bad = 0;
@@ -410,9 +415,11 @@ static void check_lists(std::mt19937_64& rng)
                     int hm = -1, hu = -1;
                     const bool nbr = q % 2 == 1;  // these lists carry neighbour positions
                     for_each_sample(fv[0], fv[1], fv[2], 0.004, [&](V3<double> p) {
+                        // (the queries without neighbours take the four-at-a-time box reads)
                         const bool gm =
                             nbr ? in_envelope_masked<true>(lst.data(), tri.data(), n, mk, p, eps2, sqrtT, hm)
-                                : in_envelope_masked(lst.data(), tri.data(), n, mk, p, eps2, sqrtT, hm);
+                                : in_envelope_masked<false, true>(lst.data(), tri.data(), n, mk, p, eps2, sqrtT,
+                                                                  hm);
                         const bool gu =
                             nbr ? in_envelope_list<true>(lst.data(), tri.data(), n, p, eps2, sqrtT, hu)
                                 : in_envelope_list(lst.data(), tri.data(), n, p, eps2, sqrtT, hu);
@@ -506,6 +513,42 @@ static void check_nearest(std::mt19937_64& rng)
     printf("nearest: %ld queries (%ld not at the hint facet) against the serial walk\n", queries, off_hint);
 }
 
+static void check_point_tri(std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(-1, 1);
+    long n = 0, edge_pts = 0;
+    for (int it = 0; it < 400000; ++it) {
+        double t[9], q[3];
+        for (double& v : t) v = U(rng);
+        const int kind = it % 10;
+        if (kind == 7)  // thin: the third vertex near the first edge
+            for (int k = 0; k < 3; ++k) t[6 + k] = t[k] + 0.37 * (t[3 + k] - t[k]) + 1e-7 * U(rng);
+        if (kind == 8)  // collinear or a repeated vertex (the degenerate branch)
+            for (int k = 0; k < 3; ++k) t[6 + k] = (it % 20 < 10) ? t[k] + 2.0 * (t[3 + k] - t[k]) : t[3 + k];
+        const double spread = kind == 9 ? 0.01 : (kind < 3 ? 3.0 : 1.0);
+        for (double& v : q) v = spread * U(rng);
+        if (kind == 6) {  // on a vertex, or on an edge
+            const int a = it % 3, b = (it + 1) % 3;
+            const double w = (it % 6 < 3) ? 0.0 : 0.5 * (1 + U(rng));
+            for (int k = 0; k < 3; ++k) q[k] = t[3 * a + k] + w * (t[3 * b + k] - t[3 * a + k]);
+            edge_pts++;
+        }
+        double cpo[3];
+        const double d2o = orc_point_tri_d2(q, t, t + 3, t + 6, cpo);
+        V3<double> cp;
+        const double d2 = point_tri_d2<double, true>(mk3(q[0], q[1], q[2]), mk3(t[0], t[1], t[2]),
+                                                     mk3(t[3], t[4], t[5]), mk3(t[6], t[7], t[8]), cp);
+        V3<double> cp2;
+        const double d2b = point_tri_d2<double, false>(mk3(q[0], q[1], q[2]), mk3(t[0], t[1], t[2]),
+                                                       mk3(t[3], t[4], t[5]), mk3(t[6], t[7], t[8]), cp2);
+        CHECK(!memcmp(&d2, &d2o, 8) && !memcmp(&d2b, &d2o, 8), "point_tri_d2: case %d: %.17g vs %.17g\n", it,
+              d2, d2o);
+        CHECK(cp.x == cpo[0] && cp.y == cpo[1] && cp.z == cpo[2], "point_tri_d2: case %d closest point\n", it);
+        n++;
+    }
+    printf("point_tri_d2: %ld cases (%ld on vertices and edges) vs orc_point_tri_d2\n", n, edge_pts);
+}
+
 int main()
 {
     std::mt19937_64 rng(20240601);
@@ -514,6 +557,7 @@ int main()
     check_group(rng);
     check_lists(rng);
     check_nearest(rng);
+    check_point_tri(rng);
     printf("%ld failures\n", g_bad);
     return g_bad != 0;
 }

@@ -884,8 +884,8 @@ __device__ bool ring_faces_out(const SmoothArgs& A, const TileCtx<TILE>& c, V3<d
                     }
                 }
                 if (got)
-                    out = nl >= 0 ? !in_envelope_masked(c.lst, c.tri, nl, c.fmask + 2 * f, smp,
-                                                        A.eps2, A.sqrtT, hl)
+                    out = nl >= 0 ? !in_envelope_masked<false, true>(c.lst, c.tri, nl, c.fmask + 2 * f,
+                                                                     smp, A.eps2, A.sqrtT, hl)
                                   : !in_envelope<false, false>(W, smp, A.eps2, A.T, A.sqrtT, hint, st);
             }
             if (tile_any(c, out)) return true;

@@ -681,8 +681,11 @@ __device__ __forceinline__ void face_list_mask(L lst, int nl, V3<double> a, V3<d
 // first, then the set positions' FP32 box tests, then the triangles of the
 // boxes that pass, in ascending position as in_envelope_list.  Every facet
 // that could hold p is at a set position, so the decision and the facet left
-// in hl are in_envelope_list's.
-template <bool NB = false, class M>
+// in hl are in_envelope_list's.  BATCH: the set positions' boxes are read
+// four at a time before their tests (one LDS round trip per four; the
+// smoothing kernels gain from it, the envelope list kernel, short of
+// registers, loses).
+template <bool NB = false, bool BATCH = false, class M>
 __device__ __forceinline__ bool in_envelope_masked(LdsNode32W* lst, LdsF64c* tri, int nl, M mask,
                                                    V3<double> p, double eps2, double sqrtT, int& hl)
 {
@@ -712,11 +715,34 @@ __device__ __forceinline__ bool in_envelope_masked(LdsNode32W* lst, LdsF64c* tri
         for (int k = 0; k < 3; ++k)
             if (nb[k] >= 64 * w && nb[k] < 64 * w + 64) mm &= ~(1ull << (nb[k] - 64 * w));
         uint64_t m = 0;
-        while (mm) {
+        while (!BATCH && mm) {
             const int b = __builtin_ctzll(mm);
             mm &= mm - 1;
             const BvhNode32 e = from_lds(lst + 64 * w + b);
             if (box_d2(pf, e.lo, e.hi) <= Tf) m |= 1ull << b;
+        }
+        while (BATCH && mm) {
+            // four set positions at a time, their boxes read before their
+            // tests (past the last set bit, the first is read again)
+            int bq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                bq[u] = mm ? __builtin_ctzll(mm) : -1;
+                mm &= mm - 1;
+            }
+            float lo[4][3], hi[4][3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const LdsNode32W* e = lst + 64 * w + (bq[u] >= 0 ? bq[u] : bq[0]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lo[u][k] = e->lo[k];
+                    hi[u][k] = e->hi[k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (bq[u] >= 0 && box_d2(pf, lo[u], hi[u]) <= Tf) m |= 1ull << bq[u];
         }
         while (m) {
             const int i = 64 * w + __builtin_ctzll(m);
