@@ -218,9 +218,100 @@ __device__ __forceinline__ void load_ring_tet(const SmoothArgs& A, int4 t, int v
     }
 }
 
+// id slot i (0..3) of t, by selects (no run-time array index)
+__device__ __forceinline__ int pick4(int4 t, int i)
+{
+    return i == 0 ? t.x : i == 1 ? t.y : i == 2 ? t.z : t.w;
+}
+
+// The tile's LDS copy of the ring's leading tets for the sweeps of one
+// problem, after the slice's reduction rows: kRingEntry doubles per tet -- the
+// three vertices other than v, from v's slot on (cyclically), and v's slot
+// (-1: the tet is read from global memory; v is not in it exactly once).
+// Lane l writes and reads only the entries of its own tets j = l, l + TILE, ...
+// The copy lives until the slice is reused (a surface problem's projection).
+constexpr int kRingEntry = 10;
+template <int TILE>
+constexpr int ring_cache_cap()
+{
+    return (tile_bytes<TILE>() - TILE * kQ * 8) / (kRingEntry * 8);
+}
+static_assert(ring_cache_cap<16>() >= 32 && ring_cache_cap<64>() >= 128, "ring copy capacity");
+
+struct RingSrc {
+    const int4* rt;  // the ring in global memory
+    int cached;      // tets [0, cached) have entries in the tile's copy
+};
+
+template <int TILE>
+__device__ __forceinline__ LdsF64* ring_cache(const TileCtx<TILE>& c)
+{
+    // the slice's start (the sampler area follows the list and the masks),
+    // then the reduction rows
+    return c.frec - (ring_cap<TILE>() * kListEntryBytes + face_chunk<TILE>() * kFaceMaskBytes) / 8 + TILE * kQ;
+}
+
+// checks the ids of the lane's ring tets and writes the entries of those
+// among the first ring_cache_cap; true when some id is out of range
+template <int TILE>
+__device__ __forceinline__ bool ring_fill(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+                                          int v)
+{
+    bool bid = false;
+    for (int j = c.tl; j < m; j += TILE) {
+        const int4 t = rt[j];
+        const bool tb = t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 || t.z >= A.nv || t.w < 0 ||
+                        t.w >= A.nv;
+        bid = bid || tb;
+        if (j < ring_cache_cap<TILE>()) {
+            const int nvv = (t.x == v) + (t.y == v) + (t.z == v) + (t.w == v);
+            const int iv = tb || nvv != 1 ? -1 : t.x == v ? 0 : t.y == v ? 1 : t.z == v ? 2 : 3;
+            LdsF64* e = ring_cache(c) + kRingEntry * j;
+            if (iv >= 0) {
+#pragma unroll
+                for (int k = 1; k < 4; ++k) {
+                    const V3<double> q = vtx(A, pick4(t, (iv + k) & 3));
+                    e[3 * (k - 1)] = q.x;
+                    e[3 * (k - 1) + 1] = q.y;
+                    e[3 * (k - 1) + 2] = q.z;
+                }
+            }
+            e[9] = (double)iv;
+        }
+    }
+    return bid;
+}
+
+// ring tet j with v at p, as load_ring_tet, from the tile's copy when it
+// holds the tet
+template <int TILE>
+__device__ __forceinline__ void ring_tet(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R, int j,
+                                         int v, V3<double> p, bool rot, Tet& T)
+{
+    if (j < R.cached) {
+        LdsF64* e = ring_cache(c) + kRingEntry * j;
+        const int iv = (int)e[9];
+        if (iv >= 0) {
+            double w[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) w[k] = e[k];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                // slot jj holds v (k = 0) or the k-th vertex after v
+                const int k = rot ? jj : (jj - iv) & 3;
+                T.p[jj][0] = k == 0 ? p.x : k == 1 ? w[0] : k == 2 ? w[3] : w[6];
+                T.p[jj][1] = k == 0 ? p.y : k == 1 ? w[1] : k == 2 ? w[4] : w[7];
+                T.p[jj][2] = k == 0 ? p.z : k == 1 ? w[2] : k == 2 ? w[5] : w[8];
+            }
+            return;
+        }
+    }
+    load_ring_tet(A, R.rt[j], v, p, rot, T);
+}
+
 // NewtonsUpdate: sums of E, J, H over the ring (rotated tets) in ring order
 template <int TILE>
-__device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+__device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R, int m,
                             int v, V3<double> pos, double& E, double J[3], double H[9])
 {
     double acc = 0.0;
@@ -231,7 +322,7 @@ __device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const i
         for (int q = 0; q < kQ; ++q) val[q] = 0.0;
         if (j < m) {
             Tet T;
-            load_ring_tet(A, rt[j], v, pos, true, T);
+            ring_tet<TILE>(A, c, R, j, v, pos, true, T);
             AmipsCore k = amips_core(T);
             double g[3], u[3], al[3], be[3], h[9];
             amips_grad(T, k, g, u, al, be);
@@ -269,7 +360,7 @@ __device__ bool ring_update(const SmoothArgs& A, const TileCtx<TILE>& c, const i
 
 // getNewEnergy with v at X: sum in ring order, clamped to MAX_ENERGY
 template <int TILE>
-__device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+__device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R, int m,
                               int v, V3<double> X)
 {
     double acc = 0.0;
@@ -278,7 +369,7 @@ __device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const
         double e = 0.0;
         if (j < m) {
             Tet T;
-            load_ring_tet(A, rt[j], v, X, false, T);
+            ring_tet<TILE>(A, c, R, j, v, X, false, T);
             e = amips_energy(T);
         }
         c.red[c.tl * kQ] = e;
@@ -296,7 +387,7 @@ __device__ double ring_energy(const SmoothArgs& A, const TileCtx<TILE>& c, const
 // to the FP64 sum (per-tet bound from amips_core32 plus the summation error of
 // both sums).  ok = false when any tet's FP32 value is unusable.
 template <int TILE>
-__device__ void ring_energy32(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+__device__ void ring_energy32(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R, int m,
                               int v, V3<double> X, double& sum, double& bound, bool& ok)
 {
     double s = 0.0, b = 0.0;
@@ -305,7 +396,7 @@ __device__ void ring_energy32(const SmoothArgs& A, const TileCtx<TILE>& c, const
         int j = base + c.tl;
         if (j < m) {
             Tet T;
-            load_ring_tet(A, rt[j], v, X, false, T);
+            ring_tet<TILE>(A, c, R, j, v, X, false, T);
             float d[3][3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -335,7 +426,7 @@ __device__ void ring_energy32(const SmoothArgs& A, const TileCtx<TILE>& c, const
 }
 
 template <int TILE>
-__device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+__device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R, int m,
                           int v, V3<double> X)
 {
     bool f = false;
@@ -343,7 +434,7 @@ __device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const int
         int j = s + c.tl;
         if (j < m && !f) {
             Tet T;
-            load_ring_tet(A, rt[j], v, X, false, T);
+            ring_tet<TILE>(A, c, R, j, v, X, false, T);
             f = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
         }
     }
@@ -352,7 +443,7 @@ __device__ bool ring_flip(const SmoothArgs& A, const TileCtx<TILE>& c, const int
 
 // ring_flip over the ring tets j with skip[j] == 0 (skip may be NULL)
 template <int TILE>
-__device__ bool ring_flip_masked(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt,
+__device__ bool ring_flip_masked(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R,
                                  int m, int v, V3<double> X, const uint8_t* skip)
 {
     bool f = false;
@@ -360,7 +451,7 @@ __device__ bool ring_flip_masked(const SmoothArgs& A, const TileCtx<TILE>& c, co
         int j = s + c.tl;
         if (j < m && !f && !(skip && skip[j])) {
             Tet T;
-            load_ring_tet(A, rt[j], v, X, false, T);
+            ring_tet<TILE>(A, c, R, j, v, X, false, T);
             f = orient3d(T.p[0], T.p[1], T.p[2], T.p[3]) != 1;
         }
     }
@@ -492,14 +583,14 @@ __device__ __forceinline__ void solve3_qrcp(const double* Ain, const double* b, 
 
 // NewtonsMethod; returns is_moved, pout = the final position
 template <int TILE, bool MIXED>
-__device__ bool newtons_method(const SmoothArgs& A, const TileCtx<TILE>& c, const int4* rt, int m,
+__device__ bool newtons_method(const SmoothArgs& A, const TileCtx<TILE>& c, const RingSrc& R, int m,
                                int v, V3<double> p0, V3<double>& pout)
 {
     V3<double> pos = p0;
     bool is_moved = false;
     for (int step = 0; step < 15; ++step) {
         double old_energy, J[3], H[9];
-        if (!ring_update<TILE>(A, c, rt, m, v, pos, old_energy, J, H)) break;
+        if (!ring_update<TILE>(A, c, R, m, v, pos, old_energy, J, H)) break;
         V3<double> X0 = pos;
         double a = 1.0;
         bool taken = false;
@@ -515,7 +606,7 @@ __device__ bool newtons_method(const SmoothArgs& A, const TileCtx<TILE>& c, cons
                 continue;
             }
             V3<double> X = mk3(Xs[0], Xs[1], Xs[2]);
-            if (ring_flip<TILE>(A, c, rt, m, v, X)) {
+            if (ring_flip<TILE>(A, c, R, m, v, X)) {
                 a /= 2.0;
                 continue;
             }
@@ -523,7 +614,7 @@ __device__ bool newtons_method(const SmoothArgs& A, const TileCtx<TILE>& c, cons
             if (MIXED) {
                 double s32, b32;
                 bool ok;
-                ring_energy32<TILE>(A, c, rt, m, v, X, s32, b32, ok);
+                ring_energy32<TILE>(A, c, R, m, v, X, s32, b32, ok);
                 // the FP64 sum lies in [s32 - b32, s32 + b32] and is finite
                 // below MAX_ENERGY when ok; decide what the interval allows
                 if (ok && s32 - b32 > 0 && s32 + b32 < kMaxEnergy) {
@@ -538,7 +629,7 @@ __device__ bool newtons_method(const SmoothArgs& A, const TileCtx<TILE>& c, cons
                 }
             }
             if (!decided) {
-                double new_energy = ring_energy<TILE>(A, c, rt, m, v, X);
+                double new_energy = ring_energy<TILE>(A, c, R, m, v, X);
                 reject = new_energy >= old_energy || is_inf(new_energy) || is_nan(new_energy);
                 tiny = !reject && fabs_(new_energy - old_energy) < 1e-5;
             }
@@ -913,12 +1004,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
     const int m = bad ? 0 : (int)(r1 - r0);
     const int4* rt = A.ring + r0;
     if (!bad) {
-        bool bid = false;
-        for (int j = c.tl; j < m; j += TILE) {
-            int4 t = rt[j];
-            bid = bid || t.x < 0 || t.x >= A.nv || t.y < 0 || t.y >= A.nv || t.z < 0 ||
-                  t.z >= A.nv || t.w < 0 || t.w >= A.nv;
-        }
+        bool bid = ring_fill<TILE>(A, c, rt, m, v);
         for (int64_t k = t0 + c.tl; k < t1; k += TILE) {
             int a = A.tri_ab[2 * k], b = A.tri_ab[2 * k + 1];
             bid = bid || a < 0 || a >= A.nv || b < 0 || b >= A.nv;
@@ -926,6 +1012,9 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
         bad = tile_any(c, bid);
     }
     V3<double> p0 = bad ? mk3(0.0, 0.0, 0.0) : vtx(A, v);
+    // the sweeps up to a surface problem's projection read the tile's copy
+    const RingSrc R = {rt, bad ? 0 : (m < ring_cache_cap<TILE>() ? m : ring_cache_cap<TILE>())};
+    const RingSrc Rg = {rt, 0};
     uint8_t st = TWG_SMOOTH_MOVED;
     V3<double> pfin = p0;
     if (bad) {
@@ -933,17 +1022,17 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
     } else if (kind == 3) {
         // NewtonsMethod alone: is_moved and the loop's final position
         V3<double> pn;
-        st = newtons_method<TILE, MIXED>(A, c, rt, m, v, p0, pn) ? TWG_SMOOTH_MOVED
+        st = newtons_method<TILE, MIXED>(A, c, R, m, v, p0, pn) ? TWG_SMOOTH_MOVED
                                                                  : TWG_SMOOTH_NOT_MOVED;
         pfin = pn;
-    } else if (ring_flip<TILE>(A, c, rt, m, v, p0)) {
+    } else if (ring_flip<TILE>(A, c, R, m, v, p0)) {
         st = TWG_SMOOTH_INVALID;
     } else {
         V3<double> pn;
-        if (!newtons_method<TILE, MIXED>(A, c, rt, m, v, p0, pn)) {
+        if (!newtons_method<TILE, MIXED>(A, c, R, m, v, p0, pn)) {
             st = TWG_SMOOTH_NOT_MOVED;
         } else if (kind == 0) {
-            if (ring_flip<TILE>(A, c, rt, m, v, pn)) st = TWG_SMOOTH_FLIP_END;
+            if (ring_flip<TILE>(A, c, R, m, v, pn)) st = TWG_SMOOTH_FLIP_END;
             else pfin = pn;
         } else {
             V3<double> pf;
@@ -960,7 +1049,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
                 hint = nearest_facet_group<TILE>(c.sfw, pn, pf, sq, c.lst, c.fr, ring_cap<TILE>(),
                                                  c.tl, c.wbase);
             }
-            if (ring_flip<TILE>(A, c, rt, m, v, pf)) {
+            if (ring_flip<TILE>(A, c, Rg, m, v, pf)) {
                 st = TWG_SMOOTH_REJECT_FLIP;
             } else {
                 double om = 0.0, nm = 0.0;
@@ -1101,7 +1190,8 @@ __global__ __launch_bounds__(64) void newton_step_kernel(SmoothArgs A, const int
         return;
     }
     double E, J[3], H[9];
-    const bool ok = ring_update<64>(A, c, rt, m, v, p0, E, J, H);
+    const RingSrc R = {rt, 0};
+    const bool ok = ring_update<64>(A, c, R, m, v, p0, E, J, H);
     if (c.tl == 0) {
         out->ok = ok ? 1 : 0;
         out->energy = E;
@@ -1121,8 +1211,8 @@ __global__ __launch_bounds__(64) void newton_step_kernel(SmoothArgs A, const int
         double en = 0.0;
         if (fin) {
             const V3<double> X = mk3(Xs[0], Xs[1], Xs[2]);
-            flip = ring_flip_masked<64>(A, c, rt, m, v, X, skip);
-            en = ring_energy<64>(A, c, rt, m, v, X);
+            flip = ring_flip_masked<64>(A, c, R, m, v, X, skip);
+            en = ring_energy<64>(A, c, R, m, v, X);
         }
         if (c.tl == 0) {
             for (int k = 0; k < 3; ++k) out->X[it][k] = Xs[k];
