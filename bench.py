@@ -589,7 +589,7 @@ def bench_smoothing(ctx, args, result, surf_cache):
         wall, kern = events_time(plan.run, max(2, args.steps // 4), 1)
         split = tile == C.TWG_SMOOTH_TILE_SPLIT
         sec[f"problems_per_s_{name}"] = n / wall
-        sec[f"tile_{name}"] = "split: 64 lanes per surface problem, 16 per other" if split else tile
+        sec[f"tile_{name}"] = "split: 64 lanes per surface problem, 32 per other" if split else tile
         sec[f"chunk_{name}"] = chunk
         sec[f"graph_ms_{name}"] = kern * 1e3
         if prec == 0:

@@ -231,7 +231,7 @@ typedef struct twg_smooth_desc {
  * src/tetwild/VertexSmoother.cpp:114-702).  Results are bit-identical to the
  * CPU restatement for any tile (16, 32, 64 lanes per problem, or
  * TWG_SMOOTH_TILE_SPLIT: 64 lanes for each problem of kind 1 or 2, which
- * the launch takes first, and 16 for the others; 0 = tuned or default) and
+ * the launch takes first, and 32 for the others; 0 = tuned or default) and
  * chunk (problems per queue pop; 0 = tuned or default).
  * precision TWG_MIXED screens line-search energies in FP32; results are
  * identical to TWG_FP64. */
