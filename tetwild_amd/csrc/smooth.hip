@@ -229,7 +229,8 @@ __device__ __forceinline__ int pick4(int4 t, int i)
 // three vertices other than v, from v's slot on (cyclically), and v's slot
 // (-1: the tet is read from global memory; v is not in it exactly once).
 // Lane l writes and reads only the entries of its own tets j = l, l + TILE, ...
-// The copy lives until the slice is reused (a surface problem's projection).
+// The copy lives until the face test's facet list overwrites it (a
+// projection's list stays below it).
 constexpr int kRingEntry = 10;
 template <int TILE>
 constexpr int ring_cache_cap()
@@ -237,6 +238,15 @@ constexpr int ring_cache_cap()
     return (tile_bytes<TILE>() - TILE * kQ * 8) / (kRingEntry * 8);
 }
 static_assert(ring_cache_cap<16>() >= 32 && ring_cache_cap<64>() >= 128, "ring copy capacity");
+// a projection's facet list (records and work lists, collect_facets) stays
+// below the reduction rows, so the copy outlives the projection
+template <int TILE>
+constexpr bool proj_list_below_rows()
+{
+    return ring_cap<TILE>() * (int)(sizeof(BvhNode32) + 2 * sizeof(uint32_t)) <= TILE * kQ * 8;
+}
+static_assert(proj_list_below_rows<16>() && proj_list_below_rows<32>() && proj_list_below_rows<64>(),
+              "projection list inside the reduction rows");
 
 struct RingSrc {
     const int4* rt;  // the ring in global memory
@@ -1012,9 +1022,8 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
         bad = tile_any(c, bid);
     }
     V3<double> p0 = bad ? mk3(0.0, 0.0, 0.0) : vtx(A, v);
-    // the sweeps up to a surface problem's projection read the tile's copy
+    // the sweeps up to a surface problem's face test read the tile's copy
     const RingSrc R = {rt, bad ? 0 : (m < ring_cache_cap<TILE>() ? m : ring_cache_cap<TILE>())};
-    const RingSrc Rg = {rt, 0};
     uint8_t st = TWG_SMOOTH_MOVED;
     V3<double> pfin = p0;
     if (bad) {
@@ -1049,7 +1058,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
                 hint = nearest_facet_group<TILE>(c.sfw, pn, pf, sq, c.lst, c.fr, ring_cap<TILE>(),
                                                  c.tl, c.wbase);
             }
-            if (ring_flip<TILE>(A, c, Rg, m, v, pf)) {
+            if (ring_flip<TILE>(A, c, R, m, v, pf)) {
                 st = TWG_SMOOTH_REJECT_FLIP;
             } else {
                 double om = 0.0, nm = 0.0;
@@ -1057,7 +1066,7 @@ __device__ void solve_problem(const SmoothArgs& A, const TileCtx<TILE>& c, int64
                     double o = A.old_q[r0 + j];
                     om = o > om ? o : om;
                     Tet T;
-                    load_ring_tet(A, rt[j], v, pf, false, T);
+                    ring_tet<TILE>(A, c, R, j, v, pf, false, T);
                     double q = quality_at(T);
                     nm = q > nm ? q : nm;
                 }
