@@ -150,6 +150,117 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (STATS) flush_stats(st, stats);
 }
 
+// face_out_listed for the wave's 64 triangles at once (a list of nl <= 128
+// facets, nl >= 0; act: the lane has a triangle, t9 its corners).  Every lane
+// tests its triangle's first sample; most triangles outside the envelope stop
+// there.  The other samples of the triangles still open are then spread over
+// the wave: the concatenation of their sequences (the first samples left
+// out) is cut into 64 consecutive blocks of K = ceil(total / 64) and lane r
+// walks block r, moving to the next open triangle when one ends.  A lane
+// takes the triangle it moves to -- corners, list mask, the facet that held
+// its first sample -- from the triangle's own lane (shuffles, every lane
+// active), and a sample found outside is passed to that lane by ballot.  The
+// decision -- some sample has no facet within eps -- does not depend on which
+// lane tests which sample, so it is face_out_listed's.  Returns the lane's
+// own triangle's decision.
+__device__ bool faces_wave_listed(bool act, const double* t9, double eps2, double d, double sqrtT,
+                                  LdsNode32W* lst, LdsF64c* tri, int nl, int lane)
+{
+    double tv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tv[k] = act ? t9[k] : 0.0;
+    bool out = false;
+    int64_t rem = 0;  // samples after the first, while the triangle is open
+    uint64_t mk[2] = {0ull, 0ull};
+    int hl = -1;
+    if (act) {
+        const V3<double> a = mk3(tv[0], tv[1], tv[2]), b = mk3(tv[3], tv[4], tv[5]), c = mk3(tv[6], tv[7], tv[8]);
+        const double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
+        if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) {
+            out = true;
+        } else {
+            face_list_mask(lst, nl, a, b, c, sqrtT, mk);
+            SampleGen g;
+            g.init(a, b, c, d);
+            V3<double> s;
+            g.next(s);  // every sequence has a first sample
+            if (!in_envelope_masked<true>(lst, tri, nl, mk, s, eps2, sqrtT, hl)) out = true;
+            else rem = g.skip(INT64_MAX);
+        }
+    }
+    const unsigned long long open = __ballot(rem > 0);
+    if (open == 0ull) return out;
+    int64_t incl = rem;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    const int64_t total = __shfl(incl, 63, 64);
+    const int64_t K = (total + 63) / 64;
+    const int64_t first = (int64_t)lane * K;
+    // the triangle holding remaining sample `first`: the first lane f with
+    // incl[f] > first (a binary search, each lane its own)
+    int f = 0;
+    for (int step = 32; step > 0; step >>= 1) {
+        const int probe = f + step - 1;
+        const int64_t v = __shfl(incl, probe < 63 ? probe : 63, 64);
+        if (probe < 63 && v <= first) f = probe + 1;
+    }
+    if (first >= total) f = 64;  // no samples for this lane
+    const int fs = f < 64 ? f : lane;
+    const int64_t ofs = first - (__shfl(incl, fs, 64) - __shfl(rem, fs, 64));
+    // the lane's triangle: corners, mask and hint from its owner
+    SampleGen g2;
+    uint64_t m2[2];
+    int h2;
+    {
+        double cv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) cv[k] = __shfl(tv[k], fs, 64);
+        m2[0] = __shfl(mk[0], fs, 64);
+        m2[1] = __shfl(mk[1], fs, 64);
+        h2 = __shfl(hl, fs, 64);
+        if (f < 64) {
+            g2.init(mk3(cv[0], cv[1], cv[2]), mk3(cv[3], cv[4], cv[5]), mk3(cv[6], cv[7], cv[8]), d);
+            g2.skip(1 + ofs);
+        }
+    }
+    for (int64_t j = 0; j < K; ++j) {
+        const bool have = f < 64 && first + j < total;
+        V3<double> s;
+        bool got = have && g2.next(s);
+        const bool need = have && !got;
+        if (__ballot(need)) {
+            // the next open triangle after f (one exists: first + j < total)
+            const unsigned long long later = f >= 63 ? 0ull : open & ~((2ull << f) - 1ull);
+            const int nf = need && later ? __ffsll((long long)later) - 1 : lane;
+            double cv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) cv[k] = __shfl(tv[k], nf, 64);
+            const uint64_t n0 = __shfl(mk[0], nf, 64), n1 = __shfl(mk[1], nf, 64);
+            const int nh = __shfl(hl, nf, 64);
+            if (need && later) {
+                f = nf;
+                m2[0] = n0;
+                m2[1] = n1;
+                h2 = nh;
+                g2.init(mk3(cv[0], cv[1], cv[2]), mk3(cv[3], cv[4], cv[5]), mk3(cv[6], cv[7], cv[8]), d);
+                g2.skip(1);
+                got = g2.next(s);
+            }
+        }
+        const bool o2 = got && !in_envelope_masked<true>(lst, tri, nl, m2, s, eps2, sqrtT, h2);
+        // the triangles found outside, to their lanes
+        unsigned long long ob = __ballot(o2);
+        while (ob) {
+            const int l = __ffsll((long long)ob) - 1;
+            ob &= ob - 1;
+            if (__shfl(f, l, 64) == lane) out = true;
+        }
+    }
+    return out;
+}
+
 // One triangle per lane, as faces_kernel, for batches walked in Morton
 // order: the wave's 64 consecutive triangles lie close together, so the wave
 // lists once (collect_facets, LDS) the facets whose boxes meet the box of its
@@ -222,7 +333,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         for (int k = 0; k < 9; ++k) tri[9 * e + k] = f[k];
     }
     list_neighbours<64>(W, lst, nl, lane);
-    if (act) out[i] = face_out_listed<F32>(W, t, eps2, d, T, sqrtT, lst, tri, nl) ? 1 : 0;
+    static_assert(CAP <= 128, "two list-mask words");
+    if (nl < 0) {
+        if (act) out[i] = face_out_listed<F32>(W, t, eps2, d, T, sqrtT, lst, tri, nl) ? 1 : 0;
+        return;
+    }
+    const bool o = faces_wave_listed(act, t, eps2, d, sqrtT, lst, tri, nl, lane);
+    if (act) out[i] = o ? 1 : 0;
 }
 
 // facets listed per wave in faces_group_kernel<F32, 64>
