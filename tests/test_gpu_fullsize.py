@@ -6,7 +6,7 @@
 * envelope points: 2M samples in a +-1 % band around eps, every decision;
 * smoothing: the 200k one-ring batch (20 % surface) in full -- status and
   final position bit-exact, new ring qualities bit-exact for moved surface
-  vertices, FP64 and mixed, 64-lane tiles and the 64/16 split.
+  vertices, FP64 and mixed, 64-lane tiles and the 64/32 split.
 
 Bar: identical decisions and bit-identical positions / qualities."""
 import numpy as np

@@ -29,6 +29,37 @@ def bits(a):
     return np.ascontiguousarray(a).view(np.int64)
 
 
+# row-table entries of a 64-lane tile in the face test (seg_rows<64>): a chunk
+# of up to 16 ring faces whose rows fit is sampled segment by segment from the
+# table, a chunk whose rows do not by the stored-record sampler
+ROW_TABLE = 94
+
+
+def ring_rows(B, d, chunk=16):
+    """Rows of each surface problem's first chunk of ring faces, the centre at
+    its input position: per face SampleGen's row count M, the height over the
+    longest edge in rows of sqrt(3)/2 d (none for faces of at most one sampling
+    distance).  The kernel's faces take the projected centre, so this is an
+    estimate of which sampler a ring's face test takes."""
+    X = np.stack([B["x"], B["y"], B["z"]], 1)
+    to, ab, c = B["tri_off"], B["tri_ab"].reshape(-1, 2), B["center"]
+    rows = []
+    for p in np.nonzero(B["kind"] > 0)[0]:
+        k = np.arange(to[p], min(to[p + 1], to[p] + chunk))
+        T = np.stack([np.repeat(X[c[p]][None], len(k), 0), X[ab[k, 0]], X[ab[k, 1]]], 1)
+        e = np.stack([T[:, 1] - T[:, 0], T[:, 2] - T[:, 1], T[:, 0] - T[:, 2]], 1)
+        l2 = (e ** 2).sum(2)
+        i = 2 - np.argmax(l2[:, ::-1], 1)  # the last longest edge
+        r = np.arange(len(k))
+        v0, v1, v2 = T[r, i], T[r, (i + 1) % 3], T[r, (i + 2) % 3]
+        e01 = v1 - v0
+        foot = e01 * ((v2 - v0) * e01).sum(1, keepdims=True) / l2[r, i][:, None] + v0
+        m = np.floor(np.linalg.norm(foot - v2, axis=1) / (np.sqrt(3) / 2 * d)).astype(int)
+        m[np.sqrt(l2[r, i]) / d <= 1] = 0
+        rows.append(int(m.sum()))
+    return np.array(rows)
+
+
 def check(batch, ref, B):
     p, s, q = (t.cpu().numpy() for t in (batch.p_out, batch.status, batch.new_q))
     rp, rs, rq = ref
@@ -79,7 +110,35 @@ def test_surface_batch(ctx, surf_data):
     st = ref[1]
     print("statuses", np.bincount(st, minlength=8))
     assert ((st == 0) & (B["kind"] == 1)).any() and ((st == 0) & (B["kind"] == 2)).any()
+    # the 64-lane tiles sampled these rings from the row table
+    assert (ring_rows(B, D) <= ROW_TABLE - 20).mean() > 0.95
     del S
+
+
+def test_surface_batch_dense_samples(ctx, surf_data):
+    """A sampling distance a quarter of the other tests': the rings' rows
+    overflow the 64-lane tile's row table, so its face test takes the
+    stored-record sampler; eps small enough that some rings leave the
+    envelope."""
+    V, F, _, _ = surf_data
+    sf = ctx.surface(V, F)
+    d, eps2 = 2.5e-4, 9e-8
+    B = W.smoothing_batch(160, seed=15, surface_frac=1.0, V=V, F=F)
+    assert (ring_rows(B, d) > ROW_TABLE + 20).mean() > 0.9
+    q = O.tet_qualities(B["ring_tets"], B["x"], B["y"], B["z"])
+    S = O.Surface(V, F[sf.order()])
+    ref = O.smooth_batch(B, q, S, S, eps2, d)
+    st = ref[1]
+    print("statuses", np.bincount(st, minlength=8))
+    assert (st == 0).sum() > 50 and (st == C.TWG_SMOOTH_STATUS.index("REJECT_ENVELOPE")).sum() > 3
+    batch = SmoothBatch(ctx, B, sf=sf, bt=sf, eps2=eps2, sampling_dist=d, old_q=q)
+    for tile, prec in ((64, 0), (64, 1), (C.TWG_SMOOTH_TILE_SPLIT, 0), (32, 0)):
+        print(f"tile={tile} precision={prec}", flush=True)
+        batch.status.fill_(255)
+        batch.new_q.zero_()
+        batch.run(precision=prec, tile=tile, chunk=2)
+        torch.cuda.synchronize()
+        check(batch, ref, B)
 
 
 def test_edge_statuses(ctx, surf_data):
