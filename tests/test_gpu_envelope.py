@@ -173,6 +173,32 @@ def test_dense_faces_wave_lists(ctx, dev):
         assert listed + walked == (n + 255) // 256 * 4 and listed > 0.9 * (listed + walked)
 
 
+def test_dense_faces_mixed_sizes(ctx, dev):
+    """faces_list_kernel's wave walk with uneven waves: every 97th triangle
+    exactly collinear (left to the exact pass: a lane with no triangle that
+    still walks others' samples) and every 89th scaled five times about its
+    centroid (~25x the samples of its neighbours, spread over the wave);
+    decisions vs the oracle"""
+    V, F = W.bumpy_sphere(freq=16, seed=0)
+    eps = 1e-3
+    n = 70000
+    surf = ctx.surface(V, F)
+    Q = W.query_triangles(V, F, n, eps=eps, edge=0.002, seed=23).reshape(n, 3, 3)
+    c = Q.mean(axis=1, keepdims=True)
+    Q[::89] = c[::89] + 5.0 * (Q[::89] - c[::89])
+    Q[::97, 2] = 0.5 * (Q[::97, 0] + Q[::97, 1])
+    Q = np.ascontiguousarray(Q.reshape(n, 9))
+    ref, _ = O.Surface(V, F).face_out(Q, eps * eps, eps)
+    assert 0.2 < ref.mean() < 0.8 and 0.0 < ref[::89].mean() < 1.0
+    for precision in (0, 1):
+        ctx.list_stats(True)
+        out = ctx.envelope_faces(surf, dev_t(dev, Q), eps * eps, eps, precision=precision).cpu().numpy()
+        listed, walked = ctx.list_stats(False)
+        print(f"precision {precision}: {listed} waves listed, {walked} walked", flush=True)
+        assert listed > 0.9 * (listed + walked)
+        assert np.array_equal(out, ref), (precision, np.nonzero(out != ref)[0][:10])
+
+
 def test_dense_points_wave_lists(ctx, dev):
     """A Morton-ordered batch with >= 8 points per facet runs points_list_kernel
     (a wave's facets listed in LDS): decisions vs the oracle and vs the
